@@ -1,0 +1,98 @@
+"""ctypes binding of the C ABI declared in include/difftetvr_amd.h.
+
+The shared library is built in-tree by ``difftetvr_amd/build.py`` (``__graft_entry__.build()``).  There is no
+fallback: if the library is missing or no GPU is visible, every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdifftetvr_amd.so")
+
+DTV_OK = 0
+DTV_ADJOINT_EXACT = 0
+DTV_ADJOINT_REFERENCE = 1
+STAGES = ["count_pairs", "scan", "emit", "sort", "ranges", "forward", "adjoint"]
+
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+
+
+class DtvMesh(ctypes.Structure):
+    _fields_ = [("cells", c_void_p), ("positions", c_void_p), ("colors", c_void_p),
+                ("num_cells", c_int64), ("num_vertices", c_int64)]
+
+
+class DtvCamera(ctypes.Structure):
+    _fields_ = [("view", c_float * 16), ("fovy", c_float), ("znear", c_float), ("zfar", c_float),
+                ("width", c_int32), ("height", c_int32)]
+
+
+class DtvRenderParams(ctypes.Structure):
+    _fields_ = [("attenuation", c_float), ("clear_color", c_float * 4), ("use_early_ray_termination", c_int32),
+                ("early_ray_out_alpha", c_float), ("tile_size", c_int32), ("checkpoint_interval", c_int32)]
+
+
+class DtvViewStats(ctypes.Structure):
+    _fields_ = [("num_triangles", c_int64), ("num_pairs", c_int64), ("num_tiles", c_int64),
+                ("sort_passes", c_int32), ("width", c_int32), ("height", c_int32)]
+
+
+class DtvAdamGroup(ctypes.Structure):
+    _fields_ = [("param", c_void_p), ("grad", c_void_p), ("exp_avg", c_void_p), ("exp_avg_sq", c_void_p),
+                ("numel", c_int64), ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
+                ("step", c_int64), ("clamp_nonneg", c_int32), ("zero_grad", c_int32)]
+
+
+# (name, restype, argtypes) of every exported entry point; tests check the library exports exactly these.
+SIGNATURES = [
+    ("dtv_get_version", ctypes.c_int, []),
+    ("dtv_last_error", ctypes.c_char_p, []),
+    ("dtv_ctx_create", ctypes.c_int, [c_int32, ctypes.POINTER(c_void_p)]),
+    ("dtv_ctx_destroy", ctypes.c_int, [c_void_p]),
+    ("dtv_view_create", ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    ("dtv_view_destroy", ctypes.c_int, [c_void_p]),
+    ("dtv_camera_uniforms", ctypes.c_int, [ctypes.POINTER(DtvCamera), ctypes.POINTER(c_float)]),
+    ("dtv_render", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), ctypes.POINTER(DtvCamera),
+                                  ctypes.POINTER(DtvRenderParams), c_void_p, c_void_p]),
+    ("dtv_render_adjoint", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_void_p, c_int32, c_int32, c_void_p,
+                                          c_void_p, c_void_p]),
+    ("dtv_view_fragment_counts", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    ("dtv_view_export_fragments", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_int32, c_void_p, c_void_p]),
+    ("dtv_view_get_stats", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvViewStats)]),
+    ("dtv_view_set_timing", ctypes.c_int, [c_void_p, c_int32]),
+    ("dtv_view_get_timings", ctypes.c_int, [c_void_p, ctypes.POINTER(c_float)]),
+    ("dtv_adam_step", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p]),
+    ("dtv_mse_loss_grad", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+]
+
+_lib = None
+
+
+class DtvError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DtvError(f"difftetvr_amd HIP library not built: {LIB_PATH} is missing "
+                       "(run __graft_entry__.build() or python difftetvr_amd/build.py)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != DTV_OK:
+        msg = load().dtv_last_error()
+        raise DtvError(f"{what}: {msg.decode() if msg else 'error'} (status {status})")

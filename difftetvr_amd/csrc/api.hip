@@ -1,0 +1,495 @@
+// C ABI of difftetvr_amd (include/difftetvr_amd.h): views, planning of the per-view binning/sort/render
+// pipeline, and the optimiser entry points.  Host C++; all device work is enqueued on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/difftetvr_amd.h"
+#include "dtv_kernels.h"
+
+using namespace dtv;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define DTV_HIP(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(DTV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer that grows geometrically and is never shrunk.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p != nullptr) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+        }
+        p = nullptr;
+        cap = 0;
+        size_t nb = bytes + bytes / 4 + 256;
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = nb;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+constexpr int kMaxCheckpoints = 24;
+
+// Camera uniforms: perspective RH, depth in [0,1], y flipped; built in double, rounded once to float.
+// Column-major 4x4 (m[c*4 + r]).
+void mul4(const double* A, const double* B, double* C) {
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++)
+            C[c * 4 + r] = ((A[0 * 4 + r] * B[c * 4 + 0] + A[1 * 4 + r] * B[c * 4 + 1]) + A[2 * 4 + r] * B[c * 4 + 2]) +
+                           A[3 * 4 + r] * B[c * 4 + 3];
+}
+
+bool inv4(const double* m, double* out) {
+    double a[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            a[r][c] = m[c * 4 + r];
+            a[r][c + 4] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 4; r++)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k < 8; k++) std::swap(a[col][k], a[piv][k]);
+        const double inv = 1.0 / a[col][col];
+        for (int k = 0; k < 8; k++) a[col][k] *= inv;
+        for (int r = 0; r < 4; r++) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f == 0.0) continue;
+            for (int k = 0; k < 8; k++) a[r][k] -= f * a[col][k];
+        }
+    }
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) out[c * 4 + r] = a[r][c + 4];
+    return true;
+}
+
+bool camera_uniforms(const dtv_camera& cam, double u[68]) {
+    double V[16], P[16] = {0}, iP[16] = {0}, VP[16], iV[16];
+    for (int i = 0; i < 16; i++) V[i] = (double)cam.view[i];
+    const double th = std::tan(0.5 * (double)cam.fovy);
+    const double aspect = (double)cam.width / (double)cam.height;
+    P[0] = 1.0 / (aspect * th);
+    P[5] = -1.0 / th;
+    P[10] = (double)cam.zfar / ((double)cam.znear - (double)cam.zfar);
+    P[11] = -1.0;
+    P[14] = -((double)cam.zfar * (double)cam.znear) / ((double)cam.zfar - (double)cam.znear);
+    iP[0] = 1.0 / P[0];
+    iP[5] = 1.0 / P[5];
+    iP[11] = 1.0 / P[14];
+    iP[14] = -1.0;
+    iP[15] = P[10] / P[14];
+    mul4(P, V, VP);
+    if (!inv4(V, iV)) return false;
+    std::memcpy(u + 0, VP, sizeof(VP));
+    std::memcpy(u + 16, V, sizeof(V));
+    std::memcpy(u + 32, iP, sizeof(iP));
+    std::memcpy(u + 48, iV, sizeof(iV));
+    u[64] = iV[12];
+    u[65] = iV[13];
+    u[66] = iV[14];
+    u[67] = 0.0;
+    return true;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+struct dtv_ctx {
+    int device = 0;
+};
+
+struct dtv_view {
+    dtv_ctx* ctx = nullptr;
+    ViewUniforms hU{};
+    ViewUniforms* pinnedU = nullptr;
+    BinStats* pinnedStats = nullptr;
+    DevBuf dU, counts, offsets, scan_status, small, keys, vals, keys2, vals2, hist, lookback, ranges;
+    DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
+    // state of the last render
+    bool rendered = false;
+    const int32_t* cells = nullptr;
+    int64_t T = 0, V = 0;
+    int64_t num_pairs = 0, num_tris = 0;
+    int sort_passes = 0;
+    bool result_in_alt = false;
+    // per-stage timers
+    bool timing = false;
+    hipEvent_t ev[DTV_NUM_STAGES][2] = {};
+    bool ev_recorded[DTV_NUM_STAGES] = {};
+    void mark(int stage, int which, hipStream_t s) {
+        if (!timing) return;
+        (void)hipEventRecord(ev[stage][which], s);
+        if (which == 1) ev_recorded[stage] = true;
+    }
+    const uint32_t* sorted_vals() const { return result_in_alt ? vals2.as<uint32_t>() : vals.as<uint32_t>(); }
+    PixelState pixel_state() const {
+        PixelState ps;
+        ps.ray_color = ray_color.as<float4>();
+        ps.frag_count = frag_count.as<uint32_t>();
+        ps.last_rec = last_rec.as<uint32_t>();
+        ps.num_ckpt = num_ckpt.as<uint32_t>();
+        ps.ckpt_color = ckpt_color.as<float4>();
+        ps.ckpt_n = ckpt_n.as<uint32_t>();
+        return ps;
+    }
+};
+
+extern "C" {
+
+int dtv_get_version(void) { return 1; }
+
+const char* dtv_last_error(void) { return g_last_error.c_str(); }
+
+int dtv_ctx_create(int32_t device, dtv_ctx** out_ctx) {
+    if (!out_ctx) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_ctx_create: out_ctx is NULL");
+    int n = 0;
+    DTV_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_ctx_create: invalid device index");
+    DTV_HIP(hipSetDevice(device));
+    dtv_ctx* c = new dtv_ctx();
+    c->device = device;
+    *out_ctx = c;
+    return DTV_OK;
+}
+
+int dtv_ctx_destroy(dtv_ctx* ctx) {
+    delete ctx;
+    return DTV_OK;
+}
+
+int dtv_view_create(dtv_ctx* ctx, dtv_view** out_view) {
+    if (!ctx || !out_view) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_create: NULL argument");
+    DTV_HIP(hipSetDevice(ctx->device));
+    dtv_view* v = new dtv_view();
+    v->ctx = ctx;
+    hipError_t e = hipHostMalloc((void**)&v->pinnedU, sizeof(ViewUniforms), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&v->pinnedStats, sizeof(BinStats) + 16, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete v;
+        return fail(DTV_ERR_HIP, std::string("dtv_view_create: hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    *out_view = v;
+    return DTV_OK;
+}
+
+int dtv_view_destroy(dtv_view* v) {
+    if (!v) return DTV_OK;
+    (void)hipSetDevice(v->ctx->device);
+    (void)hipDeviceSynchronize();
+    DevBuf* bufs[] = {&v->dU, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->keys, &v->vals, &v->keys2,
+                      &v->vals2, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
+                      &v->num_ckpt, &v->ckpt_color, &v->ckpt_n};
+    for (DevBuf* b : bufs) b->release();
+    for (int i = 0; i < DTV_NUM_STAGES; i++)
+        for (int k = 0; k < 2; k++)
+            if (v->ev[i][k]) (void)hipEventDestroy(v->ev[i][k]);
+    if (v->pinnedU) (void)hipHostFree(v->pinnedU);
+    if (v->pinnedStats) (void)hipHostFree(v->pinnedStats);
+    delete v;
+    return DTV_OK;
+}
+
+int dtv_camera_uniforms(const dtv_camera* camera, float* out68) {
+    if (!camera || !out68) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_camera_uniforms: NULL argument");
+    double u[68];
+    if (!camera_uniforms(*camera, u)) return fail(DTV_ERR_INVALID_ARGUMENT, "singular view matrix");
+    for (int i = 0; i < 68; i++) out68[i] = (float)u[i];
+    return DTV_OK;
+}
+
+int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, const dtv_render_params* params,
+               float* out_image, void* stream) {
+    if (!view || !mesh || !camera || !params || !out_image)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: NULL argument");
+    if (camera->width <= 0 || camera->height <= 0 || camera->width > 16384 || camera->height > 16384)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: viewport must be 1..16384 pixels per side");
+    if (!(camera->znear > 0.0f) || !(camera->zfar > camera->znear) || !(camera->fovy > 0.0f) ||
+        !(camera->fovy < 3.14159f))
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: invalid fovy / clip planes");
+    if (mesh->num_cells < 0 || mesh->num_vertices < 0 || mesh->num_cells > (int64_t)(1u << 29))
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: invalid mesh size");
+    if (mesh->num_cells > 0 && (!mesh->cells || !mesh->positions || !mesh->colors))
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: NULL mesh buffer");
+    if (!aligned16(mesh->cells) || !aligned16(mesh->colors) || !aligned16(out_image))
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: cells, colors and out_image must be 16-byte aligned");
+    const int tile = params->tile_size == 0 ? 8 : params->tile_size;
+    if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: tile_size must be 8");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    DTV_HIP(hipSetDevice(view->ctx->device));
+
+    double u[68];
+    if (!camera_uniforms(*camera, u)) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: singular view matrix");
+    ViewUniforms& U = view->hU;
+    for (int i = 0; i < 16; i++) {
+        U.viewProj[i] = (float)u[i];
+        U.invProj[i] = (float)u[32 + i];
+        U.invView[i] = (float)u[48 + i];
+    }
+    U.camPos[0] = (float)u[64];
+    U.camPos[1] = (float)u[65];
+    U.camPos[2] = (float)u[66];
+    U.znear = camera->znear;
+    U.width = camera->width;
+    U.height = camera->height;
+    U.tile_shift = 3;
+    U.tiles_x = (camera->width + tile - 1) / tile;
+    U.tiles_y = (camera->height + tile - 1) / tile;
+    U.attenuation = params->attenuation;
+    U.ert_alpha = params->early_ray_out_alpha;
+    U.use_ert = params->use_early_ray_termination ? 1 : 0;
+    for (int i = 0; i < 4; i++) U.clear[i] = params->clear_color[i];
+    U.ckpt_interval = params->checkpoint_interval > 0 ? params->checkpoint_interval : 64;
+    U.max_ckpt = kMaxCheckpoints;
+
+    const int64_t T = mesh->num_cells;
+    const int64_t P = (int64_t)U.width * U.height;
+    const int64_t ntiles = (int64_t)U.tiles_x * U.tiles_y;
+    DTV_HIP(view->dU.ensure(sizeof(ViewUniforms)));
+    *view->pinnedU = U;
+    DTV_HIP(hipMemcpyAsync(view->dU.p, view->pinnedU, sizeof(ViewUniforms), hipMemcpyHostToDevice, s));
+    const ViewUniforms* dU = view->dU.as<ViewUniforms>();
+
+    // 1. count (tile, face) records per face, 2. scan
+    const int64_t nslots = 4 * (T > 0 ? T : 1);
+    DTV_HIP(view->counts.ensure(sizeof(uint32_t) * nslots));
+    DTV_HIP(view->offsets.ensure(sizeof(uint32_t) * nslots));
+    DTV_HIP(view->scan_status.ensure(sizeof(uint64_t) * (scan_partitions(nslots) + 1)));
+    DTV_HIP(view->small.ensure(256));
+    BinStats* dstats = view->small.as<BinStats>();
+    uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
+    uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
+    {
+        BinStats init{};
+        init.key_and = 0xFFFFFFFFu;
+        *view->pinnedStats = init;
+        DTV_HIP(hipMemcpyAsync(dstats, view->pinnedStats, sizeof(BinStats), hipMemcpyHostToDevice, s));
+    }
+    view->mark(DTV_STAGE_COUNT_PAIRS, 0, s);
+    launch_count_pairs(dU, mesh->cells, mesh->positions, T, view->counts.as<uint32_t>(), dstats, s);
+    view->mark(DTV_STAGE_COUNT_PAIRS, 1, s);
+    view->mark(DTV_STAGE_SCAN, 0, s);
+    scan_exclusive_u32(view->counts.as<uint32_t>(), view->offsets.as<uint32_t>(), 4 * T,
+                       view->scan_status.as<uint64_t>(), scan_counter, scan_total, s);
+    view->mark(DTV_STAGE_SCAN, 1, s);
+    DTV_HIP(hipGetLastError());
+    DTV_HIP(hipMemcpyAsync(view->pinnedStats, dstats, sizeof(BinStats), hipMemcpyDeviceToHost, s));
+    DTV_HIP(hipStreamSynchronize(s));
+    const BinStats st = *view->pinnedStats;
+    if (st.total_pairs >= (1ull << 30))
+        return fail(DTV_ERR_OUT_OF_MEMORY, "dtv_render: more than 2^30 (tile, face) records");
+    const int64_t K = (int64_t)st.total_pairs;
+    view->num_pairs = K;
+    view->num_tris = st.num_tris;
+
+    // 3. emit, 4. sort, 5. tile ranges
+    DTV_HIP(view->keys.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->vals.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->keys2.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
+    view->mark(DTV_STAGE_EMIT, 0, s);
+    launch_emit_pairs(dU, mesh->cells, mesh->positions, T, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
+                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), s);
+    view->mark(DTV_STAGE_EMIT, 1, s);
+    uint32_t mask = 0;
+    const uint32_t varying = st.key_and ^ st.key_or;
+    for (int p = 0; p < 4; p++)
+        if ((varying >> (8 * p)) & 255u) mask |= 1u << p;
+    int tile_bits = 0;
+    while ((1ll << tile_bits) < ntiles) tile_bits++;
+    for (int p = 0; 8 * p < tile_bits; p++) mask |= 1u << (4 + p);
+    int npass = 0;
+    for (int p = 0; p < 8; p++) npass += (mask >> p) & 1;
+    DTV_HIP(view->hist.ensure(sizeof(uint32_t) * 8 * 256 + 64));
+    DTV_HIP(view->lookback.ensure(sizeof(uint32_t) * 256 * (size_t)(npass > 0 ? npass : 1) *
+                                  (size_t)(sort_partitions(K) + 1)));
+    view->mark(DTV_STAGE_SORT, 0, s);
+    view->sort_passes = radix_sort_pairs(view->keys.as<uint64_t>(), view->vals.as<uint32_t>(),
+                                         view->keys2.as<uint64_t>(), view->vals2.as<uint32_t>(), K, mask,
+                                         view->hist.as<uint32_t>(), view->lookback.as<uint32_t>(),
+                                         view->hist.as<uint32_t>() + 8 * 256, s);
+    view->mark(DTV_STAGE_SORT, 1, s);
+    view->result_in_alt = (view->sort_passes & 1) != 0;
+    const uint64_t* sorted_keys = view->result_in_alt ? view->keys2.as<uint64_t>() : view->keys.as<uint64_t>();
+    DTV_HIP(view->ranges.ensure(sizeof(uint2) * ntiles));
+    view->mark(DTV_STAGE_RANGES, 0, s);
+    launch_tile_ranges(sorted_keys, K, ntiles, view->ranges.as<uint2>(), s);
+    view->mark(DTV_STAGE_RANGES, 1, s);
+
+    // 6. render
+    DTV_HIP(view->ray_color.ensure(sizeof(float4) * P));
+    DTV_HIP(view->frag_count.ensure(sizeof(uint32_t) * P));
+    DTV_HIP(view->last_rec.ensure(sizeof(uint32_t) * P));
+    DTV_HIP(view->num_ckpt.ensure(sizeof(uint32_t) * P));
+    DTV_HIP(view->ckpt_color.ensure(sizeof(float4) * P * kMaxCheckpoints));
+    DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * P * kMaxCheckpoints));
+    view->mark(DTV_STAGE_FORWARD, 0, s);
+    launch_render_forward(dU, U, mesh->cells, mesh->positions, mesh->colors, view->sorted_vals(),
+                          view->ranges.as<uint2>(), out_image, view->pixel_state(), s);
+    view->mark(DTV_STAGE_FORWARD, 1, s);
+    DTV_HIP(hipGetLastError());
+    view->rendered = true;
+    view->cells = mesh->cells;
+    view->T = T;
+    view->V = mesh->num_vertices;
+    return DTV_OK;
+}
+
+int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage, int32_t use_abs_grad,
+                       int32_t adjoint_mode, float* grad_positions, float* grad_colors, void* stream) {
+    if (!view || !mesh || !dimage || !grad_positions || !grad_colors)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: NULL argument");
+    if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_render_adjoint: view has not been rendered");
+    if (mesh->cells != view->cells || mesh->num_cells != view->T || mesh->num_vertices != view->V)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: mesh differs from the rendered one");
+    if (adjoint_mode != DTV_ADJOINT_EXACT)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: unsupported adjoint_mode");
+    if (!aligned16(dimage)) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: dimage must be 16-byte aligned");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    view->mark(DTV_STAGE_ADJOINT, 0, s);
+    launch_render_adjoint(view->dU.as<ViewUniforms>(), view->hU, mesh->cells, mesh->positions, mesh->colors,
+                          view->sorted_vals(), view->ranges.as<uint2>(), dimage, view->pixel_state(),
+                          use_abs_grad ? 1 : 0, adjoint_mode, grad_positions, grad_colors, s);
+    view->mark(DTV_STAGE_ADJOINT, 1, s);
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
+int dtv_view_fragment_counts(dtv_view* view, int32_t* out_counts, void* stream) {
+    if (!view || !out_counts) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_fragment_counts: NULL argument");
+    if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_view_fragment_counts: view has not been rendered");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    const int64_t P = (int64_t)view->hU.width * view->hU.height;
+    DTV_HIP(hipMemcpyAsync(out_counts, view->frag_count.p, sizeof(uint32_t) * P, hipMemcpyDeviceToDevice, s));
+    return DTV_OK;
+}
+
+int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_per_pixel, int32_t* out_ids,
+                              void* stream) {
+    if (!view || !mesh || !out_ids || max_per_pixel <= 0)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: invalid argument");
+    if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_view_export_fragments: view has not been rendered");
+    if (mesh->cells != view->cells || mesh->num_cells != view->T)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: mesh differs from the rendered one");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    launch_export_fragments(view->dU.as<ViewUniforms>(), view->hU, mesh->cells, mesh->positions, view->sorted_vals(),
+                            view->ranges.as<uint2>(), view->pixel_state(), max_per_pixel, out_ids, s);
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
+int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out) {
+    if (!view || !out) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_get_stats: NULL argument");
+    out->num_triangles = view->num_tris;
+    out->num_pairs = view->num_pairs;
+    out->num_tiles = (int64_t)view->hU.tiles_x * view->hU.tiles_y;
+    out->sort_passes = view->sort_passes;
+    out->width = view->hU.width;
+    out->height = view->hU.height;
+    return DTV_OK;
+}
+
+int dtv_view_set_timing(dtv_view* view, int32_t enabled) {
+    if (!view) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_set_timing: NULL view");
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    if (enabled && !view->ev[0][0]) {
+        for (int i = 0; i < DTV_NUM_STAGES; i++)
+            for (int k = 0; k < 2; k++) DTV_HIP(hipEventCreate(&view->ev[i][k]));
+    }
+    view->timing = enabled != 0;
+    for (int i = 0; i < DTV_NUM_STAGES; i++) view->ev_recorded[i] = false;
+    return DTV_OK;
+}
+
+int dtv_view_get_timings(dtv_view* view, float* out_ms) {
+    if (!view || !out_ms) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_get_timings: NULL argument");
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    for (int i = 0; i < DTV_NUM_STAGES; i++) {
+        out_ms[i] = -1.0f;
+        if (!view->ev_recorded[i]) continue;
+        DTV_HIP(hipEventSynchronize(view->ev[i][1]));
+        float ms = 0.f;
+        DTV_HIP(hipEventElapsedTime(&ms, view->ev[i][0], view->ev[i][1]));
+        out_ms[i] = ms;
+    }
+    return DTV_OK;
+}
+
+int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream) {
+    if (!groups || num_groups <= 0 || num_groups > 8)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: 1..8 groups required");
+    AdamGroupDev g[8];
+    for (int i = 0; i < num_groups; i++) {
+        const dtv_adam_group& G = groups[i];
+        if (G.numel < 0 || (G.numel > 0 && (!G.param || !G.grad || !G.exp_avg || !G.exp_avg_sq)))
+            return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: invalid group");
+        if (!aligned16(G.param) || !aligned16(G.grad) || !aligned16(G.exp_avg) || !aligned16(G.exp_avg_sq))
+            return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: buffers must be 16-byte aligned");
+        if (G.step < 1) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: step must be >= 1");
+        const double bc1 = 1.0 - std::pow((double)G.beta1, (double)G.step);
+        const double bc2 = 1.0 - std::pow((double)G.beta2, (double)G.step);
+        g[i].param = G.param;
+        g[i].grad = G.grad;
+        g[i].m = G.exp_avg;
+        g[i].v = G.exp_avg_sq;
+        g[i].n = G.numel;
+        g[i].step_size = (float)((double)G.lr / bc1);
+        g[i].bc2_sqrt = (float)std::sqrt(bc2);
+        g[i].w1 = (float)(1.0 - (double)G.beta1);
+        g[i].beta2 = G.beta2;
+        g[i].w2 = (float)(1.0 - (double)G.beta2);
+        g[i].eps = G.eps;
+        g[i].clamp_nonneg = G.clamp_nonneg;
+        g[i].zero_grad = G.zero_grad;
+    }
+    launch_adam(g, num_groups, reinterpret_cast<hipStream_t>(stream));
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
+int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,
+                      void* stream) {
+    if (!image || !target || !dimage || n <= 0) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_mse_loss_grad: invalid argument");
+    launch_mse(image, target, n, dimage, loss_out, reinterpret_cast<hipStream_t>(stream));
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// Per-view tet projection, front-face culling and screen-tile binning on gfx950.
+//
+// Replaces GenerateTrianglesIntersect.glsl:74-132 (per-tet projection + front faces, one atomic append each),
+// ComputeTriangleDepths.glsl:72-110 (centroid-depth keys) and the rasteriser's fragment generation: each front
+// face becomes one (tile, face) record per screen tile its pixel-centre bounding box touches, keyed by
+// (tile << 32 | depth-key bits).  Counting and emission are two passes over the tets separated by a
+// device-wide scan, so records are written in (tet, face) order and the stable sort keeps that order for
+// equal depth keys.
+#include "dtv_kernels.h"
+#include "dtv_prims.h"
+
+namespace dtv {
+
+struct FaceBins {
+    int32_t tx0, tx1, ty0, ty1;
+    uint32_t count;
+};
+
+__device__ __forceinline__ FaceBins face_bins(const ViewUniforms& U, const ProjectedTet& T, int f) {
+    FaceBins b{0, -1, 0, -1, 0u};
+    ScreenTri S;
+    face_screen(U, T, f, S);
+    int32_t x0, x1, y0, y1;
+    tri_pixel_bbox(U, S, x0, x1, y0, y1);
+    if (x0 > x1 || y0 > y1) return b;
+    b.tx0 = x0 >> U.tile_shift;
+    b.tx1 = x1 >> U.tile_shift;
+    b.ty0 = y0 >> U.tile_shift;
+    b.ty1 = y1 >> U.tile_shift;
+    b.count = (uint32_t)((b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1));
+    return b;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_and_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v &= __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_count_pairs(const ViewUniforms* __restrict__ pU,
+                                                      const int32_t* __restrict__ cells,
+                                                      const float* __restrict__ pos, int64_t T,
+                                                      uint4* __restrict__ counts, BinStats* __restrict__ stats) {
+    const ViewUniforms& U = *pU;
+    const int64_t tet = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    uint32_t c[4] = {0, 0, 0, 0};
+    uint32_t ntri = 0, kand = 0xFFFFFFFFu, kor = 0u;
+    if (tet < T) {
+        ProjectedTet PT;
+        project_tet(U, cells, pos, tet, PT);
+        if (!PT.culled) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                if (!(PT.visible & (1u << f))) continue;
+                ntri++;
+                FaceBins b = face_bins(U, PT, f);
+                c[f] = b.count;
+                if (b.count) {
+                    uint32_t k = face_key(PT, f);
+                    kand &= k;
+                    kor |= k;
+                }
+            }
+        }
+        counts[tet] = make_uint4(c[0], c[1], c[2], c[3]);
+    }
+    uint32_t psum = c[0] + c[1] + c[2] + c[3];
+    uint32_t wsum = wave_sum_u32(psum);
+    uint32_t wtri = wave_sum_u32(ntri);
+    uint32_t wand = wave_and_u32(kand);
+    uint32_t wor = wave_or_u32(kor);
+    if (lane_id() == 0) {
+        if (wsum) atomicAdd((unsigned long long*)&stats->total_pairs, (unsigned long long)wsum);
+        if (wtri) atomicAdd(&stats->num_tris, wtri);
+        if (wand != 0xFFFFFFFFu) atomicAnd(&stats->key_and, wand);
+        if (wor) atomicOr(&stats->key_or, wor);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_emit_pairs(const ViewUniforms* __restrict__ pU,
+                                                     const int32_t* __restrict__ cells,
+                                                     const float* __restrict__ pos, int64_t T,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     const uint4* __restrict__ counts, uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+    const ViewUniforms& U = *pU;
+    const int64_t tet = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (tet >= T) return;
+    uint4 cc = counts[tet];
+    if ((cc.x | cc.y | cc.z | cc.w) == 0) return;
+    ProjectedTet PT;
+    project_tet(U, cells, pos, tet, PT);
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        uint32_t cf = f == 0 ? cc.x : f == 1 ? cc.y : f == 2 ? cc.z : cc.w;
+        if (cf == 0) continue;
+        FaceBins b = face_bins(U, PT, f);
+        const uint64_t k = (uint64_t)face_key(PT, f);
+        const uint32_t o = offsets[4 * tet + f];
+        const uint32_t id = (uint32_t)(tet * 4 + f);
+        const int bw = b.tx1 - b.tx0 + 1;
+        // Bounded by the stored count: the pass recomputes the same bins, but never writes past its slots.
+        const uint32_t n = min(cf, b.count);
+        for (uint32_t i = 0; i < n; i++) {
+            const int tx = b.tx0 + (int)(i % (uint32_t)bw), ty = b.ty0 + (int)(i / (uint32_t)bw);
+            keys[o + i] = ((uint64_t)(ty * U.tiles_x + tx) << 32) | k;
+            vals[o + i] = id;
+        }
+    }
+}
+
+__global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint2* __restrict__ ranges) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t t = (uint32_t)(keys[i] >> 32);
+        if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != t) ranges[t].x = (uint32_t)i;
+        if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != t) ranges[t].y = (uint32_t)(i + 1);
+    }
+}
+
+void launch_count_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T, uint32_t* counts,
+                        BinStats* stats, hipStream_t s) {
+    if (T <= 0) return;
+    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, dU, cells, pos, T,
+                       reinterpret_cast<uint4*>(counts), stats);
+}
+
+void launch_emit_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T,
+                       const uint32_t* offsets, const uint32_t* counts, uint64_t* keys, uint32_t* vals,
+                       hipStream_t s) {
+    if (T <= 0) return;
+    hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, dU, cells, pos, T, offsets,
+                       reinterpret_cast<const uint4*>(counts), keys, vals);
+}
+
+void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s) {
+    fill_u32(reinterpret_cast<uint32_t*>(ranges), 0u, 2 * num_tiles, s);
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, sorted_keys, n, ranges);
+}
+
+}  // namespace dtv

@@ -1,0 +1,72 @@
+// Host-side launchers of the difftetvr_amd kernels (internal; the public surface is include/difftetvr_amd.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dtv_device.h"
+
+namespace dtv {
+
+// sort.hip
+void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+int64_t scan_partitions(int64_t n);
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* status, uint32_t* counter,
+                        uint64_t* total, hipStream_t s);
+int64_t sort_partitions(int64_t n);
+int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                     uint32_t digit_mask, uint32_t* hist, uint32_t* lookback, uint32_t* counters, hipStream_t s);
+
+// binning.hip
+struct BinStats {
+    uint64_t total_pairs;   // sum of (tile, face) records
+    uint32_t num_tris;      // visible faces
+    uint32_t key_and;       // AND of depth keys of all faces that produce records
+    uint32_t key_or;        // OR of the same keys
+    uint32_t pad;
+};
+void launch_count_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T, uint32_t* counts,
+                        BinStats* stats, hipStream_t s);
+void launch_emit_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T,
+                       const uint32_t* offsets, const uint32_t* counts, uint64_t* keys, uint32_t* vals,
+                       hipStream_t s);
+void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s);
+
+// render.hip
+struct PixelState {
+    float4* ray_color;      // [P] composite before the clear-colour blend
+    uint32_t* frag_count;   // [P] fragments composited
+    uint32_t* last_rec;     // [P] index (in the sorted record list) one past the last composited record
+    uint32_t* num_ckpt;     // [P]
+    float4* ckpt_color;     // [max_ckpt][P]
+    uint32_t* ckpt_n;       // [max_ckpt][P]
+};
+void launch_render_forward(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
+                           const float* col, const uint32_t* rec_vals, const uint2* ranges, float* out_image,
+                           PixelState ps, hipStream_t s);
+void launch_render_adjoint(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
+                           const float* col, const uint32_t* rec_vals, const uint2* ranges, const float* dimage,
+                           PixelState ps, int use_abs, int mode, float* grad_pos, float* grad_col, hipStream_t s);
+void launch_export_fragments(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
+                             const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int max_per_pixel,
+                             int32_t* out_ids, hipStream_t s);
+
+// optim.hip
+struct AdamGroupDev {
+    float* param;
+    float* grad;
+    float* m;
+    float* v;
+    int64_t n;
+    float step_size;    // lr / (1 - beta1^t)
+    float bc2_sqrt;     // sqrt(1 - beta2^t)
+    float w1;           // 1 - beta1
+    float beta2;
+    float w2;           // 1 - beta2
+    float eps;
+    int32_t clamp_nonneg;
+    int32_t zero_grad;
+};
+void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s);
+void launch_mse(const float* img, const float* tgt, int64_t n, float* dimg, float* loss, hipStream_t s);
+
+}  // namespace dtv

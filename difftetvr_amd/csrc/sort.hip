@@ -1,0 +1,277 @@
+// Device-wide primitives for the binning stage on gfx950:
+//   * exclusive scan of u32 counts (single pass, decoupled look-back),
+//   * LSD onesweep radix sort of (u64 key, u32 value) pairs: per-digit histograms in one pass, then one
+//     scatter pass per non-trivial 8-bit digit with in-workgroup stable ranking (wave64 ballot match), LDS
+//     reordering for contiguous writes, and decoupled look-back across partitions for the global offsets.
+// Replaces fuchsia_radix_sort (TetMeshRendererIntersection.cpp:624-653) and its CPU std::sort fallback
+// (:682-696).  Stability makes equal keys keep their input order.
+#include "dtv_kernels.h"
+#include "dtv_prims.h"
+
+namespace dtv {
+
+// ------------------------------------------------------------------------------------------------------
+// Fill
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Exclusive scan, 256 threads x 16 items per partition.  status: 64-bit words {flag:2 in bits 62-63, value}.
+constexpr int kScanBlock = 256, kScanItems = 16, kScanPart = kScanBlock * kScanItems;
+constexpr uint64_t kS64Agg = 1ull << 62, kS64Inc = 2ull << 62, kS64Mask = (1ull << 62) - 1;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_exclusive(const uint32_t* __restrict__ in,
+                                                               uint32_t* __restrict__ out, int64_t n,
+                                                               uint64_t* __restrict__ status,
+                                                               uint32_t* __restrict__ part_counter,
+                                                               uint64_t* __restrict__ total_out) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_tmp[kScanBlock / 64 + 1];
+    __shared__ uint64_t s_prefix;
+    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
+    __syncthreads();
+    const uint32_t part = s_part;
+    const int64_t base = (int64_t)part * kScanPart + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        int64_t idx = base + i;
+        v[i] = idx < n ? in[idx] : 0u;
+        sum += v[i];
+    }
+    uint32_t agg;
+    uint32_t excl = block_exclusive_scan<kScanBlock>(sum, s_tmp, agg);
+    if (threadIdx.x == 0) {
+        uint64_t prefix = 0;
+        if (part == 0) {
+            status_store64(&status[0], kS64Inc | (uint64_t)agg);
+        } else {
+            status_store64(&status[part], kS64Agg | (uint64_t)agg);
+            int64_t p = (int64_t)part - 1;
+            while (p >= 0) {
+                uint64_t w = status_load64(&status[p]);
+                uint64_t flag = w & ~kS64Mask;
+                if (flag == 0) continue;
+                prefix += w & kS64Mask;
+                if (flag == kS64Inc) break;
+                p--;
+            }
+            status_store64(&status[part], kS64Inc | (prefix + agg));
+        }
+        s_prefix = prefix;
+        if ((int64_t)(part + 1) * kScanPart >= n) *total_out = prefix + agg;
+    }
+    __syncthreads();
+    uint32_t run = (uint32_t)s_prefix + excl;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        int64_t idx = base + i;
+        if (idx < n) out[idx] = run;
+        run += v[i];
+    }
+}
+
+int64_t scan_partitions(int64_t n) { return (n + kScanPart - 1) / kScanPart; }
+
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* status, uint32_t* counter,
+                        uint64_t* total, hipStream_t s) {
+    int64_t parts = scan_partitions(n);
+    if (parts == 0) parts = 1;
+    fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 2 * parts, s);
+    fill_u32(counter, 0u, 1, s);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3((unsigned)parts), dim3(kScanBlock), 0, s, in, out, n, status, counter,
+                       total);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Radix sort.
+constexpr int kSortBlock = 256, kSortItems = 16, kSortPart = kSortBlock * kSortItems, kSortWaves = kSortBlock / 64;
+
+int64_t sort_partitions(int64_t n) { return (n + kSortPart - 1) / kSortPart; }
+
+// Histograms of the digit positions selected in digit_mask (bit p -> bits [8p, 8p+8)).  hist[p][256].
+// One LDS histogram per wave keeps the atomics of a single heavy bin spread over four copies.
+__global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* __restrict__ keys, int64_t n,
+                                                          uint32_t digit_mask, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[4][8][256];
+    for (int i = threadIdx.x; i < 4 * 8 * 256; i += 256) (&s_h[0][0][0])[i] = 0;
+    __syncthreads();
+    const int w = threadIdx.x / 64;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = keys[i];
+#pragma unroll
+        for (int p = 0; p < 8; p++)
+            if (digit_mask & (1u << p)) atomicAdd(&s_h[w][p][(k >> (8 * p)) & 255], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 8 * 256; i += 256) {
+        int p = i / 256;
+        if (!(digit_mask & (1u << p))) continue;
+        uint32_t c = s_h[0][p][i % 256] + s_h[1][p][i % 256] + s_h[2][p][i % 256] + s_h[3][p][i % 256];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// Exclusive scan of each selected histogram in place: offsets[p][d] = sum_{d' < d} hist[p][d'].
+__global__ __launch_bounds__(256) void k_digit_offsets(uint32_t* __restrict__ hist, uint32_t digit_mask) {
+    __shared__ uint32_t s_tmp[256 / 64 + 1];
+    for (int p = 0; p < 8; p++) {
+        if (!(digit_mask & (1u << p))) continue;
+        uint32_t v = hist[p * 256 + threadIdx.x];
+        uint32_t tot;
+        uint32_t e = block_exclusive_scan<256>(v, s_tmp, tot);
+        hist[p * 256 + threadIdx.x] = e;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
+        const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+        uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ digit_offsets,
+        uint32_t* __restrict__ lookback, uint32_t* __restrict__ part_counter) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wcnt[kSortWaves][256];
+    __shared__ uint32_t s_local_start[256];
+    __shared__ uint32_t s_global_base[256];
+    __shared__ uint32_t s_tmp[kSortBlock / 64 + 1];
+    __shared__ uint64_t s_keys[kSortPart];
+    __shared__ uint32_t s_vals[kSortPart];
+
+    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
+    for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortBlock) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t part = s_part;
+    const int w = threadIdx.x / 64;
+    const int lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const int64_t wbase = (int64_t)part * kSortPart + (int64_t)w * (64 * kSortItems);
+
+    uint64_t key[kSortItems];
+    uint32_t val[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        int64_t idx = wbase + i * 64 + lane;
+        bool valid = idx < n;
+        key[i] = valid ? kin[idx] : ~0ull;
+        val[i] = valid ? vin[idx] : 0u;
+    }
+    // Stable rank inside the wave: items are visited in input order (slot i, then lane).
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        int64_t idx = wbase + i * 64 + lane;
+        bool valid = idx < n;
+        uint32_t d = (uint32_t)(key[i] >> shift) & 255u;
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        uint32_t below = (uint32_t)popc64(peers & lt);
+        uint32_t cnt = (uint32_t)popc64(peers);
+        uint32_t basecnt = valid ? s_wcnt[w][d] : 0u;
+        rank[i] = basecnt + below;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) s_wcnt[w][d] = basecnt + cnt;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // Per digit (one thread each): wave offsets, block total, look-back for the global base.
+    const uint32_t d = threadIdx.x;
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < kSortWaves; ww++) {
+        uint32_t c = s_wcnt[ww][d];
+        s_wcnt[ww][d] = run;
+        run += c;
+    }
+    const uint32_t total_d = run;
+    uint32_t* my = lookback + (int64_t)part * 256 + d;
+    uint32_t excl = 0;
+    if (part == 0) {
+        status_store(my, kFlagInc | total_d);
+    } else {
+        status_store(my, kFlagAgg | total_d);
+        int64_t p = (int64_t)part - 1;
+        while (p >= 0) {
+            uint32_t wv = status_load(lookback + p * 256 + d);
+            uint32_t flag = wv & ~kValMask;
+            if (flag == 0) continue;
+            excl += wv & kValMask;
+            if (flag == kFlagInc) break;
+            p--;
+        }
+        status_store(my, kFlagInc | (excl + total_d));
+    }
+    s_global_base[d] = digit_offsets[d] + excl;
+    uint32_t blk_total;
+    uint32_t lstart = block_exclusive_scan<kSortBlock>(total_d, s_tmp, blk_total);
+    s_local_start[d] = lstart;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        int64_t idx = wbase + i * 64 + lane;
+        if (idx < n) {
+            uint32_t dd = (uint32_t)(key[i] >> shift) & 255u;
+            uint32_t lp = s_local_start[dd] + s_wcnt[w][dd] + rank[i];
+            s_keys[lp] = key[i];
+            s_vals[lp] = val[i];
+        }
+    }
+    __syncthreads();
+    const int64_t pbase = (int64_t)part * kSortPart;
+    const int64_t valid_n = (n - pbase) < kSortPart ? (n - pbase) : kSortPart;
+    for (int j = threadIdx.x; j < valid_n; j += kSortBlock) {
+        uint64_t k = s_keys[j];
+        uint32_t dd = (uint32_t)(k >> shift) & 255u;
+        uint32_t dest = s_global_base[dd] + ((uint32_t)j - s_local_start[dd]);
+        kout[dest] = k;
+        vout[dest] = s_vals[j];
+    }
+}
+
+// Sorts (keys, vals) of length n by the digits in digit_mask (ascending digit positions), ping-ponging with
+// (keys_alt, vals_alt).  hist must hold 8*256 u32; lookback (#passes * partitions * 256) u32; counters 8 u32.
+// Returns the number of passes; the result is in the alternate buffers when that number is odd.
+int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                     uint32_t digit_mask, uint32_t* hist, uint32_t* lookback, uint32_t* counters, hipStream_t s) {
+    if (n <= 1 || digit_mask == 0) return 0;
+    const int64_t parts = sort_partitions(n);
+    int npass = 0;
+    for (int p = 0; p < 8; p++) npass += (digit_mask >> p) & 1;
+    fill_u32(hist, 0u, 8 * 256, s);
+    fill_u32(counters, 0u, 8, s);
+    fill_u32(lookback, 0u, (int64_t)npass * parts * 256, s);
+    int64_t hblocks = (n + 255) / 256;
+    if (hblocks > 1024) hblocks = 1024;
+    hipLaunchKernelGGL(k_digit_histogram, dim3((unsigned)hblocks), dim3(256), 0, s, keys, n, digit_mask, hist);
+    hipLaunchKernelGGL(k_digit_offsets, dim3(1), dim3(256), 0, s, hist, digit_mask);
+    uint64_t* ka = keys;
+    uint32_t* va = vals;
+    uint64_t* kb = keys_alt;
+    uint32_t* vb = vals_alt;
+    int pass = 0;
+    for (int p = 0; p < 8; p++) {
+        if (!((digit_mask >> p) & 1)) continue;
+        hipLaunchKernelGGL(k_onesweep_pass, dim3((unsigned)parts), dim3(kSortBlock), 0, s, ka, va, kb, vb, n, 8 * p,
+                           hist + p * 256, lookback + (int64_t)pass * parts * 256, counters + pass);
+        uint64_t* tk = ka; ka = kb; kb = tk;
+        uint32_t* tv = va; va = vb; vb = tv;
+        pass++;
+    }
+    return npass;
+}
+
+}  // namespace dtv

@@ -1,0 +1,70 @@
+"""Fused Adam for the mesh parameters: one HIP launch updates every parameter group.
+
+``FusedAdam`` is a ``torch.optim.Optimizer`` with torch.optim.Adam's update rule and defaults (the optimiser
+pytests/train.py:318,351 uses) plus two per-group options: ``clamp_nonneg`` applies train.py:352-356's colour
+clamp ``c -= min(c, 0)`` inside the same kernel, and ``zero_grad_after`` zeroes the consumed gradient (the
+next step's ``optimizer.zero_grad(set_to_none=False)``, train.py:342).
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class FusedAdam(torch.optim.Optimizer):
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clamp_nonneg=False, zero_grad_after=False):
+        defaults = dict(lr=lr, betas=betas, eps=eps, clamp_nonneg=clamp_nonneg, zero_grad_after=zero_grad_after)
+        super().__init__(params, defaults)
+
+    @torch.no_grad()
+    def step(self, closure=None):
+        loss = None
+        if closure is not None:
+            with torch.enable_grad():
+                loss = closure()
+        groups = []
+        for group in self.param_groups:
+            for p in group["params"]:
+                if p.grad is None:
+                    continue
+                if not (p.is_cuda and p.dtype == torch.float32 and p.is_contiguous() and p.grad.is_contiguous()):
+                    raise RuntimeError("FusedAdam: parameters must be contiguous float32 GPU tensors.")
+                st = self.state[p]
+                if len(st) == 0:
+                    st["step"] = 0
+                    st["exp_avg"] = torch.zeros_like(p)
+                    st["exp_avg_sq"] = torch.zeros_like(p)
+                st["step"] += 1
+                g = _lib.DtvAdamGroup()
+                g.param, g.grad = p.data_ptr(), p.grad.data_ptr()
+                g.exp_avg, g.exp_avg_sq = st["exp_avg"].data_ptr(), st["exp_avg_sq"].data_ptr()
+                g.numel = p.numel()
+                g.lr = group["lr"]
+                g.beta1, g.beta2 = group["betas"]
+                g.eps = group["eps"]
+                g.step = st["step"]
+                g.clamp_nonneg = int(bool(group["clamp_nonneg"]))
+                g.zero_grad = int(bool(group["zero_grad_after"]))
+                groups.append(g)
+        lib = _lib.load()
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for i in range(0, len(groups), 8):
+            chunk = groups[i:i + 8]
+            arr = (_lib.DtvAdamGroup * len(chunk))(*chunk)
+            _lib.check(lib.dtv_adam_step(arr, len(chunk), stream), "dtv_adam_step")
+        return loss
+
+
+def mse_loss_grad(image, target):
+    """Returns (loss tensor [], dL/dimage) for torch.nn.MSELoss(image, target) using one HIP launch."""
+    image = image.contiguous()
+    target = target.contiguous()
+    dimg = torch.empty_like(image)
+    loss = torch.empty((), dtype=torch.float32, device=image.device)
+    _lib.check(_lib.load().dtv_mse_loss_grad(ctypes.c_void_p(image.data_ptr()), ctypes.c_void_p(target.data_ptr()),
+                                             image.numel(), ctypes.c_void_p(dimg.data_ptr()),
+                                             ctypes.c_void_p(loss.data_ptr()),
+                                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+               "dtv_mse_loss_grad")
+    return loss, dimg

@@ -1,0 +1,158 @@
+/*
+ * difftetvr_amd: C ABI of the MI355X (gfx950) differentiable tetrahedral volume renderer.
+ *
+ * Plain pointers and sizes only.  Every device pointer is caller-owned and borrowed for the duration of the call;
+ * every call is asynchronous on the given HIP stream (hipStream_t passed as void*; NULL = default stream) unless
+ * documented otherwise.  Errors return a negative status; dtv_last_error() returns the message of the calling
+ * thread's last failure.  No C++ exceptions cross this boundary.
+ *
+ * Each entry point cites the reference interface it replaces (paths relative to the DiffTetVR source tree).
+ */
+#ifndef DIFFTETVR_AMD_H
+#define DIFFTETVR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTV_OK 0
+#define DTV_ERR_INVALID_ARGUMENT (-1)
+#define DTV_ERR_HIP (-2)
+#define DTV_ERR_OUT_OF_MEMORY (-3)
+#define DTV_ERR_NOT_RENDERED (-4)
+
+/* Adjoint flavours accepted by dtv_render_adjoint. */
+#define DTV_ADJOINT_EXACT 0     /* derivative of the forward (default) */
+#define DTV_ADJOINT_REFERENCE 1 /* AdjointIntersectRasterization.glsl as written (skips sub-segment 0) */
+
+typedef struct dtv_ctx dtv_ctx;   /* per-device scratch and settings */
+typedef struct dtv_view dtv_view; /* state of one rendered view, kept for its adjoint */
+
+/* Tet mesh, per-vertex colours.  Layouts of TetMesh::getVertexPositionTensor/getVertexColorTensor
+ * (src/Tet/TetMesh.cpp:236-367); cells are int32 vertex indices, 4 per tet, with the reference's orientation
+ * convention dot(cross(p1-p0, p2-p0), p3-p0) < 0 (src/Tet/TetMesh.cpp:532-541) (other orientations render the
+ * same). Colour alpha is the density before attenuation. */
+typedef struct dtv_mesh {
+    const int32_t* cells;    /* device [num_cells * 4] */
+    const float* positions;  /* device [num_vertices * 3] */
+    const float* colors;     /* device [num_vertices * 4] */
+    int64_t num_cells;
+    int64_t num_vertices;
+} dtv_mesh;
+
+/* Camera: view matrix as Renderer.set_view_matrix (src/Module/Module.cpp:904-911, 16 floats column-major),
+ * fovy as set_camera_fovy (:897-903), clip planes as the module's defaults (:457-463), viewport as
+ * set_viewport_size (:874-887).  Projection: right-handed, depth in [0,1], y flipped (top-left origin). */
+typedef struct dtv_camera {
+    float view[16];
+    float fovy;
+    float znear;
+    float zfar;
+    int32_t width;
+    int32_t height;
+} dtv_camera;
+
+/* Render settings: TetMeshVolumeRenderer attenuation/clear colour/early-ray-termination
+ * (src/Renderer/TetMeshVolumeRenderer.hpp:237-244, Module.cpp:864-873). */
+typedef struct dtv_render_params {
+    float attenuation;
+    float clear_color[4];
+    int32_t use_early_ray_termination; /* applied only when nonzero (the module passes use_ert && use_gradients) */
+    float early_ray_out_alpha;         /* stop once accumulated alpha > this; reference default 1 - 1e-4 */
+    int32_t tile_size;                 /* screen tile edge in pixels: 8 (default) */
+    int32_t checkpoint_interval;       /* fragments between adjoint checkpoints; 0 = 64 */
+} dtv_render_params;
+
+typedef struct dtv_view_stats {
+    int64_t num_triangles;    /* generated front faces (GenerateTrianglesIntersect.glsl:124) */
+    int64_t num_pairs;        /* (tile, face) records after binning */
+    int64_t num_tiles;
+    int32_t sort_passes;      /* radix passes actually run */
+    int32_t width, height;
+} dtv_view_stats;
+
+/* Adam over one parameter tensor: torch.optim.Adam as used by pytests/train.py:301-319,351, optional colour
+ * clamp c -= min(c, 0) (train.py:352-356), optional fused zero_grad for the next step (train.py:342). */
+typedef struct dtv_adam_group {
+    float* param;
+    float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    int64_t numel;
+    float lr;
+    float beta1, beta2, eps;
+    int64_t step;            /* 1-based step count after this update */
+    int32_t clamp_nonneg;    /* 1: param = max(param, 0) after the update */
+    int32_t zero_grad;       /* 1: grad = 0 after it is consumed */
+} dtv_adam_group;
+
+/* ---- library ---------------------------------------------------------------------------------------- */
+int dtv_get_version(void);
+const char* dtv_last_error(void);
+
+/* Replaces ApplicationState creation (src/Module/Module.cpp:192-489, ensureStateExists :646-650). */
+int dtv_ctx_create(int32_t device, dtv_ctx** out_ctx);
+int dtv_ctx_destroy(dtv_ctx* ctx);
+
+/* Replaces the renderer object's intermediate buffers (TetMeshRendererIntersection.cpp:424-487). */
+int dtv_view_create(dtv_ctx* ctx, dtv_view** out_view);
+int dtv_view_destroy(dtv_view* view);
+
+/* Host-only: the 68 camera uniform floats (viewProj, view, invProj, invView column-major, camera position),
+ * the counterpart of the uniform upload in TetMeshRendererIntersection::render (:604-616). */
+int dtv_camera_uniforms(const dtv_camera* camera, float* out68);
+
+/* Forward render into out_image [height, width, 4] (premultiplied RGBA blended onto the clear colour).
+ * Replaces Renderer.render (Module.cpp:912-921) -> TetMeshRendererIntersection::render (:603-707).
+ * Synchronises the stream once to size the tile-pair buffers. */
+int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, const dtv_render_params* params,
+               float* out_image, void* stream);
+
+/* Adjoint of the last dtv_render of this view: accumulates (+=) into grad_positions [V,3] and
+ * grad_colors [V,4].  Replaces Renderer.render_adjoint (Module.cpp:922-959) ->
+ * TetMeshRendererIntersection::renderAdjoint (:709-715).  The mesh must be the one rendered. */
+int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage, int32_t use_abs_grad,
+                       int32_t adjoint_mode, float* grad_positions, float* grad_colors, void* stream);
+
+/* Per-pixel number of fragments composited by the last render (depth complexity; the reference's
+ * SHOW_DEPTH_COMPLEXITY counter, IntersectRasterization.glsl:267-270).  out: device int32 [H*W]. */
+int dtv_view_fragment_counts(dtv_view* view, int32_t* out_counts, void* stream);
+
+/* Per-pixel ordered fragment ids (tet*4 + face), front to back, up to max_per_pixel each, padded with -1.
+ * Counterpart of the PPLL export (Module.cpp:960-981).  out: device int32 [H*W*max_per_pixel]. */
+int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_per_pixel, int32_t* out_ids,
+                              void* stream);
+
+int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out);
+
+/* Per-stage GPU timers (hipEvents recorded on the caller's stream around each stage of dtv_render /
+ * dtv_render_adjoint).  No reference counterpart (the reference only has wall-clock opt_time,
+ * pytests/train.py:366,434).  dtv_view_get_timings synchronises on the recorded events and returns the
+ * milliseconds of the most recent call of each stage in out_ms[DTV_NUM_STAGES] (negative if not run). */
+#define DTV_STAGE_COUNT_PAIRS 0 /* projection, culling and record counting */
+#define DTV_STAGE_SCAN 1
+#define DTV_STAGE_EMIT 2
+#define DTV_STAGE_SORT 3
+#define DTV_STAGE_RANGES 4
+#define DTV_STAGE_FORWARD 5
+#define DTV_STAGE_ADJOINT 6
+#define DTV_NUM_STAGES 7
+int dtv_view_set_timing(dtv_view* view, int32_t enabled);
+int dtv_view_get_timings(dtv_view* view, float* out_ms);
+
+/* One fused launch over up to 8 parameter groups.  Replaces optimizer.step() + clamp (train.py:351-356)
+ * and the GLSL Adam pass (Optimizer.glsl:136-169, OptimizerPass.cpp:120-177). */
+int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream);
+
+/* MSE loss and its image adjoint: dimage = 2 (image - target) / n, *loss_out (device float, may be NULL)
+ * = mean squared error.  Replaces torch.nn.MSELoss in train.py:325,346 and Loss.glsl:55-95. */
+int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFTETVR_AMD_H */
