@@ -85,7 +85,19 @@ def main():
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=32)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 256 MiB (scale reference for HBM PMC counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024 // 4
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU per box "
               "(views and requests would shard across GPUs; see DESIGN.md).", file=sys.stderr)

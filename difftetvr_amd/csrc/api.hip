@@ -138,7 +138,7 @@ struct dtv_view {
     ViewUniforms hU{};
     ViewUniforms* pinnedU = nullptr;
     BinStats* pinnedStats = nullptr;
-    DevBuf dU, counts, offsets, scan_status, small, keys, vals, keys2, vals2, hist, lookback, ranges;
+    DevBuf dU, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2, hist, lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
     // state of the last render
     bool rendered = false;
@@ -211,7 +211,8 @@ int dtv_view_destroy(dtv_view* v) {
     if (!v) return DTV_OK;
     (void)hipSetDevice(v->ctx->device);
     (void)hipDeviceSynchronize();
-    DevBuf* bufs[] = {&v->dU, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->keys, &v->vals, &v->keys2,
+    DevBuf* bufs[] = {&v->dU, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
+                      &v->vals, &v->keys2,
                       &v->vals2, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
                       &v->num_ckpt, &v->ckpt_color, &v->ckpt_n};
     for (DevBuf* b : bufs) b->release();
@@ -293,18 +294,15 @@ int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, c
     BinStats* dstats = view->small.as<BinStats>();
     uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
     uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
-    {
-        BinStats init{};
-        init.key_and = 0xFFFFFFFFu;
-        *view->pinnedStats = init;
-        DTV_HIP(hipMemcpyAsync(dstats, view->pinnedStats, sizeof(BinStats), hipMemcpyHostToDevice, s));
-    }
+    DTV_HIP(view->block_stats.ensure(sizeof(uint32_t) * 4 * (count_pairs_blocks(T) + 1)));
     view->mark(DTV_STAGE_COUNT_PAIRS, 0, s);
-    launch_count_pairs(dU, mesh->cells, mesh->positions, T, view->counts.as<uint32_t>(), dstats, s);
+    launch_count_pairs(dU, mesh->cells, mesh->positions, T, view->counts.as<uint32_t>(),
+                       view->block_stats.as<uint32_t>(), s);
     view->mark(DTV_STAGE_COUNT_PAIRS, 1, s);
     view->mark(DTV_STAGE_SCAN, 0, s);
     scan_exclusive_u32(view->counts.as<uint32_t>(), view->offsets.as<uint32_t>(), 4 * T,
                        view->scan_status.as<uint64_t>(), scan_counter, scan_total, s);
+    launch_reduce_bin_stats(view->block_stats.as<uint32_t>(), T, scan_total, dstats, s);
     view->mark(DTV_STAGE_SCAN, 1, s);
     DTV_HIP(hipGetLastError());
     DTV_HIP(hipMemcpyAsync(view->pinnedStats, dstats, sizeof(BinStats), hipMemcpyDeviceToHost, s));

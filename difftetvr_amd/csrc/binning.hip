@@ -50,7 +50,7 @@ __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
 __global__ __launch_bounds__(256) void k_count_pairs(const ViewUniforms* __restrict__ pU,
                                                       const int32_t* __restrict__ cells,
                                                       const float* __restrict__ pos, int64_t T,
-                                                      uint4* __restrict__ counts, BinStats* __restrict__ stats) {
+                                                      uint4* __restrict__ counts, uint4* __restrict__ block_stats) {
     const ViewUniforms& U = *pU;
     const int64_t tet = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     uint32_t c[4] = {0, 0, 0, 0};
@@ -74,16 +74,50 @@ __global__ __launch_bounds__(256) void k_count_pairs(const ViewUniforms* __restr
         }
         counts[tet] = make_uint4(c[0], c[1], c[2], c[3]);
     }
-    uint32_t psum = c[0] + c[1] + c[2] + c[3];
-    uint32_t wsum = wave_sum_u32(psum);
-    uint32_t wtri = wave_sum_u32(ntri);
-    uint32_t wand = wave_and_u32(kand);
-    uint32_t wor = wave_or_u32(kor);
-    if (lane_id() == 0) {
-        if (wsum) atomicAdd((unsigned long long*)&stats->total_pairs, (unsigned long long)wsum);
-        if (wtri) atomicAdd(&stats->num_tris, wtri);
-        if (wand != 0xFFFFFFFFu) atomicAnd(&stats->key_and, wand);
-        if (wor) atomicOr(&stats->key_or, wor);
+    // Per-block partials (no global atomics: one address hit by every wave serialises at the memory side).
+    __shared__ uint4 s_part[4];
+    const uint32_t wtri = wave_sum_u32(ntri);
+    const uint32_t wand = wave_and_u32(kand);
+    const uint32_t wor = wave_or_u32(kor);
+    if (lane_id() == 0) s_part[threadIdx.x / 64] = make_uint4(wtri, wand, wor, 0u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint4 r = s_part[0];
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) {
+            r.x += s_part[w].x;
+            r.y &= s_part[w].y;
+            r.z |= s_part[w].z;
+        }
+        block_stats[blockIdx.x] = r;
+    }
+}
+
+// Reduces the per-block partials of k_count_pairs; the record total comes from the scan.
+__global__ __launch_bounds__(256) void k_reduce_bin_stats(const uint4* __restrict__ block_stats, int64_t nblocks,
+                                                           const uint64_t* __restrict__ scan_total,
+                                                           BinStats* __restrict__ stats) {
+    __shared__ uint4 s_part[4];
+    uint32_t ntri = 0, kand = 0xFFFFFFFFu, kor = 0u;
+    for (int64_t i = threadIdx.x; i < nblocks; i += blockDim.x) {
+        const uint4 b = block_stats[i];
+        ntri += b.x;
+        kand &= b.y;
+        kor |= b.z;
+    }
+    const uint32_t wtri = wave_sum_u32(ntri), wand = wave_and_u32(kand), wor = wave_or_u32(kor);
+    if (lane_id() == 0) s_part[threadIdx.x / 64] = make_uint4(wtri, wand, wor, 0u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint4 r = s_part[0];
+        for (int w = 1; w < 4; w++) {
+            r.x += s_part[w].x;
+            r.y &= s_part[w].y;
+            r.z |= s_part[w].z;
+        }
+        stats->total_pairs = *scan_total;
+        stats->num_tris = r.x;
+        stats->key_and = r.y;
+        stats->key_or = r.z;
     }
 }
 
@@ -127,11 +161,19 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint
     }
 }
 
+int64_t count_pairs_blocks(int64_t T) { return T > 0 ? (T + 255) / 256 : 0; }
+
 void launch_count_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T, uint32_t* counts,
-                        BinStats* stats, hipStream_t s) {
+                        uint32_t* block_stats, hipStream_t s) {
     if (T <= 0) return;
-    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, dU, cells, pos, T,
-                       reinterpret_cast<uint4*>(counts), stats);
+    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)count_pairs_blocks(T)), dim3(256), 0, s, dU, cells, pos, T,
+                       reinterpret_cast<uint4*>(counts), reinterpret_cast<uint4*>(block_stats));
+}
+
+void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t T, const uint64_t* scan_total, BinStats* stats,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_bin_stats, dim3(1), dim3(256), 0, s, reinterpret_cast<const uint4*>(block_stats),
+                       count_pairs_blocks(T), scan_total, stats);
 }
 
 void launch_emit_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T,

@@ -24,8 +24,12 @@ struct BinStats {
     uint32_t key_or;        // OR of the same keys
     uint32_t pad;
 };
+int64_t count_pairs_blocks(int64_t T);
 void launch_count_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T, uint32_t* counts,
-                        BinStats* stats, hipStream_t s);
+                        uint32_t* block_stats, hipStream_t s);
+// stats.total_pairs = *scan_total; the other fields reduce the per-block partials of launch_count_pairs.
+void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t T, const uint64_t* scan_total, BinStats* stats,
+                             hipStream_t s);
 void launch_emit_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T,
                        const uint32_t* offsets, const uint32_t* counts, uint64_t* keys, uint32_t* vals,
                        hipStream_t s);

@@ -86,15 +86,19 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ img, cons
         acc += d * d;
     }
     if (loss != nullptr) {
+        // one atomic per block: the loss address is shared by the whole grid
+        __shared__ float s_part[4];
         const float ws = wave_sum(acc);
-        if (lane_id() == 0) atomic_add_f32(loss, ws * inv_n);
+        if (lane_id() == 0) s_part[threadIdx.x / 64] = ws;
+        __syncthreads();
+        if (threadIdx.x == 0) atomic_add_f32(loss, ((s_part[0] + s_part[1]) + (s_part[2] + s_part[3])) * inv_n);
     }
 }
 
 void launch_mse(const float* img, const float* tgt, int64_t n, float* dimg, float* loss, hipStream_t s) {
     if (loss != nullptr) fill_u32(reinterpret_cast<uint32_t*>(loss), 0u, 1, s);
     int64_t blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_mse, dim3((unsigned)blocks), dim3(256), 0, s, img, tgt, n, dimg, loss);
 }

@@ -1,0 +1,341 @@
+// Record staging and per-fragment maths shared by the tile renderer's forward, adjoint and export kernels.
+#pragma once
+#include "dtv_kernels.h"
+#include "dtv_prims.h"
+
+namespace dtv {
+
+constexpr int kTileEdge = 8;
+constexpr int kBatch = 32;  // records staged per batch (one 32-bit coverage mask per pixel)
+
+struct RecordLds {
+    float p[12][kBatch];   // tet vertex positions, [3*v + c]
+    float c[16][kBatch];   // tet vertex colours, [4*v + ch]
+    float N[12][kBatch];   // face plane normals cross(p1-p0, p2-p0), [3*f + c]
+    float k[4][kBatch];    // dot(N, camPos) - dot(N, p0) per face
+    int32_t A[3][kBatch];  // edge functions of the record's face: E = C + A*ox + B*oy (fixed point)
+    int32_t B[3][kBatch];
+    long long C[3][kBatch];
+    uint32_t meta[kBatch]; // bits 0-2: edge tie flags, bit 3: face normals point outward
+    int32_t vid[4][kBatch];
+};
+
+__device__ __forceinline__ void setup_record(const ViewUniforms& U, const int32_t* __restrict__ cells,
+                                             const float* __restrict__ pos, const float* __restrict__ col,
+                                             uint32_t rec, int tx, int ty, RecordLds& L, int i) {
+    const int64_t tet = rec >> 2;
+    const int f = rec & 3;
+    ProjectedTet PT;
+    project_tet(U, cells, pos, tet, PT);
+    ScreenTri S;
+    face_screen(U, PT, f, S);
+    const long long px0 = 256LL * (tx * kTileEdge) + 128, py0 = 256LL * (ty * kTileEdge) + 128;
+    uint32_t meta = 0;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const int a = e, b = (e + 1) % 3;
+        const int32_t dx = S.X[b] - S.X[a], dy = S.Y[b] - S.Y[a];
+        L.C[e][i] = (long long)dx * (py0 - S.Y[a]) - (long long)dy * (px0 - S.X[a]);
+        L.A[e][i] = -dy;
+        L.B[e][i] = dx;
+        if (dy < 0 || (dy == 0 && dx > 0)) meta |= 1u << e;
+    }
+    const int4 idx = *reinterpret_cast<const int4*>(cells + 4 * tet);
+    const int32_t vid[4] = {idx.x, idx.y, idx.z, idx.w};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        L.p[3 * v + 0][i] = PT.p[v].x;
+        L.p[3 * v + 1][i] = PT.p[v].y;
+        L.p[3 * v + 2][i] = PT.p[v].z;
+        if (col != nullptr) {
+            const float4 cv = *reinterpret_cast<const float4*>(col + 4 * (int64_t)vid[v]);
+            L.c[4 * v + 0][i] = cv.x;
+            L.c[4 * v + 1][i] = cv.y;
+            L.c[4 * v + 2][i] = cv.z;
+            L.c[4 * v + 3][i] = cv.w;
+        }
+        L.vid[v][i] = vid[v];
+    }
+    const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float3_ a = PT.p[kFaceTable[g][0]], b = PT.p[kFaceTable[g][1]], c = PT.p[kFaceTable[g][2]];
+        const float3_ n = cross(sub(b, a), sub(c, a));
+        const float d = -dot(n, a);
+        L.N[3 * g + 0][i] = n.x;
+        L.N[3 * g + 1][i] = n.y;
+        L.N[3 * g + 2][i] = n.z;
+        L.k[g][i] = dot(n, ro) + d;
+        if (g == 0 && dot(n, sub(PT.p[3], PT.p[0])) < 0.0f) meta |= 8u;
+    }
+    L.meta[i] = meta;
+}
+
+__device__ __forceinline__ bool covers(const RecordLds& L, int j, int ox, int oy) {
+    const uint32_t meta = L.meta[j];
+    bool in = true;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const long long E = L.C[e][j] + (long long)L.A[e][j] * ox + (long long)L.B[e][j] * oy;
+        in = in && (E > 0 || (E == 0 && ((meta >> e) & 1u)));
+    }
+    return in;
+}
+
+__device__ __forceinline__ float3_ lds_p(const RecordLds& L, int v, int j) {
+    return f3(L.p[3 * v][j], L.p[3 * v + 1][j], L.p[3 * v + 2][j]);
+}
+__device__ __forceinline__ float4 lds_c(const RecordLds& L, int v, int j) {
+    return make_float4(L.c[4 * v][j], L.c[4 * v + 1][j], L.c[4 * v + 2][j], L.c[4 * v + 3][j]);
+}
+
+struct FragEval {
+    bool hit;
+    int f0, f1;
+    float t0, t1;
+    float u0, v0, u1, v1;
+    float4 col0, col1;
+    float4 seg[4];
+    float4 pre[4];   // composite inside the tet before sub-segment s
+    float4 F;        // composite of the 4 sub-segments
+};
+
+__device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+
+// Barycentric coordinates with unsigned areas and the 1e-5 area floor (BarycentricInterpolation.glsl:29-38).
+__device__ __forceinline__ void bary_ref(float3_ p0, float3_ p1, float3_ p2, float3_ P, float& u, float& v) {
+    const float3_ d20 = sub(p2, p0), d21 = sub(p2, p1);
+    const float area = fmaxf(length(cross(d20, d21)), 1e-5f);
+    u = length(cross(d21, sub(P, p1))) / area;
+    v = length(cross(sub(P, p0), d20)) / area;
+}
+
+// Inside test of rayTriangleIntersect (RayIntersectionTests.glsl:45-68) for face g at ray parameter t (the t
+// that the reference computes for that face): P = ro + t rd must lie on the inner side of all three edges.
+__device__ __forceinline__ bool ref_face_contains(const RecordLds& L, int j, int g, float3_ ro, float3_ rd, float t) {
+    const float3_ p0 = lds_p(L, face_vertex(g, 0), j), p1 = lds_p(L, face_vertex(g, 1), j),
+                  p2 = lds_p(L, face_vertex(g, 2), j);
+    const float3_ N = f3(L.N[3 * g][j], L.N[3 * g + 1][j], L.N[3 * g + 2][j]);
+    const float3_ P = add(ro, scale(rd, t));
+    if (dot(N, cross(sub(p1, p0), sub(P, p0))) < 0.0f) return false;
+    if (dot(N, cross(sub(p2, p1), sub(P, p1))) < 0.0f) return false;
+    if (dot(N, cross(sub(p0, p2), sub(P, p2))) < 0.0f) return false;
+    return true;
+}
+
+// Entry/exit faces and distances with the semantics of intersectRayTet (RayIntersectionTests.glsl:78-106).
+// Fast path: the plane-slab test (last entering / first exiting plane) proposes the two faces, and the
+// reference inside test confirms both; otherwise the reference's four-face procedure decides (edges,
+// vertices and silhouettes, where float rounding can make the reference reject a face).
+__device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, float3_ rd, int& f0, int& f1, float& t0,
+                                        float& t1) {
+    const bool outward = (L.meta[j] >> 3) & 1u;
+    float tin = -__builtin_inff(), tout = __builtin_inff();
+    int fi = -1, fo = -1;
+    float tf[4];
+    bool par[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float3_ n = f3(L.N[3 * g][j], L.N[3 * g + 1][j], L.N[3 * g + 2][j]);
+        const float den = dot(n, rd);
+        par[g] = fabsf(den) < 1e-9f;
+        tf[g] = -L.k[g][j] / den;
+        if (par[g]) continue;
+        const bool entering = outward ? (den < 0.0f) : (den > 0.0f);
+        if (entering) {
+            if (tf[g] > tin) { tin = tf[g]; fi = g; }
+        } else {
+            if (tf[g] < tout) { tout = tf[g]; fo = g; }
+        }
+    }
+    if (fi >= 0 && fo >= 0 && tout >= tin && tin >= 0.0f && ref_face_contains(L, j, fi, ro, rd, tin) &&
+        ref_face_contains(L, j, fo, ro, rd, tout)) {
+        f0 = fi;
+        f1 = fo;
+        t0 = tin;
+        t1 = tout;
+        return true;
+    }
+    t0 = 1e9f;
+    t1 = -1e9f;
+    f0 = 0;
+    f1 = 0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        if (par[g]) continue;
+        if (ref_face_contains(L, j, g, ro, rd, tf[g])) {
+            if (tf[g] >= 0.0f) {
+                if (tf[g] < t0) { f0 = g; t0 = tf[g]; }
+                if (tf[g] > t1) { f1 = g; t1 = tf[g]; }
+            } else {
+                f0 = g;
+                t0 = 0.0f;
+            }
+        }
+    }
+    return t1 >= t0;
+}
+
+__device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_ ro, float3_ rd, float att,
+                                              FragEval& E) {
+    int fi, fo;
+    float tin, tout;
+    E.hit = ray_tet(L, j, ro, rd, fi, fo, tin, tout);
+    if (!E.hit) {
+        E.F = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    E.f0 = fi;
+    E.f1 = fo;
+    E.t0 = tin;
+    E.t1 = tout;
+    const float3_ P0 = add(ro, scale(rd, tin));
+    const float3_ P1 = add(ro, scale(rd, tout));
+    {
+        const int a = face_vertex(fi, 0), b = face_vertex(fi, 1), c = face_vertex(fi, 2);
+        float u, v;
+        bary_ref(lds_p(L, a, j), lds_p(L, b, j), lds_p(L, c, j), P0, u, v);
+        const float w = 1.0f - u - v;
+        const float4 ca = lds_c(L, a, j), cb = lds_c(L, b, j), cc = lds_c(L, c, j);
+        E.u0 = u;
+        E.v0 = v;
+        E.col0 = make_float4((ca.x * u + cb.x * v) + cc.x * w, (ca.y * u + cb.y * v) + cc.y * w,
+                             (ca.z * u + cb.z * v) + cc.z * w, (ca.w * u + cb.w * v) + cc.w * w);
+    }
+    {
+        const int a = face_vertex(fo, 0), b = face_vertex(fo, 1), c = face_vertex(fo, 2);
+        float u, v;
+        bary_ref(lds_p(L, a, j), lds_p(L, b, j), lds_p(L, c, j), P1, u, v);
+        const float w = 1.0f - u - v;
+        const float4 ca = lds_c(L, a, j), cb = lds_c(L, b, j), cc = lds_c(L, c, j);
+        E.u1 = u;
+        E.v1 = v;
+        E.col1 = make_float4((ca.x * u + cb.x * v) + cc.x * w, (ca.y * u + cb.y * v) + cc.y * w,
+                             (ca.z * u + cb.z * v) + cc.z * w, (ca.w * u + cb.w * v) + cc.w * w);
+    }
+    const float tSeg = (tout - tin) / 4.0f;
+    float4 rc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
+        const float c0x = mixf(E.col0.x, E.col1.x, fb), c0y = mixf(E.col0.y, E.col1.y, fb),
+                    c0z = mixf(E.col0.z, E.col1.z, fb);
+        const float c1x = mixf(E.col0.x, E.col1.x, fe), c1y = mixf(E.col0.y, E.col1.y, fe),
+                    c1z = mixf(E.col0.z, E.col1.z, fe);
+        const float a = mixf(E.col0.w, E.col1.w, fm) * att;
+        float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!(a < 1e-6f)) {
+            const float A = expf(-a * tSeg);
+            const float kk = (tSeg + 1.0f / a) * A - 1.0f / a;
+            cur = make_float4((1.0f - A) * c0x + kk * (c0x - c1x), (1.0f - A) * c0y + kk * (c0y - c1y),
+                              (1.0f - A) * c0z + kk * (c0z - c1z), 1.0f - A);
+        }
+        E.pre[s] = rc;
+        E.seg[s] = cur;
+        rc.x = rc.x + (1.0f - rc.w) * cur.x;
+        rc.y = rc.y + (1.0f - rc.w) * cur.y;
+        rc.z = rc.z + (1.0f - rc.w) * cur.z;
+        rc.w = rc.w + (1.0f - rc.w) * cur.w;
+    }
+    E.F = rc;
+}
+
+__device__ __forceinline__ int pixel_index(const ViewUniforms& U, int x, int y) { return y * U.width + x; }
+
+__device__ __forceinline__ int tile_of_block(const ViewUniforms& U) { return blockIdx.x; }
+
+// ---------------------------------------------------------------------------------------------------------
+// Exact adjoint of one fragment given dL/dF.  Writes per-local-vertex gradients gp[v][xyz], gc[v][rgba].
+// Position gradient of face vertex j:  mu_j(P) * Q, Q = s*dLdt*N/(N.d) - [G^T dC - N (dC.G d)/(N.d)] with G the
+// in-plane colour gradient of the face and s = -1 (entry face) / +1 (exit face).
+__device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, float3_ rd, const float dC[4],
+                                            float sdLdt, float3_& Q) {
+    const int ia = face_vertex(f, 0), ib = face_vertex(f, 1), ic = face_vertex(f, 2);
+    const float3_ a = lds_p(L, ia, j), b = lds_p(L, ib, j), c = lds_p(L, ic, j);
+    const float4 ca = lds_c(L, ia, j), cb = lds_c(L, ib, j), cc = lds_c(L, ic, j);
+    const float3_ e1 = sub(b, a), e2 = sub(c, a);
+    const float3_ n = cross(e1, e2);
+    const float inn = 1.0f / dot(n, n);
+    const float3_ gb = scale(cross(e2, n), inn), gc = scale(cross(n, e1), inn);
+    // G^T dC and dC . (G d)
+    const float db[4] = {cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w};
+    const float dcv[4] = {cc.x - ca.x, cc.y - ca.y, cc.z - ca.z, cc.w - ca.w};
+    float sb = 0.f, sc = 0.f;
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        sb += db[ch] * dC[ch];
+        sc += dcv[ch] * dC[ch];
+    }
+    const float3_ gtdc = add(scale(gb, sb), scale(gc, sc));
+    const float dcgd = dot(gtdc, rd);
+    const float3_ N = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);
+    const float s = (sdLdt + dcgd) / dot(N, rd);
+    Q = sub(scale(N, s), gtdc);
+}
+
+__device__ __forceinline__ void frag_backward_exact(const FragEval& E, const RecordLds& L, int j, float3_ rd,
+                                                    float att, float4 dF, float lam0[4], float lam1[4],
+                                                    float dcol0[4], float dcol1[4], float3_& Q0, float3_& Q1) {
+    float dR[4] = {dF.x, dF.y, dF.z, dF.w};
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        dcol0[ch] = 0.f;
+        dcol1[ch] = 0.f;
+    }
+    float dLdt = 0.f;
+    const float tSeg = (E.t1 - E.t0) / 4.0f;
+#pragma unroll
+    for (int s = 3; s >= 0; s--) {
+        const float4 sg = E.seg[s], st = E.pre[s];
+        const float omA = 1.0f - st.w;
+        const float dseg[4] = {omA * dR[0], omA * dR[1], omA * dR[2], omA * dR[3]};
+        dR[3] = dR[3] * (1.0f - sg.w) - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
+        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
+        const float c0[3] = {mixf(E.col0.x, E.col1.x, fb), mixf(E.col0.y, E.col1.y, fb), mixf(E.col0.z, E.col1.z, fb)};
+        const float c1[3] = {mixf(E.col0.x, E.col1.x, fe), mixf(E.col0.y, E.col1.y, fe), mixf(E.col0.z, E.col1.z, fe)};
+        const float a = mixf(E.col0.w, E.col1.w, fm) * att;
+        float dc0[3] = {0.f, 0.f, 0.f}, dc1[3] = {0.f, 0.f, 0.f}, da = 0.f, dts = 0.f;
+        if (!(a < 1e-6f)) {
+            const float A = expf(-a * tSeg);
+            const float ia = 1.0f / a;
+            const float K = (tSeg + ia) * A - ia;
+            const float dKda = ia * ia * (1.0f - A) - tSeg * tSeg * A - tSeg * ia * A;
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) {
+                const float diff = c0[ch] - c1[ch];
+                dts += dseg[ch] * (a * A * c0[ch] - a * tSeg * A * diff);
+                da += dseg[ch] * (tSeg * A * c0[ch] + dKda * diff);
+                dc0[ch] = ((1.0f - A) + K) * dseg[ch];
+                dc1[ch] = -K * dseg[ch];
+            }
+            dts += dseg[3] * a * A;
+            da += dseg[3] * tSeg * A;
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            dcol0[ch] += (1.0f - fb) * dc0[ch] + (1.0f - fe) * dc1[ch];
+            dcol1[ch] += fb * dc0[ch] + fe * dc1[ch];
+        }
+        dcol0[3] += (1.0f - fm) * att * da;
+        dcol1[3] += fm * att * da;
+        dLdt += dts / 4.0f;
+    }
+    face_side_q(L, j, E.f0, rd, dcol0, -dLdt, Q0);
+    face_side_q(L, j, E.f1, rd, dcol1, dLdt, Q1);
+    // barycentric weights over the 4 local vertices (0 at the face's opposite vertex)
+    const float mu0[3] = {E.u0, E.v0, 1.0f - E.u0 - E.v0};
+    const float mu1[3] = {E.u1, E.v1, 1.0f - E.u1 - E.v1};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        float l0 = 0.f, l1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            l0 = face_vertex(E.f0, k) == v ? mu0[k] : l0;
+            l1 = face_vertex(E.f1, k) == v ? mu1[k] : l1;
+        }
+        lam0[v] = l0;
+        lam1[v] = l1;
+    }
+}
+
+}  // namespace dtv
