@@ -463,5 +463,32 @@ class Renderer:
                 "sort_passes": st.sort_passes, "width": st.width, "height": st.height}
 
 
+class TetRegularizer:
+    """Tet-quality regulariser (TetRegularizer, src/Module/Module.cpp:663-693, bound at :983-995).
+
+    ``compute_grad()`` accumulates (+=) the gradient of sum_tets (reg_lambda / softplus_beta)
+    log(1 + exp(-softplus_beta q)) into the mesh positions' ``.grad``, q being the Parthasarathy et al. quality
+    6 sqrt(2) V / l_rms^3 (Data/Shaders/Optimizer/TetRegularizer.glsl:108-166).  As in the reference, the
+    S^{5/2} denominator of dq/dp (S = sum of squared edge lengths) is clamped below at 1e-5.
+    """
+
+    def __init__(self, tet_mesh, reg_lambda, softplus_beta):
+        _ctx()
+        self._lib = _lib.load()
+        self.tet_mesh = tet_mesh
+        self.reg_lambda = float(reg_lambda)
+        self.softplus_beta = float(softplus_beta)
+
+    def compute_grad(self):
+        mesh = self.tet_mesh
+        pos = mesh.get_vertex_positions()
+        if pos.grad is None:
+            pos.grad = torch.zeros_like(pos)
+        desc = mesh._desc()
+        _lib.check(self._lib.dtv_tet_regularizer_grad(ctypes.byref(desc), self.reg_lambda, self.softplus_beta,
+                                                      ctypes.c_void_p(pos.grad.data_ptr()), _stream_ptr()),
+                   "TetRegularizer.compute_grad")
+
+
 from . import optim  # noqa: E402,F401
 from .pyutils import DifferentiableRenderer, DifferentiableRenderingFunction  # noqa: E402,F401

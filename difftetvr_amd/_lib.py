@@ -65,6 +65,7 @@ SIGNATURES = [
     ("dtv_view_set_timing", ctypes.c_int, [c_void_p, c_int32]),
     ("dtv_view_get_timings", ctypes.c_int, [c_void_p, ctypes.POINTER(c_float)]),
     ("dtv_adam_step", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p]),
+    ("dtv_tet_regularizer_grad", ctypes.c_int, [ctypes.POINTER(DtvMesh), c_float, c_float, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
 ]
 

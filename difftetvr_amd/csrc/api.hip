@@ -482,6 +482,17 @@ int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream
     return DTV_OK;
 }
 
+int dtv_tet_regularizer_grad(const dtv_mesh* mesh, float lambda, float beta, float* grad_positions, void* stream) {
+    if (!mesh || !grad_positions) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_tet_regularizer_grad: NULL argument");
+    if (mesh->num_cells < 0 || (mesh->num_cells > 0 && (!mesh->cells || !mesh->positions)))
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_tet_regularizer_grad: invalid mesh");
+    if (!aligned16(mesh->cells)) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_tet_regularizer_grad: cells must be 16-byte aligned");
+    launch_tet_regularizer(mesh->cells, mesh->positions, mesh->num_cells, lambda, beta, grad_positions,
+                           reinterpret_cast<hipStream_t>(stream));
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
 int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,
                       void* stream) {
     if (!image || !target || !dimage || n <= 0) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_mse_loss_grad: invalid argument");

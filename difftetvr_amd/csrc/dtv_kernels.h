@@ -73,4 +73,8 @@ struct AdamGroupDev {
 void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s);
 void launch_mse(const float* img, const float* tgt, int64_t n, float* dimg, float* loss, hipStream_t s);
 
+// regularizer.hip
+void launch_tet_regularizer(const int32_t* cells, const float* pos, int64_t T, float lambda, float beta,
+                            float* grad_pos, hipStream_t s);
+
 }  // namespace dtv

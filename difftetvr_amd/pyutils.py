@@ -35,7 +35,8 @@ class DifferentiableRenderer(torch.nn.Module):
         self.use_abs_grad = False
         self.tet_regularizer = None
         if use_tet_regularizer:
-            raise RuntimeError("TetRegularizer is not available in difftetvr_amd yet.")
+            from . import TetRegularizer
+            self.tet_regularizer = TetRegularizer(renderer.get_tet_mesh(), tet_reg_lambda, tet_reg_softplus_beta)
 
     def set_use_abs_grad(self, use_abs_grad):
         self.use_abs_grad = use_abs_grad

@@ -146,6 +146,12 @@ int dtv_view_get_timings(dtv_view* view, float* out_ms);
  * and the GLSL Adam pass (Optimizer.glsl:136-169, OptimizerPass.cpp:120-177). */
 int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream);
 
+/* Tet-quality regulariser: accumulates (+=) the gradient of sum over tets of the mirrored softplus
+ * (lambda / beta) log(1 + exp(-beta q)), q = 6 sqrt(2) V / l_rms^3, into grad_positions [V,3].
+ * Replaces TetRegularizer.compute_grad (Module.cpp:984-995) -> TetRegularizerPass (TetRegularizerPass.cpp:91-95,
+ * Data/Shaders/Optimizer/TetRegularizer.glsl:108-166). */
+int dtv_tet_regularizer_grad(const dtv_mesh* mesh, float lambda, float beta, float* grad_positions, void* stream);
+
 /* MSE loss and its image adjoint: dimage = 2 (image - target) / n, *loss_out (device float, may be NULL)
  * = mean squared error.  Replaces torch.nn.MSELoss in train.py:325,346 and Loss.glsl:55-95. */
 int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,

@@ -67,6 +67,10 @@ def _load():
                                c.c_int64, c.c_int]
     lib.orc_mse_grad_f.argtypes = [_f32p, _f32p, c.c_int64, _f32p]
     lib.orc_mse_grad_f.restype = c.c_double
+    lib.orc_tet_regularizer_f.argtypes = [_i32p, c.c_int64, _f32p, c.c_float, c.c_float, _f32p]
+    lib.orc_tet_regularizer_d.argtypes = [_i32p, c.c_int64, _f64p, c.c_double, c.c_double, _f64p]
+    lib.orc_tet_regularizer_loss_d.argtypes = [_i32p, c.c_int64, _f64p, c.c_double, c.c_double]
+    lib.orc_tet_regularizer_loss_d.restype = c.c_double
     _lib = lib
     return lib
 
@@ -201,3 +205,20 @@ def mse_grad(img, tgt):
     loss = _load().orc_mse_grad_f(img.reshape(-1), np.ascontiguousarray(tgt, np.float32).reshape(-1), img.size,
                                   d.reshape(-1))
     return loss, d
+
+
+def tet_regularizer_grad(cells, positions, reg_lambda, softplus_beta, dtype=np.float32):
+    """Gradient of the tet-quality regulariser w.r.t. the vertex positions (TetRegularizer.glsl:108-166)."""
+    cells = np.ascontiguousarray(cells, np.int32).reshape(-1)
+    pos = np.ascontiguousarray(positions, dtype).reshape(-1)
+    g = np.zeros_like(pos)
+    fn = _load().orc_tet_regularizer_f if dtype == np.float32 else _load().orc_tet_regularizer_d
+    fn(cells, cells.size // 4, pos, reg_lambda, softplus_beta, g)
+    return g.reshape(-1, 3)
+
+
+def tet_regularizer_loss(cells, positions, reg_lambda, softplus_beta):
+    """sum over tets of (lambda / beta) log(1 + exp(-beta q)) in double precision."""
+    cells = np.ascontiguousarray(cells, np.int32).reshape(-1)
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    return _load().orc_tet_regularizer_loss_d(cells, cells.size // 4, pos, reg_lambda, softplus_beta)

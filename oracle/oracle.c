@@ -388,3 +388,15 @@ double orc_mse_grad_f(const float* img, const float* tgt, int64_t n, float* dimg
     }
     return loss / (double)n;
 }
+
+/* Tet regulariser (TetRegularizer.glsl:108-166): gradient accumulated into gpos, and the loss it derives from. */
+void orc_tet_regularizer_f(const int32_t* cells, int64_t T, const float* pos, float lambda, float beta, float* gpos) {
+    tet_regularizer_f(cells, T, pos, lambda, beta, gpos);
+}
+void orc_tet_regularizer_d(const int32_t* cells, int64_t T, const double* pos, double lambda, double beta,
+                           double* gpos) {
+    tet_regularizer_d(cells, T, pos, lambda, beta, gpos);
+}
+double orc_tet_regularizer_loss_d(const int32_t* cells, int64_t T, const double* pos, double lambda, double beta) {
+    return tet_regularizer_loss_d(cells, T, pos, lambda, beta);
+}

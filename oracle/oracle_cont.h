@@ -569,3 +569,59 @@ static void SFX(adjoint_reference)(const orc_frags* fr, const int32_t* cells, co
         }
     }
 }
+
+/* Tet-quality regulariser gradient, TetRegularizer.glsl:108-166 restated: V, l_rms, quality
+ * q = 6 sqrt(2) V / l_rms^3, dL/dq = -lambda / (1 + exp(beta q)) (mirrored softplus), dq/dp from
+ * T0 = 36 sqrt(3) dot(p0-p3, cross(p0-p1, p0-p2)), T1 = sum of squared edge lengths, T2 = max(T1^{5/2}, 1e-5). */
+static void SFX(tet_regularizer)(const int32_t* cells, int64_t T, const R* pos, R lambda, R beta, R* gpos) {
+    for (int64_t t = 0; t < T; t++) {
+        SFX(v3) p[4];
+        int32_t vid[4];
+        for (int i = 0; i < 4; i++) {
+            vid[i] = cells[4 * t + i];
+            p[i] = SFX(mk3)(pos[3 * (int64_t)vid[i]], pos[3 * (int64_t)vid[i] + 1], pos[3 * (int64_t)vid[i] + 2]);
+        }
+        R V = -SFX(dot3)(SFX(cross3)(SFX(sub3)(p[1], p[0]), SFX(sub3)(p[2], p[0])), SFX(sub3)(p[3], p[0])) / (R)6.0;
+        R T1 = 0;
+        for (int i = 0; i < 4; i++)
+            for (int j = i + 1; j < 4; j++) { SFX(v3) e = SFX(sub3)(p[i], p[j]); T1 += SFX(dot3)(e, e); }
+        R lrms = RSQRT_(T1 / (R)6.0);
+        R quality = (R)6.0 * RSQRT_((R)2.0) * V / (lrms * lrms * lrms);
+        R dLdq = -lambda / ((R)1.0 + REXP_(beta * quality));
+        R T0 = (R)62.353829072479584 * SFX(dot3)(SFX(sub3)(p[0], p[3]), SFX(cross3)(SFX(sub3)(p[0], p[1]), SFX(sub3)(p[0], p[2])));
+        R T2 = SFX(maxr)((R)pow((double)T1, 2.5), (R)1e-5);
+        /* second term: 12 sqrt(3) * (-dD/dp_i) * T1 with D = dot(cross(p1-p0, p2-p0), p3-p0) */
+        SFX(v3) a = SFX(sub3)(p[1], p[0]), b = SFX(sub3)(p[2], p[0]), c = SFX(sub3)(p[3], p[0]);
+        SFX(v3) dD[4];
+        dD[1] = SFX(cross3)(b, c); dD[2] = SFX(cross3)(c, a); dD[3] = SFX(cross3)(a, b);
+        dD[0] = SFX(scl3)(SFX(add3)(SFX(add3)(dD[1], dD[2]), dD[3]), (R)-1.0);
+        SFX(v3) sum = SFX(add3)(SFX(add3)(p[0], p[1]), SFX(add3)(p[2], p[3]));
+        for (int i = 0; i < 4; i++) {
+            SFX(v3) lin = SFX(sub3)(sum, SFX(scl3)(p[i], (R)4.0));  /* (-3 p_i + sum of the others) */
+            SFX(v3) dq = SFX(scl3)(SFX(add3)(SFX(scl3)(lin, T0), SFX(scl3)(dD[i], (R)-20.784609690826528 * T1)), (R)1.0 / T2);
+            gpos[3 * (int64_t)vid[i]] += dLdq * dq.x;
+            gpos[3 * (int64_t)vid[i] + 1] += dLdq * dq.y;
+            gpos[3 * (int64_t)vid[i] + 2] += dLdq * dq.z;
+        }
+    }
+}
+
+/* Loss of the regulariser: sum over tets of (lambda / beta) log(1 + exp(-beta q)). */
+static R SFX(tet_regularizer_loss)(const int32_t* cells, int64_t T, const R* pos, R lambda, R beta) {
+    R loss = 0;
+    for (int64_t t = 0; t < T; t++) {
+        SFX(v3) p[4];
+        for (int i = 0; i < 4; i++) {
+            int32_t v = cells[4 * t + i];
+            p[i] = SFX(mk3)(pos[3 * (int64_t)v], pos[3 * (int64_t)v + 1], pos[3 * (int64_t)v + 2]);
+        }
+        R V = -SFX(dot3)(SFX(cross3)(SFX(sub3)(p[1], p[0]), SFX(sub3)(p[2], p[0])), SFX(sub3)(p[3], p[0])) / (R)6.0;
+        R T1 = 0;
+        for (int i = 0; i < 4; i++)
+            for (int j = i + 1; j < 4; j++) { SFX(v3) e = SFX(sub3)(p[i], p[j]); T1 += SFX(dot3)(e, e); }
+        R lrms = RSQRT_(T1 / (R)6.0);
+        R q = (R)6.0 * RSQRT_((R)2.0) * V / (lrms * lrms * lrms);
+        loss += lambda / beta * (R)log1p(exp(-(double)(beta * q)));
+    }
+    return loss;
+}
