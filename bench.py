@@ -36,8 +36,8 @@ def algorithmic_bytes(stage, T, V, P):
     raise ValueError(stage)
 
 
-def load_traffic(kernel):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_grid64_1024.json")
+def load_traffic(kernel, workload="grid64_1024"):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -76,13 +76,136 @@ def cpu_baseline(w, view, fovy, W, H, row_step):
                       f"{t3 - t0:.2f} s measured"}
 
 
+def bench_multiview(args):
+    """Batched multi-view step: the mesh seen from `views` cameras on a Fibonacci sphere of radius 2 (512 x 512
+    each) rendered, differentiated (loss = sum of the per-view MSEs) and optimised in one batched step, against
+    the plain single-view step (fwd + adjoint + Adam of one view) run for every view."""
+    import numpy as np
+    import torch
+
+    import difftetvr_amd as d
+    from difftetvr_amd import meshgen
+    from difftetvr_amd.optim import FusedAdam, mse_loss_grad, mse_loss_grad_views
+
+    W = H = args.res or 512
+    nv = args.views
+    fovy = math.pi / 4.0
+    views = [meshgen.look_at_view_matrix(tuple(e)) for e in meshgen.fibonacci_sphere(nv, 2.0)]
+    w = meshgen.grid_workload(args.grid, seed=0)
+    T, V, P = w["cells"].shape[0], w["pos"].shape[0], W * H
+
+    def mesh_of(pos, grads):
+        m = d.TetMesh()
+        m.set_use_gradients(grads)
+        m.set_tet_mesh_data(w["cells"], pos, w["col"])
+        return m
+
+    def batch_renderer(m):
+        br = d.BatchRenderer()
+        br.set_attenuation(1.0)
+        br.set_camera_fovy(fovy)
+        br.set_views(m, views, W, H)
+        return br
+
+    targets = batch_renderer(mesh_of(w["pos_gt"], False)).render().detach().clone()
+    mesh = mesh_of(w["pos"], True)
+    br = batch_renderer(mesh)
+    opt = FusedAdam([{"params": [mesh.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
+                     {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True)
+
+    def batched_step():
+        imgs = br.render()
+        _, dimg = mse_loss_grad_views(imgs, targets)
+        br.render_adjoint(dimg, False)
+        opt.step()
+
+    # the plain single-view path on a second copy of the mesh
+    mesh1 = mesh_of(w["pos"], True)
+    opt1 = FusedAdam([{"params": [mesh1.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
+                      {"params": [mesh1.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True)
+    singles = []
+    for v in views:
+        r = d.Renderer(d.RendererType.INTERSECTION)
+        r.set_tet_mesh(mesh1)
+        r.set_attenuation(1.0)
+        r.set_viewport_size(W, H)
+        r.set_camera_fovy(fovy)
+        r.set_view_matrix(v)
+        singles.append(r)
+
+    def single_steps():
+        for i, r in enumerate(singles):
+            img = r.render()
+            _, dimg = mse_loss_grad(img, targets[i])
+            r.render_adjoint(dimg, False)
+            opt1.step()
+
+    for _ in range(args.warmup):
+        batched_step()
+        single_steps()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nsingle = max(1, args.steps // 2)
+    for _ in range(nsingle):
+        single_steps()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    br.set_timing(True)
+    acc = {}
+    nt = 3
+    for _ in range(nt):
+        batched_step()
+        for k, v in br.get_timings().items():
+            acc[k] = acc.get(k, 0.0) + v / nt
+    br.set_timing(False)
+    ms_batch = (t1 - t0) * 1000.0 / args.steps
+    ms_single_view = (t2 - t1) * 1000.0 / (nsingle * nv)
+    stats = br.get_stats()
+    dominant = max(("forward", "adjoint"), key=lambda k: acc.get(k, 0.0))
+    abytes = algorithmic_bytes(dominant, T, V, nv * P) if dominant == "forward" else \
+        16 * T + 56 * V + 32 * nv * P
+    achieved = abytes / (acc[dominant] * 1e-3) / 1e9
+    out = {
+        "metric": f"batched {nv}-view fwd+adjoint+Adam views/s, 1.57M tets @{W}x{H}",
+        "value": nv * 1000.0 / ms_batch,
+        "unit": "views/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{nv} views on a Fibonacci sphere of radius 2, {W}x{H} each, one batched "
+                               f"render + summed MSE + batched adjoint + Adam per step, {args.grid}^3 grid x6 tets",
+                   "tets": T, "vertices": V, "pixels": nv * P, "tile_pairs": stats["num_pairs"],
+                   "single_view_views_per_s": 1000.0 / ms_single_view,
+                   "speedup_vs_single_view": ms_single_view * nv / ms_batch,
+                   "stage_ms": {k: round(v, 4) for k, v in acc.items()}},
+        "roofline": {"bound": "hbm", "kernel": f"k_render_{dominant}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(f"k_render_{dominant}", f"multiview{nv}_grid{args.grid}_{W}"),
+                     "algorithmic_bytes": abytes, "avg_ms": acc[dominant]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=64)
-    ap.add_argument("--res", type=int, default=1024)
+    ap.add_argument("--res", type=int, default=None, help="viewport edge (default 1024; 512 for multiview)")
+    ap.add_argument("--workload", choices=["single", "multiview"], default="single",
+                    help="single: the plain single-view step (default); multiview: 16 views per batched step")
+    ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -103,6 +226,10 @@ def main():
               "(views and requests would shard across GPUs; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "multiview":
+        bench_multiview(args)
+        return
+
     import numpy as np
     import torch
 
@@ -110,7 +237,7 @@ def main():
     from difftetvr_amd import meshgen
     from difftetvr_amd.optim import FusedAdam, mse_loss_grad
 
-    W = H = args.res
+    W = H = args.res or 1024
     fovy = math.pi / 4.0
     eye = tuple(2.0 * np.array([0.3, 0.4, 1.0]) / np.linalg.norm([0.3, 0.4, 1.0]))
     view = meshgen.look_at_view_matrix(eye)
@@ -182,7 +309,8 @@ def main():
                    "front_faces": stats["num_triangles"], "sort_passes": stats["sort_passes"],
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
         "roofline": {"bound": "hbm", "kernel": f"k_render_{dominant}", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(dominant),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(f"k_render_{dominant}"),
                      "algorithmic_bytes": abytes, "avg_ms": stage_ms[dominant]},
     }
     if not args.no_cpu_baseline:

@@ -22,7 +22,7 @@ from . import meshgen
 from ._lib import DtvError
 
 __all__ = ["vec3", "vec4", "uvec3", "AABB3", "ColorStorage", "TestCase", "RendererType", "TetMesh", "Renderer",
-           "set_device_type", "DtvError", "meshgen"]
+           "BatchRenderer", "TetRegularizer", "set_device_type", "DtvError", "meshgen"]
 
 
 class vec3:
@@ -460,7 +460,137 @@ class Renderer:
         st = _lib.DtvViewStats()
         _lib.check(self._lib.dtv_view_get_stats(self._view, ctypes.byref(st)), "get_stats")
         return {"num_triangles": st.num_triangles, "num_pairs": st.num_pairs, "num_tiles": st.num_tiles,
-                "sort_passes": st.sort_passes, "width": st.width, "height": st.height}
+                "sort_passes": st.sort_passes, "width": st.width, "height": st.height, "num_items": st.num_items,
+                "num_pixels": st.num_pixels}
+
+
+class BatchRenderer(Renderer):
+    """Renders many (mesh, camera) items as one batch: one binning grid over all their tets, one record sort and
+    one tile-renderer launch over all their tiles (``dtv_render_batch``).  Each image equals what ``Renderer``
+    gives for its item alone.  No reference counterpart (the reference renders one view per call); the render
+    settings (attenuation, clear colour, early ray termination, fovy, clip planes) are ``Renderer``'s and apply
+    to every item.
+
+    ``set_items([(tet_mesh, view_matrix, width, height), ...])`` or ``set_views(tet_mesh, view_matrices, width,
+    height)``; ``render()`` returns [N, H, W, 4] when all items share a viewport, else a list of [H_i, W_i, 4];
+    ``render_adjoint(image_adjoints)`` accumulates into every item mesh's ``.grad``.
+    """
+
+    def __init__(self):
+        super().__init__(renderer_type=RendererType.INTERSECTION)
+        self._items = []
+        self._rendered_items = None
+        self._uniform_size = True
+
+    def set_items(self, items):
+        parsed = []
+        for mesh, view_matrix, width, height in items:
+            a = np.asarray(view_matrix, np.float32).reshape(-1)
+            if a.size != 16:
+                raise RuntimeError("set_items: 16 values (column-major) expected per view matrix.")
+            if int(width) <= 0 or int(height) <= 0:
+                raise RuntimeError("set_items: width and height must be positive.")
+            parsed.append((mesh, a.copy(), int(width), int(height)))
+        if not parsed:
+            raise RuntimeError("set_items: at least one item is required.")
+        self._items = parsed
+        self._mesh = parsed[0][0]
+
+    def set_views(self, tet_mesh, view_matrices, width, height):
+        self.set_items([(tet_mesh, v, width, height) for v in view_matrices])
+
+    def get_num_items(self):
+        return len(self._items)
+
+    def _params(self):
+        p = super()._params()
+        p.use_early_ray_termination = int(self._use_ert and all(m.get_use_gradients() for m, _, _, _ in self._items))
+        return p
+
+    def render(self):
+        if not self._items:
+            raise RuntimeError("Missing items in BatchRenderer.render.")
+        for mesh, _, _, _ in self._items:
+            if mesh is None or mesh.get_is_empty():
+                raise RuntimeError("Missing valid tet mesh in BatchRenderer.render.")
+        sizes = [(h, w) for _, _, w, h in self._items]
+        total = sum(h * w for h, w in sizes)
+        flat = torch.empty(total * 4, dtype=torch.float32, device=_device())
+        arr = (_lib.DtvRenderItem * len(self._items))()
+        images, off = [], 0
+        for i, (mesh, vm, w, h) in enumerate(self._items):
+            arr[i].mesh = mesh._desc()
+            cam = arr[i].camera
+            for k in range(16):
+                cam.view[k] = float(vm[k])
+            cam.fovy, cam.znear, cam.zfar = self._fovy, self._znear, self._zfar
+            cam.width, cam.height = w, h
+            img = flat[off * 4:(off + h * w) * 4].view(h, w, 4)
+            arr[i].out_image = img.data_ptr()
+            images.append(img)
+            off += h * w
+        prm = self._params()
+        _lib.check(self._lib.dtv_render_batch(self._view, arr, len(self._items), ctypes.byref(prm), _stream_ptr()),
+                   "BatchRenderer.render")
+        self._rendered_items = list(self._items)
+        self._rendered_mesh = self._mesh
+        self._uniform_size = len(set(sizes)) == 1
+        if self._uniform_size:
+            return flat.view(len(self._items), sizes[0][0], sizes[0][1], 4)
+        return images
+
+    def render_adjoint(self, image_adjoints, use_abs_grad=False):
+        items = self._rendered_items
+        if items is None:
+            raise RuntimeError("Error in render_adjoint: render() must be called first.")
+        if torch.is_tensor(image_adjoints):
+            if image_adjoints.dim() != 4 or image_adjoints.shape[0] != len(items):
+                raise RuntimeError("Error in render_adjoint: [N, H, W, 4] image adjoints expected.")
+            image_adjoints = list(image_adjoints.unbind(0))
+        if len(image_adjoints) != len(items):
+            raise RuntimeError("Error in render_adjoint: one image adjoint per item expected.")
+        arr = (_lib.DtvAdjointItem * len(items))()
+        keep = []  # adjoint images stay referenced until the launch is enqueued
+        for i, ((mesh, _, w, h), d) in enumerate(zip(items, image_adjoints)):
+            if not torch.is_tensor(d) or d.device.type != "cuda" or d.dtype != torch.float32:
+                raise RuntimeError("Error in render_adjoint: float32 device tensors expected.")
+            if tuple(d.shape) != (h, w, 4):
+                raise RuntimeError("Error in render_adjoint: image size differs from the viewport.")
+            d = d.contiguous()
+            if d.data_ptr() % 16:
+                d = d.clone()
+            keep.append(d)
+            pos, col = mesh.get_vertex_positions(), mesh.get_vertex_colors()
+            if pos.grad is None:
+                pos.grad = torch.zeros_like(pos)
+            if col.grad is None:
+                col.grad = torch.zeros_like(col)
+            arr[i].dimage = d.data_ptr()
+            arr[i].grad_positions = pos.grad.data_ptr()
+            arr[i].grad_colors = col.grad.data_ptr()
+        _lib.check(self._lib.dtv_render_adjoint_batch(self._view, arr, len(items), int(bool(use_abs_grad)),
+                                                      self._adjoint_mode, _stream_ptr()),
+                   "BatchRenderer.render_adjoint")
+
+    def get_fragment_counts(self):
+        """int32 [N, H, W] (uniform viewports) or a list of [H_i, W_i]: fragments composited per pixel."""
+        items = self._rendered_items
+        if items is None:
+            raise RuntimeError("get_fragment_counts: render() must be called first.")
+        total = sum(h * w for _, _, w, h in items)
+        out = torch.empty(total, dtype=torch.int32, device=_device())
+        _lib.check(self._lib.dtv_view_fragment_counts(self._view, ctypes.c_void_p(out.data_ptr()), _stream_ptr()),
+                   "get_fragment_counts")
+        if self._uniform_size:
+            return out.view(len(items), items[0][3], items[0][2])
+        res, off = [], 0
+        for _, _, w, h in items:
+            res.append(out[off:off + h * w].view(h, w))
+            off += h * w
+        return res
+
+    def get_fragment_ids(self, max_per_pixel):
+        raise RuntimeError("get_fragment_ids: single-view renders only (use Renderer).")
 
 
 class TetRegularizer:

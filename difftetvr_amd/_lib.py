@@ -37,7 +37,16 @@ class DtvRenderParams(ctypes.Structure):
 
 class DtvViewStats(ctypes.Structure):
     _fields_ = [("num_triangles", c_int64), ("num_pairs", c_int64), ("num_tiles", c_int64),
-                ("sort_passes", c_int32), ("width", c_int32), ("height", c_int32)]
+                ("sort_passes", c_int32), ("width", c_int32), ("height", c_int32), ("num_items", c_int32),
+                ("num_pixels", c_int64)]
+
+
+class DtvRenderItem(ctypes.Structure):
+    _fields_ = [("mesh", DtvMesh), ("camera", DtvCamera), ("out_image", c_void_p)]
+
+
+class DtvAdjointItem(ctypes.Structure):
+    _fields_ = [("dimage", c_void_p), ("grad_positions", c_void_p), ("grad_colors", c_void_p)]
 
 
 class DtvAdamGroup(ctypes.Structure):
@@ -59,6 +68,10 @@ SIGNATURES = [
                                   ctypes.POINTER(DtvRenderParams), c_void_p, c_void_p]),
     ("dtv_render_adjoint", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_void_p, c_int32, c_int32, c_void_p,
                                           c_void_p, c_void_p]),
+    ("dtv_render_batch", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvRenderItem), c_int32,
+                                        ctypes.POINTER(DtvRenderParams), c_void_p]),
+    ("dtv_render_adjoint_batch", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvAdjointItem), c_int32, c_int32, c_int32,
+                                                c_void_p]),
     ("dtv_view_fragment_counts", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("dtv_view_export_fragments", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_int32, c_void_p, c_void_p]),
     ("dtv_view_get_stats", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvViewStats)]),
@@ -67,6 +80,7 @@ SIGNATURES = [
     ("dtv_adam_step", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p]),
     ("dtv_tet_regularizer_grad", ctypes.c_int, [ctypes.POINTER(DtvMesh), c_float, c_float, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    ("dtv_mse_loss_grad_batch", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
 ]
 
 _lib = None

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/difftetvr_amd.h"
 #include "dtv_kernels.h"
@@ -135,15 +136,22 @@ struct dtv_ctx {
 
 struct dtv_view {
     dtv_ctx* ctx = nullptr;
-    ViewUniforms hU{};
-    ViewUniforms* pinnedU = nullptr;
+    // the batch of the last render (host copy; the device copy is dItems)
+    std::vector<BatchItem> items;
+    std::vector<int64_t> item_V;
+    BatchItem* pinnedItems = nullptr;
+    int pinned_items_cap = 0;
+    AdjointTarget* pinnedTargets = nullptr;
+    int pinned_targets_cap = 0;
+    hipEvent_t targets_copied = nullptr;   // the last upload out of pinnedTargets
+    bool targets_pending = false;
     BinStats* pinnedStats = nullptr;
-    DevBuf dU, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2, hist, lookback, ranges;
+    DevBuf dItems, dTargets, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2, hist,
+        lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
     // state of the last render
     bool rendered = false;
-    const int32_t* cells = nullptr;
-    int64_t T = 0, V = 0;
+    int64_t total_blocks = 0, total_tiles = 0, total_pixels = 0;
     int64_t num_pairs = 0, num_tris = 0;
     int sort_passes = 0;
     bool result_in_alt = false;
@@ -156,6 +164,7 @@ struct dtv_view {
         (void)hipEventRecord(ev[stage][which], s);
         if (which == 1) ev_recorded[stage] = true;
     }
+    int nitems() const { return (int)items.size(); }
     const uint32_t* sorted_vals() const { return result_in_alt ? vals2.as<uint32_t>() : vals.as<uint32_t>(); }
     PixelState pixel_state() const {
         PixelState ps;
@@ -168,6 +177,220 @@ struct dtv_view {
         return ps;
     }
 };
+
+namespace {
+
+constexpr int kMaxBatchItems = 4096;
+
+bool valid_camera(const dtv_camera& c) {
+    return c.width > 0 && c.height > 0 && c.width <= 16384 && c.height <= 16384 && c.znear > 0.0f &&
+           c.zfar > c.znear && c.fovy > 0.0f && c.fovy < 3.14159f;
+}
+
+// Uniforms of one camera; false for a singular view matrix.
+bool make_uniforms(const dtv_camera& camera, const dtv_render_params& params, ViewUniforms& U) {
+    double u[68];
+    if (!camera_uniforms(camera, u)) return false;
+    for (int i = 0; i < 16; i++) {
+        U.viewProj[i] = (float)u[i];
+        U.invProj[i] = (float)u[32 + i];
+        U.invView[i] = (float)u[48 + i];
+    }
+    U.camPos[0] = (float)u[64];
+    U.camPos[1] = (float)u[65];
+    U.camPos[2] = (float)u[66];
+    U.znear = camera.znear;
+    U.width = camera.width;
+    U.height = camera.height;
+    U.tile_shift = 3;
+    U.tiles_x = (camera.width + 7) / 8;
+    U.tiles_y = (camera.height + 7) / 8;
+    U.attenuation = params.attenuation;
+    U.ert_alpha = params.early_ray_out_alpha;
+    U.use_ert = params.use_early_ray_termination ? 1 : 0;
+    for (int i = 0; i < 4; i++) U.clear[i] = params.clear_color[i];
+    U.ckpt_interval = params.checkpoint_interval > 0 ? params.checkpoint_interval : 64;
+    U.max_ckpt = kMaxCheckpoints;
+    return true;
+}
+
+// Binning, sort and forward render of a batch (one host synchronisation, after the record count).
+int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_render_params* params, hipStream_t s,
+                 const char* fn) {
+    const std::string f(fn);
+    if (!view || !in || !params) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
+    if (n <= 0 || n > kMaxBatchItems) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": 1..4096 items required");
+    const int tile = params->tile_size == 0 ? 8 : params->tile_size;
+    if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_size must be 8");
+    std::vector<BatchItem> items((size_t)n);
+    std::vector<int64_t> item_V((size_t)n);
+    int64_t blocks = 0, tiles = 0, pixels = 0;
+    for (int i = 0; i < n; i++) {
+        const dtv_render_item& it = in[i];
+        const dtv_mesh& m = it.mesh;
+        if (!it.out_image) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
+        if (!valid_camera(it.camera)) {
+            if (it.camera.width <= 0 || it.camera.height <= 0 || it.camera.width > 16384 || it.camera.height > 16384)
+                return fail(DTV_ERR_INVALID_ARGUMENT, f + ": viewport must be 1..16384 pixels per side");
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid fovy / clip planes");
+        }
+        if (m.num_cells < 0 || m.num_vertices < 0 || m.num_cells > (int64_t)(1u << 29))
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid mesh size");
+        if (m.num_cells > 0 && (!m.cells || !m.positions || !m.colors))
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL mesh buffer");
+        if (!aligned16(m.cells) || !aligned16(m.colors) || !aligned16(it.out_image))
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": cells, colors and out_image must be 16-byte aligned");
+        BatchItem& B = items[(size_t)i];
+        if (!make_uniforms(it.camera, *params, B.U)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": singular view matrix");
+        B.cells = m.cells;
+        B.pos = m.positions;
+        B.col = m.colors;
+        B.image = it.out_image;
+        B.T = m.num_cells;
+        B.block0 = (uint32_t)blocks;
+        B.tile0 = (uint32_t)tiles;
+        B.pixel0 = pixels;
+        item_V[(size_t)i] = m.num_vertices;
+        blocks += count_pairs_blocks(m.num_cells);
+        tiles += (int64_t)B.U.tiles_x * B.U.tiles_y;
+        pixels += (int64_t)B.U.width * B.U.height;
+        if (blocks >= (1ll << 31) / 256 || tiles >= (1ll << 31) || pixels >= (1ll << 31))
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": batch too large");
+    }
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    view->rendered = false;
+    if (view->pinned_items_cap < n) {
+        if (view->pinnedItems) DTV_HIP(hipHostFree(view->pinnedItems));
+        view->pinnedItems = nullptr;
+        view->pinned_items_cap = 0;
+        DTV_HIP(hipHostMalloc((void**)&view->pinnedItems, sizeof(BatchItem) * (size_t)n, hipHostMallocDefault));
+        view->pinned_items_cap = n;
+    }
+    std::memcpy(view->pinnedItems, items.data(), sizeof(BatchItem) * (size_t)n);
+    DTV_HIP(view->dItems.ensure(sizeof(BatchItem) * (size_t)n));
+    DTV_HIP(hipMemcpyAsync(view->dItems.p, view->pinnedItems, sizeof(BatchItem) * (size_t)n, hipMemcpyHostToDevice,
+                           s));
+    const BatchItem* dI = view->dItems.as<BatchItem>();
+
+    // 1. count (tile, face) records per tet, 2. scan
+    const int64_t nslots = 256 * (blocks > 0 ? blocks : 1);
+    DTV_HIP(view->counts.ensure(sizeof(uint32_t) * nslots));
+    DTV_HIP(view->offsets.ensure(sizeof(uint32_t) * nslots));
+    DTV_HIP(view->scan_status.ensure(sizeof(uint64_t) * (scan_partitions(nslots) + 1)));
+    DTV_HIP(view->small.ensure(256));
+    BinStats* dstats = view->small.as<BinStats>();
+    uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
+    uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
+    DTV_HIP(view->block_stats.ensure(sizeof(uint32_t) * 4 * (blocks + 1)));
+    view->mark(DTV_STAGE_COUNT_PAIRS, 0, s);
+    launch_count_pairs(dI, n, blocks, view->counts.as<uint32_t>(), view->block_stats.as<uint32_t>(), s);
+    view->mark(DTV_STAGE_COUNT_PAIRS, 1, s);
+    view->mark(DTV_STAGE_SCAN, 0, s);
+    scan_exclusive_u32(view->counts.as<uint32_t>(), view->offsets.as<uint32_t>(), 256 * blocks,
+                       view->scan_status.as<uint64_t>(), scan_counter, scan_total, s);
+    launch_reduce_bin_stats(view->block_stats.as<uint32_t>(), blocks, scan_total, dstats, s);
+    view->mark(DTV_STAGE_SCAN, 1, s);
+    DTV_HIP(hipGetLastError());
+    DTV_HIP(hipMemcpyAsync(view->pinnedStats, dstats, sizeof(BinStats), hipMemcpyDeviceToHost, s));
+    DTV_HIP(hipStreamSynchronize(s));
+    const BinStats st = *view->pinnedStats;
+    if (st.total_pairs >= (1ull << 30)) return fail(DTV_ERR_OUT_OF_MEMORY, f + ": more than 2^30 (tile, face) records");
+    const int64_t K = (int64_t)st.total_pairs;
+    view->num_pairs = K;
+    view->num_tris = st.num_tris;
+
+    // 3. emit, 4. sort, 5. tile ranges
+    DTV_HIP(view->keys.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->vals.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->keys2.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
+    view->mark(DTV_STAGE_EMIT, 0, s);
+    launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
+                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), s);
+    view->mark(DTV_STAGE_EMIT, 1, s);
+    uint32_t mask = 0;
+    const uint32_t varying = st.key_and ^ st.key_or;
+    for (int p = 0; p < 4; p++)
+        if ((varying >> (8 * p)) & 255u) mask |= 1u << p;
+    int tile_bits = 0;
+    while ((1ll << tile_bits) < tiles) tile_bits++;
+    for (int p = 0; 8 * p < tile_bits; p++) mask |= 1u << (4 + p);
+    int npass = 0;
+    for (int p = 0; p < 8; p++) npass += (mask >> p) & 1;
+    DTV_HIP(view->hist.ensure(sizeof(uint32_t) * 8 * 256 + 64));
+    DTV_HIP(view->lookback.ensure(sizeof(uint32_t) * 256 * (size_t)(npass > 0 ? npass : 1) *
+                                  (size_t)(sort_partitions(K) + 1)));
+    view->mark(DTV_STAGE_SORT, 0, s);
+    view->sort_passes = radix_sort_pairs(view->keys.as<uint64_t>(), view->vals.as<uint32_t>(),
+                                         view->keys2.as<uint64_t>(), view->vals2.as<uint32_t>(), K, mask,
+                                         view->hist.as<uint32_t>(), view->lookback.as<uint32_t>(),
+                                         view->hist.as<uint32_t>() + 8 * 256, s);
+    view->mark(DTV_STAGE_SORT, 1, s);
+    view->result_in_alt = (view->sort_passes & 1) != 0;
+    const uint64_t* sorted_keys = view->result_in_alt ? view->keys2.as<uint64_t>() : view->keys.as<uint64_t>();
+    DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
+    view->mark(DTV_STAGE_RANGES, 0, s);
+    launch_tile_ranges(sorted_keys, K, tiles, view->ranges.as<uint2>(), s);
+    view->mark(DTV_STAGE_RANGES, 1, s);
+
+    // 6. render
+    DTV_HIP(view->ray_color.ensure(sizeof(float4) * pixels));
+    DTV_HIP(view->frag_count.ensure(sizeof(uint32_t) * pixels));
+    DTV_HIP(view->last_rec.ensure(sizeof(uint32_t) * pixels));
+    DTV_HIP(view->num_ckpt.ensure(sizeof(uint32_t) * pixels));
+    DTV_HIP(view->ckpt_color.ensure(sizeof(float4) * pixels * kMaxCheckpoints));
+    DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * pixels * kMaxCheckpoints));
+    view->mark(DTV_STAGE_FORWARD, 0, s);
+    launch_render_forward(dI, n, tiles, view->sorted_vals(), view->ranges.as<uint2>(), view->pixel_state(), s);
+    view->mark(DTV_STAGE_FORWARD, 1, s);
+    DTV_HIP(hipGetLastError());
+    view->items.swap(items);
+    view->item_V.swap(item_V);
+    view->total_blocks = blocks;
+    view->total_tiles = tiles;
+    view->total_pixels = pixels;
+    view->rendered = true;
+    return DTV_OK;
+}
+
+// Adjoint of the last render_batch; tg[i] belongs to its item i.
+int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_abs_grad, int32_t adjoint_mode,
+                  hipStream_t s, const char* fn) {
+    const std::string f(fn);
+    if (adjoint_mode != DTV_ADJOINT_EXACT) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": unsupported adjoint_mode");
+    if (n != view->nitems()) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": item count differs from the render");
+    for (int i = 0; i < n; i++) {
+        if (!tg[i].dimage || !tg[i].grad_pos || !tg[i].grad_col) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
+        if (!aligned16(tg[i].dimage)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": dimage must be 16-byte aligned");
+    }
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    if (!view->targets_copied) DTV_HIP(hipEventCreateWithFlags(&view->targets_copied, hipEventDisableTiming));
+    // the pinned staging of the previous call must have been consumed before it is rewritten
+    if (view->targets_pending) DTV_HIP(hipEventSynchronize(view->targets_copied));
+    view->targets_pending = false;
+    if (view->pinned_targets_cap < n) {
+        if (view->pinnedTargets) DTV_HIP(hipHostFree(view->pinnedTargets));
+        view->pinnedTargets = nullptr;
+        view->pinned_targets_cap = 0;
+        DTV_HIP(hipHostMalloc((void**)&view->pinnedTargets, sizeof(AdjointTarget) * (size_t)n, hipHostMallocDefault));
+        view->pinned_targets_cap = n;
+    }
+    std::memcpy(view->pinnedTargets, tg, sizeof(AdjointTarget) * (size_t)n);
+    DTV_HIP(view->dTargets.ensure(sizeof(AdjointTarget) * (size_t)n));
+    DTV_HIP(hipMemcpyAsync(view->dTargets.p, view->pinnedTargets, sizeof(AdjointTarget) * (size_t)n,
+                           hipMemcpyHostToDevice, s));
+    DTV_HIP(hipEventRecord(view->targets_copied, s));
+    view->targets_pending = true;
+    view->mark(DTV_STAGE_ADJOINT, 0, s);
+    launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
+                          view->sorted_vals(), view->ranges.as<uint2>(), view->pixel_state(), use_abs_grad ? 1 : 0,
+                          adjoint_mode, s);
+    view->mark(DTV_STAGE_ADJOINT, 1, s);
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -197,8 +420,7 @@ int dtv_view_create(dtv_ctx* ctx, dtv_view** out_view) {
     DTV_HIP(hipSetDevice(ctx->device));
     dtv_view* v = new dtv_view();
     v->ctx = ctx;
-    hipError_t e = hipHostMalloc((void**)&v->pinnedU, sizeof(ViewUniforms), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&v->pinnedStats, sizeof(BinStats) + 16, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc((void**)&v->pinnedStats, sizeof(BinStats) + 16, hipHostMallocDefault);
     if (e != hipSuccess) {
         delete v;
         return fail(DTV_ERR_HIP, std::string("dtv_view_create: hipHostMalloc: ") + hipGetErrorString(e));
@@ -211,7 +433,7 @@ int dtv_view_destroy(dtv_view* v) {
     if (!v) return DTV_OK;
     (void)hipSetDevice(v->ctx->device);
     (void)hipDeviceSynchronize();
-    DevBuf* bufs[] = {&v->dU, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
+    DevBuf* bufs[] = {&v->dItems, &v->dTargets, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
                       &v->vals, &v->keys2,
                       &v->vals2, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
                       &v->num_ckpt, &v->ckpt_color, &v->ckpt_n};
@@ -219,7 +441,9 @@ int dtv_view_destroy(dtv_view* v) {
     for (int i = 0; i < DTV_NUM_STAGES; i++)
         for (int k = 0; k < 2; k++)
             if (v->ev[i][k]) (void)hipEventDestroy(v->ev[i][k]);
-    if (v->pinnedU) (void)hipHostFree(v->pinnedU);
+    if (v->targets_copied) (void)hipEventDestroy(v->targets_copied);
+    if (v->pinnedItems) (void)hipHostFree(v->pinnedItems);
+    if (v->pinnedTargets) (void)hipHostFree(v->pinnedTargets);
     if (v->pinnedStats) (void)hipHostFree(v->pinnedStats);
     delete v;
     return DTV_OK;
@@ -237,134 +461,16 @@ int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, c
                float* out_image, void* stream) {
     if (!view || !mesh || !camera || !params || !out_image)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: NULL argument");
-    if (camera->width <= 0 || camera->height <= 0 || camera->width > 16384 || camera->height > 16384)
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: viewport must be 1..16384 pixels per side");
-    if (!(camera->znear > 0.0f) || !(camera->zfar > camera->znear) || !(camera->fovy > 0.0f) ||
-        !(camera->fovy < 3.14159f))
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: invalid fovy / clip planes");
-    if (mesh->num_cells < 0 || mesh->num_vertices < 0 || mesh->num_cells > (int64_t)(1u << 29))
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: invalid mesh size");
-    if (mesh->num_cells > 0 && (!mesh->cells || !mesh->positions || !mesh->colors))
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: NULL mesh buffer");
-    if (!aligned16(mesh->cells) || !aligned16(mesh->colors) || !aligned16(out_image))
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: cells, colors and out_image must be 16-byte aligned");
-    const int tile = params->tile_size == 0 ? 8 : params->tile_size;
-    if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: tile_size must be 8");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    DTV_HIP(hipSetDevice(view->ctx->device));
+    dtv_render_item it;
+    it.mesh = *mesh;
+    it.camera = *camera;
+    it.out_image = out_image;
+    return render_batch(view, &it, 1, params, reinterpret_cast<hipStream_t>(stream), "dtv_render");
+}
 
-    double u[68];
-    if (!camera_uniforms(*camera, u)) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render: singular view matrix");
-    ViewUniforms& U = view->hU;
-    for (int i = 0; i < 16; i++) {
-        U.viewProj[i] = (float)u[i];
-        U.invProj[i] = (float)u[32 + i];
-        U.invView[i] = (float)u[48 + i];
-    }
-    U.camPos[0] = (float)u[64];
-    U.camPos[1] = (float)u[65];
-    U.camPos[2] = (float)u[66];
-    U.znear = camera->znear;
-    U.width = camera->width;
-    U.height = camera->height;
-    U.tile_shift = 3;
-    U.tiles_x = (camera->width + tile - 1) / tile;
-    U.tiles_y = (camera->height + tile - 1) / tile;
-    U.attenuation = params->attenuation;
-    U.ert_alpha = params->early_ray_out_alpha;
-    U.use_ert = params->use_early_ray_termination ? 1 : 0;
-    for (int i = 0; i < 4; i++) U.clear[i] = params->clear_color[i];
-    U.ckpt_interval = params->checkpoint_interval > 0 ? params->checkpoint_interval : 64;
-    U.max_ckpt = kMaxCheckpoints;
-
-    const int64_t T = mesh->num_cells;
-    const int64_t P = (int64_t)U.width * U.height;
-    const int64_t ntiles = (int64_t)U.tiles_x * U.tiles_y;
-    DTV_HIP(view->dU.ensure(sizeof(ViewUniforms)));
-    *view->pinnedU = U;
-    DTV_HIP(hipMemcpyAsync(view->dU.p, view->pinnedU, sizeof(ViewUniforms), hipMemcpyHostToDevice, s));
-    const ViewUniforms* dU = view->dU.as<ViewUniforms>();
-
-    // 1. count (tile, face) records per face, 2. scan
-    const int64_t nslots = 4 * (T > 0 ? T : 1);
-    DTV_HIP(view->counts.ensure(sizeof(uint32_t) * nslots));
-    DTV_HIP(view->offsets.ensure(sizeof(uint32_t) * nslots));
-    DTV_HIP(view->scan_status.ensure(sizeof(uint64_t) * (scan_partitions(nslots) + 1)));
-    DTV_HIP(view->small.ensure(256));
-    BinStats* dstats = view->small.as<BinStats>();
-    uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
-    uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
-    DTV_HIP(view->block_stats.ensure(sizeof(uint32_t) * 4 * (count_pairs_blocks(T) + 1)));
-    view->mark(DTV_STAGE_COUNT_PAIRS, 0, s);
-    launch_count_pairs(dU, mesh->cells, mesh->positions, T, view->counts.as<uint32_t>(),
-                       view->block_stats.as<uint32_t>(), s);
-    view->mark(DTV_STAGE_COUNT_PAIRS, 1, s);
-    view->mark(DTV_STAGE_SCAN, 0, s);
-    scan_exclusive_u32(view->counts.as<uint32_t>(), view->offsets.as<uint32_t>(), 4 * T,
-                       view->scan_status.as<uint64_t>(), scan_counter, scan_total, s);
-    launch_reduce_bin_stats(view->block_stats.as<uint32_t>(), T, scan_total, dstats, s);
-    view->mark(DTV_STAGE_SCAN, 1, s);
-    DTV_HIP(hipGetLastError());
-    DTV_HIP(hipMemcpyAsync(view->pinnedStats, dstats, sizeof(BinStats), hipMemcpyDeviceToHost, s));
-    DTV_HIP(hipStreamSynchronize(s));
-    const BinStats st = *view->pinnedStats;
-    if (st.total_pairs >= (1ull << 30))
-        return fail(DTV_ERR_OUT_OF_MEMORY, "dtv_render: more than 2^30 (tile, face) records");
-    const int64_t K = (int64_t)st.total_pairs;
-    view->num_pairs = K;
-    view->num_tris = st.num_tris;
-
-    // 3. emit, 4. sort, 5. tile ranges
-    DTV_HIP(view->keys.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
-    DTV_HIP(view->vals.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
-    DTV_HIP(view->keys2.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
-    DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
-    view->mark(DTV_STAGE_EMIT, 0, s);
-    launch_emit_pairs(dU, mesh->cells, mesh->positions, T, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
-                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), s);
-    view->mark(DTV_STAGE_EMIT, 1, s);
-    uint32_t mask = 0;
-    const uint32_t varying = st.key_and ^ st.key_or;
-    for (int p = 0; p < 4; p++)
-        if ((varying >> (8 * p)) & 255u) mask |= 1u << p;
-    int tile_bits = 0;
-    while ((1ll << tile_bits) < ntiles) tile_bits++;
-    for (int p = 0; 8 * p < tile_bits; p++) mask |= 1u << (4 + p);
-    int npass = 0;
-    for (int p = 0; p < 8; p++) npass += (mask >> p) & 1;
-    DTV_HIP(view->hist.ensure(sizeof(uint32_t) * 8 * 256 + 64));
-    DTV_HIP(view->lookback.ensure(sizeof(uint32_t) * 256 * (size_t)(npass > 0 ? npass : 1) *
-                                  (size_t)(sort_partitions(K) + 1)));
-    view->mark(DTV_STAGE_SORT, 0, s);
-    view->sort_passes = radix_sort_pairs(view->keys.as<uint64_t>(), view->vals.as<uint32_t>(),
-                                         view->keys2.as<uint64_t>(), view->vals2.as<uint32_t>(), K, mask,
-                                         view->hist.as<uint32_t>(), view->lookback.as<uint32_t>(),
-                                         view->hist.as<uint32_t>() + 8 * 256, s);
-    view->mark(DTV_STAGE_SORT, 1, s);
-    view->result_in_alt = (view->sort_passes & 1) != 0;
-    const uint64_t* sorted_keys = view->result_in_alt ? view->keys2.as<uint64_t>() : view->keys.as<uint64_t>();
-    DTV_HIP(view->ranges.ensure(sizeof(uint2) * ntiles));
-    view->mark(DTV_STAGE_RANGES, 0, s);
-    launch_tile_ranges(sorted_keys, K, ntiles, view->ranges.as<uint2>(), s);
-    view->mark(DTV_STAGE_RANGES, 1, s);
-
-    // 6. render
-    DTV_HIP(view->ray_color.ensure(sizeof(float4) * P));
-    DTV_HIP(view->frag_count.ensure(sizeof(uint32_t) * P));
-    DTV_HIP(view->last_rec.ensure(sizeof(uint32_t) * P));
-    DTV_HIP(view->num_ckpt.ensure(sizeof(uint32_t) * P));
-    DTV_HIP(view->ckpt_color.ensure(sizeof(float4) * P * kMaxCheckpoints));
-    DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * P * kMaxCheckpoints));
-    view->mark(DTV_STAGE_FORWARD, 0, s);
-    launch_render_forward(dU, U, mesh->cells, mesh->positions, mesh->colors, view->sorted_vals(),
-                          view->ranges.as<uint2>(), out_image, view->pixel_state(), s);
-    view->mark(DTV_STAGE_FORWARD, 1, s);
-    DTV_HIP(hipGetLastError());
-    view->rendered = true;
-    view->cells = mesh->cells;
-    view->T = T;
-    view->V = mesh->num_vertices;
-    return DTV_OK;
+int dtv_render_batch(dtv_view* view, const dtv_render_item* items, int32_t num_items,
+                     const dtv_render_params* params, void* stream) {
+    return render_batch(view, items, num_items, params, reinterpret_cast<hipStream_t>(stream), "dtv_render_batch");
 }
 
 int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage, int32_t use_abs_grad,
@@ -372,20 +478,31 @@ int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage
     if (!view || !mesh || !dimage || !grad_positions || !grad_colors)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: NULL argument");
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_render_adjoint: view has not been rendered");
-    if (mesh->cells != view->cells || mesh->num_cells != view->T || mesh->num_vertices != view->V)
+    if (view->nitems() != 1 || mesh->cells != view->items[0].cells || mesh->num_cells != view->items[0].T ||
+        mesh->num_vertices != view->item_V[0])
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: mesh differs from the rendered one");
-    if (adjoint_mode != DTV_ADJOINT_EXACT)
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: unsupported adjoint_mode");
-    if (!aligned16(dimage)) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: dimage must be 16-byte aligned");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    DTV_HIP(hipSetDevice(view->ctx->device));
-    view->mark(DTV_STAGE_ADJOINT, 0, s);
-    launch_render_adjoint(view->dU.as<ViewUniforms>(), view->hU, mesh->cells, mesh->positions, mesh->colors,
-                          view->sorted_vals(), view->ranges.as<uint2>(), dimage, view->pixel_state(),
-                          use_abs_grad ? 1 : 0, adjoint_mode, grad_positions, grad_colors, s);
-    view->mark(DTV_STAGE_ADJOINT, 1, s);
-    DTV_HIP(hipGetLastError());
-    return DTV_OK;
+    AdjointTarget t;
+    t.dimage = dimage;
+    t.grad_pos = grad_positions;
+    t.grad_col = grad_colors;
+    return adjoint_batch(view, &t, 1, use_abs_grad, adjoint_mode, reinterpret_cast<hipStream_t>(stream),
+                         "dtv_render_adjoint");
+}
+
+int dtv_render_adjoint_batch(dtv_view* view, const dtv_adjoint_item* items, int32_t num_items, int32_t use_abs_grad,
+                             int32_t adjoint_mode, void* stream) {
+    if (!view || !items) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint_batch: NULL argument");
+    if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_render_adjoint_batch: view has not been rendered");
+    if (num_items <= 0 || num_items > kMaxBatchItems)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint_batch: item count differs from the render");
+    std::vector<AdjointTarget> tg((size_t)num_items);
+    for (int i = 0; i < num_items; i++) {
+        tg[(size_t)i].dimage = items[i].dimage;
+        tg[(size_t)i].grad_pos = items[i].grad_positions;
+        tg[(size_t)i].grad_col = items[i].grad_colors;
+    }
+    return adjoint_batch(view, tg.data(), num_items, use_abs_grad, adjoint_mode, reinterpret_cast<hipStream_t>(stream),
+                         "dtv_render_adjoint_batch");
 }
 
 int dtv_view_fragment_counts(dtv_view* view, int32_t* out_counts, void* stream) {
@@ -393,8 +510,8 @@ int dtv_view_fragment_counts(dtv_view* view, int32_t* out_counts, void* stream) 
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_view_fragment_counts: view has not been rendered");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     DTV_HIP(hipSetDevice(view->ctx->device));
-    const int64_t P = (int64_t)view->hU.width * view->hU.height;
-    DTV_HIP(hipMemcpyAsync(out_counts, view->frag_count.p, sizeof(uint32_t) * P, hipMemcpyDeviceToDevice, s));
+    DTV_HIP(hipMemcpyAsync(out_counts, view->frag_count.p, sizeof(uint32_t) * view->total_pixels,
+                           hipMemcpyDeviceToDevice, s));
     return DTV_OK;
 }
 
@@ -403,11 +520,11 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
     if (!view || !mesh || !out_ids || max_per_pixel <= 0)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: invalid argument");
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_view_export_fragments: view has not been rendered");
-    if (mesh->cells != view->cells || mesh->num_cells != view->T)
+    if (view->nitems() != 1 || mesh->cells != view->items[0].cells || mesh->num_cells != view->items[0].T)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: mesh differs from the rendered one");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     DTV_HIP(hipSetDevice(view->ctx->device));
-    launch_export_fragments(view->dU.as<ViewUniforms>(), view->hU, mesh->cells, mesh->positions, view->sorted_vals(),
+    launch_export_fragments(view->dItems.as<BatchItem>(), view->nitems(), view->total_tiles, view->sorted_vals(),
                             view->ranges.as<uint2>(), view->pixel_state(), max_per_pixel, out_ids, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;
@@ -417,10 +534,12 @@ int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out) {
     if (!view || !out) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_get_stats: NULL argument");
     out->num_triangles = view->num_tris;
     out->num_pairs = view->num_pairs;
-    out->num_tiles = (int64_t)view->hU.tiles_x * view->hU.tiles_y;
+    out->num_tiles = view->total_tiles;
     out->sort_passes = view->sort_passes;
-    out->width = view->hU.width;
-    out->height = view->hU.height;
+    out->width = view->rendered ? view->items[0].U.width : 0;
+    out->height = view->rendered ? view->items[0].U.height : 0;
+    out->num_items = view->rendered ? view->nitems() : 0;
+    out->num_pixels = view->total_pixels;
     return DTV_OK;
 }
 
@@ -496,7 +615,17 @@ int dtv_tet_regularizer_grad(const dtv_mesh* mesh, float lambda, float beta, flo
 int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,
                       void* stream) {
     if (!image || !target || !dimage || n <= 0) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_mse_loss_grad: invalid argument");
-    launch_mse(image, target, n, dimage, loss_out, reinterpret_cast<hipStream_t>(stream));
+    launch_mse(image, target, n, n, dimage, loss_out, reinterpret_cast<hipStream_t>(stream));
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
+}
+
+int dtv_mse_loss_grad_batch(const float* image, const float* target, int64_t n_per_item, int32_t num_items,
+                            float* dimage, float* loss_out, void* stream) {
+    if (!image || !target || !dimage || n_per_item <= 0 || num_items <= 0)
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_mse_loss_grad_batch: invalid argument");
+    launch_mse(image, target, n_per_item * num_items, n_per_item, dimage, loss_out,
+               reinterpret_cast<hipStream_t>(stream));
     DTV_HIP(hipGetLastError());
     return DTV_OK;
 }

@@ -47,24 +47,23 @@ __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_count_pairs(const ViewUniforms* __restrict__ pU,
-                                                      const int32_t* __restrict__ cells,
-                                                      const float* __restrict__ pos, int64_t T,
-                                                      uint4* __restrict__ counts, uint4* __restrict__ block_stats) {
-    const ViewUniforms& U = *pU;
-    const int64_t tet = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    uint32_t c[4] = {0, 0, 0, 0};
-    uint32_t ntri = 0, kand = 0xFFFFFFFFu, kor = 0u;
-    if (tet < T) {
+// One thread per tet slot of the batch's block grid; counts[slot] = records of the tet (all its faces).
+__global__ __launch_bounds__(256) void k_count_pairs(const BatchItem* __restrict__ items, int nitems,
+                                                      uint32_t* __restrict__ counts, uint4* __restrict__ block_stats) {
+    const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
+    const ViewUniforms& U = B.U;
+    const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
+    uint32_t c = 0, ntri = 0, kand = 0xFFFFFFFFu, kor = 0u;
+    if (tet < B.T) {
         ProjectedTet PT;
-        project_tet(U, cells, pos, tet, PT);
+        project_tet(U, B.cells, B.pos, tet, PT);
         if (!PT.culled) {
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 if (!(PT.visible & (1u << f))) continue;
                 ntri++;
                 FaceBins b = face_bins(U, PT, f);
-                c[f] = b.count;
+                c += b.count;
                 if (b.count) {
                     uint32_t k = face_key(PT, f);
                     kand &= k;
@@ -72,8 +71,8 @@ __global__ __launch_bounds__(256) void k_count_pairs(const ViewUniforms* __restr
                 }
             }
         }
-        counts[tet] = make_uint4(c[0], c[1], c[2], c[3]);
     }
+    counts[(int64_t)blockIdx.x * 256 + threadIdx.x] = c;
     // Per-block partials (no global atomics: one address hit by every wave serialises at the memory side).
     __shared__ uint4 s_part[4];
     const uint32_t wtri = wave_sum_u32(ntri);
@@ -121,35 +120,38 @@ __global__ __launch_bounds__(256) void k_reduce_bin_stats(const uint4* __restric
     }
 }
 
-__global__ __launch_bounds__(256) void k_emit_pairs(const ViewUniforms* __restrict__ pU,
-                                                     const int32_t* __restrict__ cells,
-                                                     const float* __restrict__ pos, int64_t T,
+// Same grid as k_count_pairs: writes the tet's records at offsets[slot], faces in order, tiles row-major.
+__global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict__ items, int nitems,
                                                      const uint32_t* __restrict__ offsets,
-                                                     const uint4* __restrict__ counts, uint64_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ vals) {
-    const ViewUniforms& U = *pU;
-    const int64_t tet = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (tet >= T) return;
-    uint4 cc = counts[tet];
-    if ((cc.x | cc.y | cc.z | cc.w) == 0) return;
+                                                     const uint32_t* __restrict__ counts,
+                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
+    const ViewUniforms& U = B.U;
+    const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
+    if (tet >= B.T) return;
+    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t left = counts[slot];
+    if (left == 0) return;
+    uint32_t o = offsets[slot];
     ProjectedTet PT;
-    project_tet(U, cells, pos, tet, PT);
+    project_tet(U, B.cells, B.pos, tet, PT);
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        uint32_t cf = f == 0 ? cc.x : f == 1 ? cc.y : f == 2 ? cc.z : cc.w;
-        if (cf == 0) continue;
+        if (!(PT.visible & (1u << f)) || left == 0) continue;
         FaceBins b = face_bins(U, PT, f);
+        if (b.count == 0) continue;
         const uint64_t k = (uint64_t)face_key(PT, f);
-        const uint32_t o = offsets[4 * tet + f];
         const uint32_t id = (uint32_t)(tet * 4 + f);
         const int bw = b.tx1 - b.tx0 + 1;
         // Bounded by the stored count: the pass recomputes the same bins, but never writes past its slots.
-        const uint32_t n = min(cf, b.count);
+        const uint32_t n = min(left, b.count);
         for (uint32_t i = 0; i < n; i++) {
             const int tx = b.tx0 + (int)(i % (uint32_t)bw), ty = b.ty0 + (int)(i / (uint32_t)bw);
-            keys[o + i] = ((uint64_t)(ty * U.tiles_x + tx) << 32) | k;
+            keys[o + i] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
             vals[o + i] = id;
         }
+        o += n;
+        left -= n;
     }
 }
 
@@ -163,25 +165,24 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint
 
 int64_t count_pairs_blocks(int64_t T) { return T > 0 ? (T + 255) / 256 : 0; }
 
-void launch_count_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T, uint32_t* counts,
-                        uint32_t* block_stats, hipStream_t s) {
-    if (T <= 0) return;
-    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)count_pairs_blocks(T)), dim3(256), 0, s, dU, cells, pos, T,
-                       reinterpret_cast<uint4*>(counts), reinterpret_cast<uint4*>(block_stats));
+void launch_count_pairs(const BatchItem* items, int nitems, int64_t nblocks, uint32_t* counts, uint32_t* block_stats,
+                        hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, counts,
+                       reinterpret_cast<uint4*>(block_stats));
 }
 
-void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t T, const uint64_t* scan_total, BinStats* stats,
-                             hipStream_t s) {
+void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const uint64_t* scan_total,
+                             BinStats* stats, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_bin_stats, dim3(1), dim3(256), 0, s, reinterpret_cast<const uint4*>(block_stats),
-                       count_pairs_blocks(T), scan_total, stats);
+                       nblocks, scan_total, stats);
 }
 
-void launch_emit_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T,
-                       const uint32_t* offsets, const uint32_t* counts, uint64_t* keys, uint32_t* vals,
-                       hipStream_t s) {
-    if (T <= 0) return;
-    hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, dU, cells, pos, T, offsets,
-                       reinterpret_cast<const uint4*>(counts), keys, vals);
+void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, offsets, counts, keys,
+                       vals);
 }
 
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s) {

@@ -44,6 +44,46 @@ struct ViewUniforms {
     int32_t max_ckpt;
 };
 
+// One render of a batch: a mesh seen through one camera.  A batch is a device array of these, ordered so that
+// block0, tile0 and pixel0 ascend; its tiles are numbered globally (tile0 + local tile) in the record keys, and
+// its per-pixel state is one concatenation (pixel0 + local pixel).
+struct BatchItem {
+    ViewUniforms U;
+    const int32_t* cells;
+    const float* pos;
+    const float* col;
+    float* image;        // forward output [height * width * 4]
+    int64_t T;           // tets
+    int64_t pixel0;      // first pixel in the batch's per-pixel state
+    uint32_t block0;     // first 256-tet block in the batch's count / emit grids
+    uint32_t tile0;      // first tile in the batch's tile numbering
+};
+
+// Adjoint inputs and gradient targets of one batch item (targets of items sharing a mesh may alias).
+struct AdjointTarget {
+    const float* dimage; // [height * width * 4]
+    float* grad_pos;     // [V * 3], accumulated
+    float* grad_col;     // [V * 4], accumulated
+};
+
+// Item owning global block / tile `b` (the last item whose first block / tile <= b).  Uniform per workgroup.
+__device__ __forceinline__ int item_of_block(const BatchItem* __restrict__ items, int n, uint32_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int item_of_tile(const BatchItem* __restrict__ items, int n, uint32_t t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct float3_ {
     float x, y, z;
 };

@@ -25,17 +25,20 @@ struct BinStats {
     uint32_t pad;
 };
 int64_t count_pairs_blocks(int64_t T);
-void launch_count_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T, uint32_t* counts,
-                        uint32_t* block_stats, hipStream_t s);
+// Grid of nblocks 256-tet blocks over all items (item i owns blocks [block0_i, block0_{i+1})).  counts[256 * nblocks]
+// gets the (tile, face) record count of every tet slot (0 for padding); block_stats the per-block partials.
+void launch_count_pairs(const BatchItem* items, int nitems, int64_t nblocks, uint32_t* counts, uint32_t* block_stats,
+                        hipStream_t s);
 // stats.total_pairs = *scan_total; the other fields reduce the per-block partials of launch_count_pairs.
-void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t T, const uint64_t* scan_total, BinStats* stats,
-                             hipStream_t s);
-void launch_emit_pairs(const ViewUniforms* dU, const int32_t* cells, const float* pos, int64_t T,
-                       const uint32_t* offsets, const uint32_t* counts, uint64_t* keys, uint32_t* vals,
-                       hipStream_t s);
+void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const uint64_t* scan_total,
+                             BinStats* stats, hipStream_t s);
+void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, hipStream_t s);
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s);
 
 // render.hip
+// Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's
+// checkpoint planes start at max_ckpt * pixel0_i.
 struct PixelState {
     float4* ray_color;      // [P] composite before the clear-colour blend
     uint32_t* frag_count;   // [P] fragments composited
@@ -44,15 +47,13 @@ struct PixelState {
     float4* ckpt_color;     // [max_ckpt][P]
     uint32_t* ckpt_n;       // [max_ckpt][P]
 };
-void launch_render_forward(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
-                           const float* col, const uint32_t* rec_vals, const uint2* ranges, float* out_image,
-                           PixelState ps, hipStream_t s);
-void launch_render_adjoint(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
-                           const float* col, const uint32_t* rec_vals, const uint2* ranges, const float* dimage,
-                           PixelState ps, int use_abs, int mode, float* grad_pos, float* grad_col, hipStream_t s);
-void launch_export_fragments(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
-                             const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int max_per_pixel,
-                             int32_t* out_ids, hipStream_t s);
+void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+                           const uint2* ranges, PixelState ps, hipStream_t s);
+void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
+                           const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int use_abs, int mode,
+                           hipStream_t s);
+void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+                             const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 
 // optim.hip
 struct AdamGroupDev {
@@ -71,7 +72,8 @@ struct AdamGroupDev {
     int32_t zero_grad;
 };
 void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s);
-void launch_mse(const float* img, const float* tgt, int64_t n, float* dimg, float* loss, hipStream_t s);
+void launch_mse(const float* img, const float* tgt, int64_t n, int64_t denom, float* dimg, float* loss,
+                hipStream_t s);
 
 // regularizer.hip
 void launch_tet_regularizer(const int32_t* cells, const float* pos, int64_t T, float lambda, float beta,

@@ -75,10 +75,11 @@ void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s) {
     hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, s, A);
 }
 
-// dimg = 2 (img - tgt) / n ; loss += sum((img - tgt)^2) / n
+// dimg = 2 (img - tgt) / denom ; loss += sum((img - tgt)^2) / denom  (denom = n: the mean; denom = n / items:
+// the sum of the items' means)
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ img, const float* __restrict__ tgt, int64_t n,
-                                              float* __restrict__ dimg, float* __restrict__ loss) {
-    const float inv_n = 1.0f / (float)n;
+                                              int64_t denom, float* __restrict__ dimg, float* __restrict__ loss) {
+    const float inv_n = 1.0f / (float)denom;
     float acc = 0.f;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float d = img[i] - tgt[i];
@@ -95,12 +96,13 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ img, cons
     }
 }
 
-void launch_mse(const float* img, const float* tgt, int64_t n, float* dimg, float* loss, hipStream_t s) {
+void launch_mse(const float* img, const float* tgt, int64_t n, int64_t denom, float* dimg, float* loss,
+                hipStream_t s) {
     if (loss != nullptr) fill_u32(reinterpret_cast<uint32_t*>(loss), 0u, 1, s);
     int64_t blocks = (n + 255) / 256;
     if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_mse, dim3((unsigned)blocks), dim3(256), 0, s, img, tgt, n, dimg, loss);
+    hipLaunchKernelGGL(k_mse, dim3((unsigned)blocks), dim3(256), 0, s, img, tgt, n, denom, dimg, loss);
 }
 
 }  // namespace dtv

@@ -52,17 +52,20 @@ __device__ __forceinline__ uint32_t build_fragment_list(FragListLds& F, uint32_t
     return off;
 }
 
-__global__ __launch_bounds__(64) void k_render_forward(const ViewUniforms* __restrict__ pU,
-                                                        const int32_t* __restrict__ cells,
-                                                        const float* __restrict__ pos,
-                                                        const float* __restrict__ col,
+__global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
                                                         const uint32_t* __restrict__ rec_vals,
-                                                        const uint2* __restrict__ ranges,
-                                                        float4* __restrict__ out_image, PixelState ps) {
+                                                        const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ RecordLds L;
     __shared__ FragListLds F;
-    const ViewUniforms& U = *pU;
-    const int tile = tile_of_block(U);
+    const uint32_t gtile = global_tile_of_block();
+    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    const ViewUniforms& U = B.U;
+    const int32_t* __restrict__ cells = B.cells;
+    const float* __restrict__ pos = B.pos;
+    const float* __restrict__ col = B.col;
+    float4* __restrict__ out_image = reinterpret_cast<float4*>(B.image);
+    const PixelState ps = item_pixel_state(ps_all, B);
+    const int tile = (int)(gtile - B.tile0);
     const int tx = tile % U.tiles_x, ty = tile / U.tiles_x;
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const ViewUniforms* __res
         F.rd[2][lane] = rd.z;
     }
     const float att = U.attenuation;
-    const uint2 rg = ranges[tile];
+    const uint2 rg = ranges[gtile];
     const int P = U.width * U.height;
     const int pix = valid ? pixel_index(U, x, y) : 0;
 
@@ -155,21 +158,27 @@ __global__ __launch_bounds__(64) void k_render_forward(const ViewUniforms* __res
     }
 }
 
-__global__ __launch_bounds__(64) void k_render_adjoint(const ViewUniforms* __restrict__ pU,
-                                                        const int32_t* __restrict__ cells,
-                                                        const float* __restrict__ pos,
-                                                        const float* __restrict__ col,
+__global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restrict__ items,
+                                                        const AdjointTarget* __restrict__ targets, int nitems,
                                                         const uint32_t* __restrict__ rec_vals,
-                                                        const uint2* __restrict__ ranges,
-                                                        const float4* __restrict__ dimage, PixelState ps,
-                                                        int use_abs, float* __restrict__ grad_pos,
-                                                        float* __restrict__ grad_col) {
+                                                        const uint2* __restrict__ ranges, PixelState ps_all,
+                                                        int use_abs) {
     __shared__ RecordLds L;
     __shared__ FragListLds F;
     __shared__ float s_w[kRoundAdj][29];  // per fragment: 4 vertices x (3 position + 4 colour), padded row
     __shared__ uint8_t s_hit[kRoundAdj];
-    const ViewUniforms& U = *pU;
-    const int tile = tile_of_block(U);
+    const uint32_t gtile = global_tile_of_block();
+    const int item = item_of_tile(items, nitems, gtile);
+    const BatchItem& B = items[item];
+    const ViewUniforms& U = B.U;
+    const int32_t* __restrict__ cells = B.cells;
+    const float* __restrict__ pos = B.pos;
+    const float* __restrict__ col = B.col;
+    const float4* __restrict__ dimage = reinterpret_cast<const float4*>(targets[item].dimage);
+    float* __restrict__ grad_pos = targets[item].grad_pos;
+    float* __restrict__ grad_col = targets[item].grad_col;
+    const PixelState ps = item_pixel_state(ps_all, B);
+    const int tile = (int)(gtile - B.tile0);
     const int tx = tile % U.tiles_x, ty = tile / U.tiles_x;
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
@@ -184,7 +193,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const ViewUniforms* __res
         F.rd[2][lane] = rd.z;
     }
     const float att = U.attenuation;
-    const uint2 rg = ranges[tile];
+    const uint2 rg = ranges[gtile];
     const int P = U.width * U.height;
     const int pix = valid ? pixel_index(U, x, y) : 0;
 
@@ -358,22 +367,26 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const ViewUniforms* __res
 }
 
 // Ordered fragment export (tet*4 + face per covered record), up to max_per_pixel per pixel.
-__global__ __launch_bounds__(64) void k_export_fragments(const ViewUniforms* __restrict__ pU,
-                                                          const int32_t* __restrict__ cells,
-                                                          const float* __restrict__ pos,
+__global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __restrict__ items, int nitems,
                                                           const uint32_t* __restrict__ rec_vals,
-                                                          const uint2* __restrict__ ranges, PixelState ps,
-                                                          int max_per_pixel, int32_t* __restrict__ out_ids) {
+                                                          const uint2* __restrict__ ranges, PixelState ps_all,
+                                                          int max_per_pixel, int32_t* __restrict__ out_ids_all) {
     __shared__ RecordLds L;
-    const ViewUniforms& U = *pU;
-    const int tile = tile_of_block(U);
+    const uint32_t gtile = global_tile_of_block();
+    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    const ViewUniforms& U = B.U;
+    const int32_t* __restrict__ cells = B.cells;
+    const float* __restrict__ pos = B.pos;
+    const PixelState ps = item_pixel_state(ps_all, B);
+    int32_t* __restrict__ out_ids = out_ids_all + B.pixel0 * max_per_pixel;
+    const int tile = (int)(gtile - B.tile0);
     const int tx = tile % U.tiles_x, ty = tile / U.tiles_x;
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
     const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
     const bool valid = x < U.width && y < U.height;
     const int ox = lx * 256, oy = ly * 256;
-    const uint2 rg = ranges[tile];
+    const uint2 rg = ranges[gtile];
     const int pix = valid ? pixel_index(U, x, y) : 0;
     const uint32_t last = valid ? ps.last_rec[pix] : 0u;
     int k = 0;
@@ -392,29 +405,26 @@ __global__ __launch_bounds__(64) void k_export_fragments(const ViewUniforms* __r
         for (; k < max_per_pixel; k++) out_ids[(int64_t)pix * max_per_pixel + k] = -1;
 }
 
-void launch_render_forward(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
-                           const float* col, const uint32_t* rec_vals, const uint2* ranges, float* out_image,
-                           PixelState ps, hipStream_t s) {
-    const unsigned ntiles = (unsigned)(hU.tiles_x * hU.tiles_y);
-    hipLaunchKernelGGL(k_render_forward, dim3(ntiles), dim3(64), 0, s, dU, cells, pos, col, rec_vals, ranges,
-                       reinterpret_cast<float4*>(out_image), ps);
+void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+                           const uint2* ranges, PixelState ps, hipStream_t s) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_render_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, rec_vals, ranges, ps);
 }
 
-void launch_render_adjoint(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
-                           const float* col, const uint32_t* rec_vals, const uint2* ranges, const float* dimage,
-                           PixelState ps, int use_abs, int mode, float* grad_pos, float* grad_col, hipStream_t s) {
+void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
+                           const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int use_abs, int mode,
+                           hipStream_t s) {
     (void)mode;
-    const unsigned ntiles = (unsigned)(hU.tiles_x * hU.tiles_y);
-    hipLaunchKernelGGL(k_render_adjoint, dim3(ntiles), dim3(64), 0, s, dU, cells, pos, col, rec_vals, ranges,
-                       reinterpret_cast<const float4*>(dimage), ps, use_abs, grad_pos, grad_col);
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_render_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, rec_vals,
+                       ranges, ps, use_abs);
 }
 
-void launch_export_fragments(const ViewUniforms* dU, const ViewUniforms& hU, const int32_t* cells, const float* pos,
-                             const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int max_per_pixel,
-                             int32_t* out_ids, hipStream_t s) {
-    const unsigned ntiles = (unsigned)(hU.tiles_x * hU.tiles_y);
-    hipLaunchKernelGGL(k_export_fragments, dim3(ntiles), dim3(64), 0, s, dU, cells, pos, rec_vals, ranges, ps,
-                       max_per_pixel, out_ids);
+void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+                             const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_export_fragments, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, rec_vals, ranges,
+                       ps, max_per_pixel, out_ids);
 }
 
 }  // namespace dtv

@@ -242,7 +242,20 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
 
 __device__ __forceinline__ int pixel_index(const ViewUniforms& U, int x, int y) { return y * U.width + x; }
 
-__device__ __forceinline__ int tile_of_block(const ViewUniforms& U) { return blockIdx.x; }
+// Batch tile processed by this workgroup.
+__device__ __forceinline__ uint32_t global_tile_of_block() { return blockIdx.x; }
+
+// Per-pixel state of one batch item: its slice of the batch's concatenated arrays.
+__device__ __forceinline__ PixelState item_pixel_state(const PixelState& all, const BatchItem& B) {
+    PixelState ps;
+    ps.ray_color = all.ray_color + B.pixel0;
+    ps.frag_count = all.frag_count + B.pixel0;
+    ps.last_rec = all.last_rec + B.pixel0;
+    ps.num_ckpt = all.num_ckpt + B.pixel0;
+    ps.ckpt_color = all.ckpt_color + (int64_t)B.U.max_ckpt * B.pixel0;
+    ps.ckpt_n = all.ckpt_n + (int64_t)B.U.max_ckpt * B.pixel0;
+    return ps;
+}
 
 // ---------------------------------------------------------------------------------------------------------
 // Exact adjoint of one fragment given dL/dF.  Writes per-local-vertex gradients gp[v][xyz], gc[v][rgba].

@@ -68,3 +68,21 @@ def mse_loss_grad(image, target):
                                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
                "dtv_mse_loss_grad")
     return loss, dimg
+
+
+def mse_loss_grad_views(images, targets):
+    """Sum over views of torch.nn.MSELoss(images[i], targets[i]) for [N, ...] stacks, and its image adjoint,
+    in one HIP launch (the multi-view loss of a batched step)."""
+    images = images.contiguous()
+    targets = targets.contiguous()
+    if images.shape != targets.shape or images.dim() < 2:
+        raise RuntimeError("mse_loss_grad_views: [N, ...] images and targets of one shape expected.")
+    dimg = torch.empty_like(images)
+    loss = torch.empty((), dtype=torch.float32, device=images.device)
+    n = images.shape[0]
+    _lib.check(_lib.load().dtv_mse_loss_grad_batch(ctypes.c_void_p(images.data_ptr()),
+                                                   ctypes.c_void_p(targets.data_ptr()), images.numel() // n, n,
+                                                   ctypes.c_void_p(dimg.data_ptr()), ctypes.c_void_p(loss.data_ptr()),
+                                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+               "dtv_mse_loss_grad_batch")
+    return loss, dimg

@@ -70,8 +70,25 @@ typedef struct dtv_view_stats {
     int64_t num_pairs;        /* (tile, face) records after binning */
     int64_t num_tiles;
     int32_t sort_passes;      /* radix passes actually run */
-    int32_t width, height;
+    int32_t width, height;    /* viewport of the first item */
+    int32_t num_items;        /* renders in the last batch (1 after dtv_render) */
+    int64_t num_pixels;       /* pixels of all items */
 } dtv_view_stats;
+
+/* One render of a batch: a mesh seen through a camera into out_image [height, width, 4]. */
+typedef struct dtv_render_item {
+    dtv_mesh mesh;
+    dtv_camera camera;
+    float* out_image;
+} dtv_render_item;
+
+/* Adjoint input and gradient targets of one batch item: dimage [height, width, 4]; gradients accumulate (+=) into
+ * grad_positions [V,3] and grad_colors [V,4] (items that share a mesh may share them). */
+typedef struct dtv_adjoint_item {
+    const float* dimage;
+    float* grad_positions;
+    float* grad_colors;
+} dtv_adjoint_item;
 
 /* Adam over one parameter tensor: torch.optim.Adam as used by pytests/train.py:301-319,351, optional colour
  * clamp c -= min(c, 0) (train.py:352-356), optional fused zero_grad for the next step (train.py:342). */
@@ -110,6 +127,19 @@ int dtv_camera_uniforms(const dtv_camera* camera, float* out68);
 int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, const dtv_render_params* params,
                float* out_image, void* stream);
 
+/* Renders num_items (1..4096) (mesh, camera) items as one batch: one binning grid over all their tets, one sort of
+ * all (tile, face) records and one tile-renderer launch over all their tiles, so that many small or uneven views
+ * fill the GPU together.  Each image equals what dtv_render gives for its item alone.  No reference counterpart:
+ * the reference renders one view per call (Module.cpp:912-921); dtv_render is the one-item case.
+ * Synchronises the stream once to size the record buffers. */
+int dtv_render_batch(dtv_view* view, const dtv_render_item* items, int32_t num_items,
+                     const dtv_render_params* params, void* stream);
+
+/* Adjoint of the last dtv_render_batch of this view, items[i] belonging to its render item i (same count).
+ * One launch over all tiles of the batch. */
+int dtv_render_adjoint_batch(dtv_view* view, const dtv_adjoint_item* items, int32_t num_items, int32_t use_abs_grad,
+                             int32_t adjoint_mode, void* stream);
+
 /* Adjoint of the last dtv_render of this view: accumulates (+=) into grad_positions [V,3] and
  * grad_colors [V,4].  Replaces Renderer.render_adjoint (Module.cpp:922-959) ->
  * TetMeshRendererIntersection::renderAdjoint (:709-715).  The mesh must be the one rendered. */
@@ -117,11 +147,13 @@ int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage
                        int32_t adjoint_mode, float* grad_positions, float* grad_colors, void* stream);
 
 /* Per-pixel number of fragments composited by the last render (depth complexity; the reference's
- * SHOW_DEPTH_COMPLEXITY counter, IntersectRasterization.glsl:267-270).  out: device int32 [H*W]. */
+ * SHOW_DEPTH_COMPLEXITY counter, IntersectRasterization.glsl:267-270).  out: device int32 [H*W], or for a batch
+ * the items' [H_i*W_i] one after another. */
 int dtv_view_fragment_counts(dtv_view* view, int32_t* out_counts, void* stream);
 
 /* Per-pixel ordered fragment ids (tet*4 + face), front to back, up to max_per_pixel each, padded with -1.
- * Counterpart of the PPLL export (Module.cpp:960-981).  out: device int32 [H*W*max_per_pixel]. */
+ * Counterpart of the PPLL export (Module.cpp:960-981).  out: device int32 [H*W*max_per_pixel].  Single-item
+ * renders only. */
 int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_per_pixel, int32_t* out_ids,
                               void* stream);
 
@@ -156,6 +188,12 @@ int dtv_tet_regularizer_grad(const dtv_mesh* mesh, float lambda, float beta, flo
  * = mean squared error.  Replaces torch.nn.MSELoss in train.py:325,346 and Loss.glsl:55-95. */
 int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,
                       void* stream);
+
+/* MSE summed over num_items equally sized images laid out one after another (n_per_item floats each):
+ * *loss_out = sum over items of the item's mean squared error, dimage = 2 (image - target) / n_per_item.
+ * The multi-view loss of a batched step (the reference sums per-view losses, train.py:325-346). */
+int dtv_mse_loss_grad_batch(const float* image, const float* target, int64_t n_per_item, int32_t num_items,
+                            float* dimage, float* loss_out, void* stream);
 
 #ifdef __cplusplus
 }
