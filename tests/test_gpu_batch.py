@@ -125,9 +125,10 @@ def test_batch_two_meshes_adjoint_targets(gpu):
     for m, w, e in ((ma, wa, EYES[0]), (mb, wb, EYES[1])):
         uf, ud = oracle.camera_uniforms(view_from(e), FOVY, ZNEAR, ZFAR, W, H)
         fr = oracle.Fragments(w["cells"], w["pos"], uf, ZNEAR, W, H)
-        op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 1.0, ones)
+        # meshes with gradients render with early ray termination (the renderer's default)
+        op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 1.0, ones, use_ert=True)
         dp, _ = oracle.adjoint_exact(fr, w["pos"].astype(np.float64), w["col"].astype(np.float64), ud, 1.0,
-                                     ones.astype(np.float64), dtype=np.float64)
+                                     ones.astype(np.float64), use_ert=True, dtype=np.float64)
         assert rel_l2(m.get_vertex_colors().grad.cpu().numpy(), oc) <= 1e-4
         # position gradients: the FP64 exact gradient, matched as closely as the FP32 oracle matches it
         assert rel_l2(m.get_vertex_positions().grad.cpu().numpy(), dp) <= max(1e-4, 1.5 * rel_l2(op, dp))
@@ -150,3 +151,16 @@ def test_batch_validation(gpu):
         br.render_adjoint(torch.zeros((3, 16, 16, 4), device="cuda"))
     with pytest.raises(RuntimeError):
         br.render_adjoint(torch.zeros((2, 16, 15, 4), device="cuda"))
+
+
+def test_mse_loss_grad_views_matches_torch(gpu):
+    from difftetvr_amd.optim import mse_loss_grad_views
+    g = torch.Generator(device="cuda").manual_seed(3)
+    imgs = torch.rand((5, 17, 23, 4), device="cuda", generator=g).requires_grad_(True)
+    tgts = torch.rand((5, 17, 23, 4), device="cuda", generator=g)
+    loss, dimg = mse_loss_grad_views(imgs.detach(), tgts)
+    ref = sum(torch.nn.functional.mse_loss(imgs[i], tgts[i]) for i in range(5))
+    ref.backward()
+    torch.cuda.synchronize()
+    assert abs(float(loss) - float(ref)) <= 1e-5 * float(ref)
+    assert torch.allclose(dimg, imgs.grad, rtol=1e-6, atol=1e-9)
