@@ -146,7 +146,8 @@ struct dtv_view {
     hipEvent_t targets_copied = nullptr;   // the last upload out of pinnedTargets
     bool targets_pending = false;
     BinStats* pinnedStats = nullptr;
-    DevBuf dItems, dTargets, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2, hist,
+    DevBuf dItems, dTargets, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2, rec_id,
+        recs, hist,
         lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
     // state of the last render
@@ -165,7 +166,7 @@ struct dtv_view {
         if (which == 1) ev_recorded[stage] = true;
     }
     int nitems() const { return (int)items.size(); }
-    const uint32_t* sorted_vals() const { return result_in_alt ? vals2.as<uint32_t>() : vals.as<uint32_t>(); }
+    const uint4* sorted_recs() const { return recs.as<uint4>(); }
     PixelState pixel_state() const {
         PixelState ps;
         ps.ray_color = ray_color.as<float4>();
@@ -304,9 +305,11 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->vals.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->keys2.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->rec_id.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
+    DTV_HIP(view->recs.ensure(sizeof(uint4) * (K > 0 ? K : 1)));
     view->mark(DTV_STAGE_EMIT, 0, s);
     launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
-                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), s);
+                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), view->rec_id.as<uint32_t>(), s);
     view->mark(DTV_STAGE_EMIT, 1, s);
     uint32_t mask = 0;
     const uint32_t varying = st.key_and ^ st.key_or;
@@ -331,6 +334,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
     view->mark(DTV_STAGE_RANGES, 0, s);
     launch_tile_ranges(sorted_keys, K, tiles, view->ranges.as<uint2>(), s);
+    const uint32_t* sorted_idx = view->result_in_alt ? view->vals2.as<uint32_t>() : view->vals.as<uint32_t>();
+    launch_gather_records(dI, n, sorted_keys, sorted_idx, view->rec_id.as<uint32_t>(), K, view->recs.as<uint4>(), s);
     view->mark(DTV_STAGE_RANGES, 1, s);
 
     // 6. render
@@ -341,7 +346,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->ckpt_color.ensure(sizeof(float4) * pixels * kMaxCheckpoints));
     DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * pixels * kMaxCheckpoints));
     view->mark(DTV_STAGE_FORWARD, 0, s);
-    launch_render_forward(dI, n, tiles, view->sorted_vals(), view->ranges.as<uint2>(), view->pixel_state(), s);
+    launch_render_forward(dI, n, tiles, view->sorted_recs(), view->ranges.as<uint2>(), view->pixel_state(), s);
     view->mark(DTV_STAGE_FORWARD, 1, s);
     DTV_HIP(hipGetLastError());
     view->items.swap(items);
@@ -383,7 +388,7 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     view->targets_pending = true;
     view->mark(DTV_STAGE_ADJOINT, 0, s);
     launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
-                          view->sorted_vals(), view->ranges.as<uint2>(), view->pixel_state(), use_abs_grad ? 1 : 0,
+                          view->sorted_recs(), view->ranges.as<uint2>(), view->pixel_state(), use_abs_grad ? 1 : 0,
                           adjoint_mode, s);
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
@@ -435,7 +440,7 @@ int dtv_view_destroy(dtv_view* v) {
     (void)hipDeviceSynchronize();
     DevBuf* bufs[] = {&v->dItems, &v->dTargets, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
                       &v->vals, &v->keys2,
-                      &v->vals2, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
+                      &v->vals2, &v->rec_id, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
                       &v->num_ckpt, &v->ckpt_color, &v->ckpt_n};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < DTV_NUM_STAGES; i++)
@@ -524,7 +529,7 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: mesh differs from the rendered one");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     DTV_HIP(hipSetDevice(view->ctx->device));
-    launch_export_fragments(view->dItems.as<BatchItem>(), view->nitems(), view->total_tiles, view->sorted_vals(),
+    launch_export_fragments(view->dItems.as<BatchItem>(), view->nitems(), view->total_tiles, view->sorted_recs(),
                             view->ranges.as<uint2>(), view->pixel_state(), max_per_pixel, out_ids, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;

@@ -1,34 +1,119 @@
 // Per-view tet projection, front-face culling and screen-tile binning on gfx950.
 //
 // Replaces GenerateTrianglesIntersect.glsl:74-132 (per-tet projection + front faces, one atomic append each),
-// ComputeTriangleDepths.glsl:72-110 (centroid-depth keys) and the rasteriser's fragment generation: each front
-// face becomes one (tile, face) record per screen tile its pixel-centre bounding box touches, keyed by
-// (tile << 32 | depth-key bits).  Counting and emission are two passes over the tets separated by a
-// device-wide scan, so records are written in (tet, face) order and the stable sort keeps that order for
-// equal depth keys.
+// ComputeTriangleDepths.glsl:72-110 (centroid-depth keys) and the rasteriser's fragment generation.  Each front
+// face becomes one (tile, face) record per 8x8 screen tile of its pixel-centre bounding box that the face's edge
+// functions do not exclude (tile_may_cover), keyed by (tile << 32 | depth-key bits).  Counting and emission are
+// two passes over the tets separated by a device-wide scan, so records are written in (tet, face, tile) order
+// and the stable sort keeps that order for equal depth keys.  The sort moves record indices; k_gather_records
+// then lays the sorted records out for the tile renderer as {face id, exact 64-bit coverage mask of the tile}
+// (one thread per record, so the per-pixel edge tests are spread evenly over the GPU).
 #include "dtv_kernels.h"
 #include "dtv_prims.h"
 
 namespace dtv {
 
 struct FaceBins {
-    int32_t tx0, tx1, ty0, ty1;
-    uint32_t count;
+    int32_t tx0, tx1, ty0, ty1;  // tile range of the face's pixel-centre bounding box (empty: tx0 > tx1)
+    ScreenTri S;
 };
 
 __device__ __forceinline__ FaceBins face_bins(const ViewUniforms& U, const ProjectedTet& T, int f) {
-    FaceBins b{0, -1, 0, -1, 0u};
-    ScreenTri S;
-    face_screen(U, T, f, S);
+    FaceBins b;
+    b.tx0 = 0;
+    b.tx1 = -1;
+    b.ty0 = 0;
+    b.ty1 = -1;
+    face_screen(U, T, f, b.S);
     int32_t x0, x1, y0, y1;
-    tri_pixel_bbox(U, S, x0, x1, y0, y1);
+    tri_pixel_bbox(U, b.S, x0, x1, y0, y1);
     if (x0 > x1 || y0 > y1) return b;
-    b.tx0 = x0 >> U.tile_shift;
-    b.tx1 = x1 >> U.tile_shift;
-    b.ty0 = y0 >> U.tile_shift;
-    b.ty1 = y1 >> U.tile_shift;
-    b.count = (uint32_t)((b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1));
+    b.tx0 = x0 >> 3;
+    b.tx1 = x1 >> 3;
+    b.ty0 = y0 >> 3;
+    b.ty1 = y1 >> 3;
     return b;
+}
+
+// Edge e of the oriented screen triangle: E(px, py) = dx (py - Y_a) - dy (px - X_a) in 1/256-pixel units; a
+// pixel centre is inside when every E > 0, or E == 0 on a tie edge (dy < 0, or dy == 0 and dx > 0).
+struct TileEdge {
+    long long C;       // E at the centre of the tile's first pixel
+    long long sx, sy;  // change of E per pixel step in x and y
+    bool tie;
+};
+
+__device__ __forceinline__ TileEdge tile_edge(const ScreenTri& S, int e, int tx, int ty) {
+    const int a = e, b = e == 2 ? 0 : e + 1;
+    const long long dx = S.X[b] - S.X[a], dy = S.Y[b] - S.Y[a];
+    const long long px0 = 256LL * (8 * tx) + 128, py0 = 256LL * (8 * ty) + 128;
+    TileEdge t;
+    t.C = dx * (py0 - S.Y[a]) - dy * (px0 - S.X[a]);
+    t.sx = -dy * 256;
+    t.sy = dx * 256;
+    t.tie = dy < 0 || (dy == 0 && dx > 0);
+    return t;
+}
+
+// False when some edge function is negative at every pixel centre of tile (tx, ty): a record of that tile
+// could not cover a pixel.  (The maximum of E over the tile's centre lattice sits at a corner.)
+__device__ __forceinline__ bool tile_may_cover(const ScreenTri& S, int tx, int ty) {
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const TileEdge t = tile_edge(S, e, tx, ty);
+        const long long mx = t.C + (t.sx > 0 ? 7 * t.sx : 0) + (t.sy > 0 ? 7 * t.sy : 0);
+        if (mx < 0) return false;
+    }
+    return true;
+}
+
+// Faces whose bounding box spans more tiles than this keep every tile of it (bounded per-thread work).
+constexpr int kMaxTestedTiles = 64;
+
+__device__ __forceinline__ bool test_tiles(const FaceBins& b) {
+    return (b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1) <= kMaxTestedTiles;
+}
+
+__device__ __forceinline__ uint32_t face_record_count(const FaceBins& b) {
+    if (b.tx0 > b.tx1 || b.ty0 > b.ty1) return 0u;
+    if (!test_tiles(b)) return (uint32_t)((b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1));
+    uint32_t c = 0;
+    for (int ty = b.ty0; ty <= b.ty1; ty++)
+        for (int tx = b.tx0; tx <= b.tx1; tx++) c += tile_may_cover(b.S, tx, ty) ? 1u : 0u;
+    return c;
+}
+
+// Exact coverage of tile (tx, ty): bit ly * 8 + lx is set when the centre of pixel (8 tx + lx, 8 ty + ly) is
+// inside the triangle; pixels outside the viewport are clear.  Edges whose extremes over the tile decide every
+// pixel at once are not evaluated per pixel.
+__device__ __forceinline__ uint64_t tile_coverage(const ScreenTri& S, int tx, int ty, int width, int height) {
+    const int nx = min(8, width - 8 * tx), ny = min(8, height - 8 * ty);
+    const uint64_t row = nx >= 8 ? 0xFFull : ((1ull << nx) - 1ull);
+    uint64_t m = 0;
+    for (int ly = 0; ly < ny; ly++) m |= row << (8 * ly);
+#pragma unroll 1
+    for (int e = 0; e < 3; e++) {
+        const TileEdge t = tile_edge(S, e, tx, ty);
+        const long long ex = 7 * t.sx, ey = 7 * t.sy;
+        const long long emax = t.C + (ex > 0 ? ex : 0) + (ey > 0 ? ey : 0);
+        const long long emin = t.C + (ex < 0 ? ex : 0) + (ey < 0 ? ey : 0);
+        if (emax < 0 || (emax == 0 && !t.tie)) return 0ull;
+        if (emin > 0 || (emin == 0 && t.tie)) continue;  // the edge admits every pixel of the tile
+        uint64_t em = 0;
+        long long Er = t.C;
+#pragma unroll 1
+        for (int ly = 0; ly < 8; ly++) {
+            long long E = Er;
+            for (int lx = 0; lx < 8; lx++) {
+                if (E > 0 || (E == 0 && t.tie)) em |= 1ull << (8 * ly + lx);
+                E += t.sx;
+            }
+            Er += t.sy;
+        }
+        m &= em;
+        if (m == 0) return 0ull;
+    }
+    return m;
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
@@ -62,9 +147,10 @@ __global__ __launch_bounds__(256) void k_count_pairs(const BatchItem* __restrict
             for (int f = 0; f < 4; f++) {
                 if (!(PT.visible & (1u << f))) continue;
                 ntri++;
-                FaceBins b = face_bins(U, PT, f);
-                c += b.count;
-                if (b.count) {
+                const FaceBins b = face_bins(U, PT, f);
+                const uint32_t cf = face_record_count(b);
+                c += cf;
+                if (cf) {
                     uint32_t k = face_key(PT, f);
                     kand &= k;
                     kor |= k;
@@ -121,10 +207,12 @@ __global__ __launch_bounds__(256) void k_reduce_bin_stats(const uint4* __restric
 }
 
 // Same grid as k_count_pairs: writes the tet's records at offsets[slot], faces in order, tiles row-major.
+// keys/vals are the sort input (vals = record index); rec_id[i] = tet * 4 + face of record i.
 __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict__ items, int nitems,
                                                      const uint32_t* __restrict__ offsets,
                                                      const uint32_t* __restrict__ counts,
-                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                     uint32_t* __restrict__ rec_id) {
     const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
     const ViewUniforms& U = B.U;
     const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
@@ -138,20 +226,20 @@ __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict_
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         if (!(PT.visible & (1u << f)) || left == 0) continue;
-        FaceBins b = face_bins(U, PT, f);
-        if (b.count == 0) continue;
+        const FaceBins b = face_bins(U, PT, f);
         const uint64_t k = (uint64_t)face_key(PT, f);
         const uint32_t id = (uint32_t)(tet * 4 + f);
-        const int bw = b.tx1 - b.tx0 + 1;
-        // Bounded by the stored count: the pass recomputes the same bins, but never writes past its slots.
-        const uint32_t n = min(left, b.count);
-        for (uint32_t i = 0; i < n; i++) {
-            const int tx = b.tx0 + (int)(i % (uint32_t)bw), ty = b.ty0 + (int)(i / (uint32_t)bw);
-            keys[o + i] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
-            vals[o + i] = id;
-        }
-        o += n;
-        left -= n;
+        const bool test = test_tiles(b);
+        // Bounded by the stored count: the pass recomputes the same records, but never writes past its slots.
+        for (int ty = b.ty0; ty <= b.ty1 && left > 0; ty++)
+            for (int tx = b.tx0; tx <= b.tx1 && left > 0; tx++) {
+                if (test && !tile_may_cover(b.S, tx, ty)) continue;
+                keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
+                vals[o] = o;
+                rec_id[o] = id;
+                o++;
+                left--;
+            }
     }
 }
 
@@ -161,6 +249,28 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint
         if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != t) ranges[t].x = (uint32_t)i;
         if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != t) ranges[t].y = (uint32_t)(i + 1);
     }
+}
+
+// Sorted record i = {face id, coverage mask low, coverage mask high, 0}: the face of record sorted_idx[i] is
+// projected again (the same arithmetic as binning) and its exact coverage of the record's tile computed.
+__global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restrict__ items, int nitems,
+                                                         const uint64_t* __restrict__ sorted_keys,
+                                                         const uint32_t* __restrict__ sorted_idx,
+                                                         const uint32_t* __restrict__ rec_id, int64_t n,
+                                                         uint4* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = rec_id[sorted_idx[i]];
+    const uint32_t gtile = (uint32_t)(sorted_keys[i] >> 32);
+    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    const ViewUniforms& U = B.U;
+    const int tile = (int)(gtile - B.tile0);
+    ProjectedTet PT;
+    project_tet(U, B.cells, B.pos, id >> 2, PT);
+    ScreenTri S;
+    face_screen(U, PT, id & 3, S);
+    const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
+    out[i] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
 }
 
 int64_t count_pairs_blocks(int64_t T) { return T > 0 ? (T + 255) / 256 : 0; }
@@ -179,10 +289,10 @@ void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const
 }
 
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
-                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, hipStream_t s) {
     if (nblocks <= 0) return;
     hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, offsets, counts, keys,
-                       vals);
+                       vals, rec_id);
 }
 
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s) {
@@ -191,6 +301,14 @@ void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tile
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, sorted_keys, n, ranges);
+}
+
+void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
+                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, uint4* out,
+                           hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items, nitems,
+                       sorted_keys, sorted_idx, rec_id, n, out);
 }
 
 }  // namespace dtv

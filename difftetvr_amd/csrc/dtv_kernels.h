@@ -32,9 +32,14 @@ void launch_count_pairs(const BatchItem* items, int nitems, int64_t nblocks, uin
 // stats.total_pairs = *scan_total; the other fields reduce the per-block partials of launch_count_pairs.
 void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const uint64_t* scan_total,
                              BinStats* stats, hipStream_t s);
+// keys/vals: sort input (vals = record index); rec_id[i] = tet * 4 + face of record i.
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
-                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, hipStream_t s);
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s);
+// out[i] = {face id, 8x8 coverage mask low, high, 0} of sorted record i: the tile renderer's record stream.
+void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
+                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, uint4* out,
+                           hipStream_t s);
 
 // render.hip
 // Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's
@@ -47,12 +52,12 @@ struct PixelState {
     float4* ckpt_color;     // [max_ckpt][P]
     uint32_t* ckpt_n;       // [max_ckpt][P]
 };
-void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                            const uint2* ranges, PixelState ps, hipStream_t s);
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
-                           const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int use_abs, int mode,
+                           const uint4* recs, const uint2* ranges, PixelState ps, int use_abs, int mode,
                            hipStream_t s);
-void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 
 // optim.hip

@@ -53,7 +53,7 @@ __device__ __forceinline__ uint32_t build_fragment_list(FragListLds& F, uint32_t
 }
 
 __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
-                                                        const uint32_t* __restrict__ rec_vals,
+                                                        const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ RecordLds L;
     __shared__ FragListLds F;
@@ -71,7 +71,6 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     const int lx = lane & 7, ly = lane >> 3;
     const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
     const bool valid = x < U.width && y < U.height;
-    const int ox = lx * 256, oy = ly * 256;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
     {
         const float3_ rd = ray_direction(U, x, y);
@@ -90,12 +89,9 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     bool done = !valid;
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, cells, pos, col, rec_vals[base + lane], tx, ty, L, lane);
+        if (lane < cnt) setup_record(U, cells, pos, col, recs[base + lane], L, lane);
         __syncthreads();
-        uint32_t mask = 0;
-        if (!done)
-            for (int j = 0; j < cnt; j++)
-                if (covers(L, j, ox, oy)) mask |= 1u << j;
+        const uint32_t mask = done ? 0u : lane_coverage(L, cnt, lane);
         uint32_t total;
         const uint32_t off = build_fragment_list(F, mask, false, lane, total);
         const uint32_t mine_end = off + (uint32_t)__popc(mask);
@@ -160,7 +156,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
 
 __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restrict__ items,
                                                         const AdjointTarget* __restrict__ targets, int nitems,
-                                                        const uint32_t* __restrict__ rec_vals,
+                                                        const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all,
                                                         int use_abs) {
     __shared__ RecordLds L;
@@ -184,7 +180,6 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     const int lx = lane & 7, ly = lane >> 3;
     const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
     const bool valid = x < U.width && y < U.height;
-    const int ox = lx * 256, oy = ly * 256;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
     {
         const float3_ rd = ray_direction(U, x, y);
@@ -221,29 +216,29 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     for (int bi = nbatch - 1; bi >= 0; bi--) {
         const uint32_t base = rg.x + (uint32_t)bi * kBatch;
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, cells, pos, col, rec_vals[base + lane], tx, ty, L, lane);
+        if (lane < cnt) setup_record(U, cells, pos, col, recs[base + lane], L, lane);
         // lanes j and j + 32 own record j's 28 values (first / second half of 14) for this batch
         float acc[14];
 #pragma unroll
         for (int q = 0; q < 14; q++) acc[q] = 0.f;
         __syncthreads();
-        uint32_t mask = 0;
-        if (!done)
-            for (int j = 0; j < cnt; j++)
-                if (base + j < last && covers(L, j, ox, oy)) mask |= 1u << j;
+        // records at or after `last` were not composited in the forward (early ray termination)
+        const uint32_t nlive = last > base ? min((uint32_t)cnt, last - base) : 0u;
+        const uint32_t live = nlive >= 32u ? 0xFFFFFFFFu : ((1u << nlive) - 1u);
+        const uint32_t mask = done ? 0u : (lane_coverage(L, cnt, lane) & live);
         uint32_t total;
         const uint32_t off = build_fragment_list(F, mask, true, lane, total);
         const uint32_t mine_end = off + (uint32_t)__popc(mask);
         __syncthreads();
         for (uint32_t r0 = 0; r0 < total; r0 += kRoundAdj) {
-            // A: fragment colours (one fragment per lane)
+            // A: fragment colours (one fragment per lane; E stays in registers for phase C)
+            FragEval E;
             {
                 const uint32_t e = r0 + lane;
                 if (e < total) {
                     const uint32_t ent = F.entry[e];
                     const int pl = ent >> 5, j = ent & 31;
                     const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
-                    FragEval E;
                     eval_fragment(L, j, ro, rdp, att, E);
                     F.val[lane] = E.F;
                     s_hit[lane] = E.hit ? 1 : 0;
@@ -289,8 +284,6 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
                     const uint32_t ent = F.entry[e];
                     const int pl = ent >> 5, j = ent & 31;
                     const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
-                    FragEval E;
-                    eval_fragment(L, j, ro, rdp, att, E);
                     float lam0[4], lam1[4], dcol0[4], dcol1[4];
                     float3_ Q0, Q1;
                     frag_backward_exact(E, L, j, rdp, att, F.val[lane], lam0, lam1, dcol0, dcol1, Q0, Q1);
@@ -368,7 +361,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
 
 // Ordered fragment export (tet*4 + face per covered record), up to max_per_pixel per pixel.
 __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __restrict__ items, int nitems,
-                                                          const uint32_t* __restrict__ rec_vals,
+                                                          const uint4* __restrict__ recs,
                                                           const uint2* __restrict__ ranges, PixelState ps_all,
                                                           int max_per_pixel, int32_t* __restrict__ out_ids_all) {
     __shared__ RecordLds L;
@@ -385,18 +378,20 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
     const int lx = lane & 7, ly = lane >> 3;
     const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
     const bool valid = x < U.width && y < U.height;
-    const int ox = lx * 256, oy = ly * 256;
     const uint2 rg = ranges[gtile];
     const int pix = valid ? pixel_index(U, x, y) : 0;
     const uint32_t last = valid ? ps.last_rec[pix] : 0u;
     int k = 0;
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, cells, pos, nullptr, rec_vals[base + lane], tx, ty, L, lane);
+        if (lane < cnt) setup_record(U, cells, pos, nullptr, recs[base + lane], L, lane);
         __syncthreads();
-        for (int j = 0; j < cnt; j++) {
-            if (!valid || base + j >= last || !covers(L, j, ox, oy)) continue;
-            if (k < max_per_pixel) out_ids[(int64_t)pix * max_per_pixel + k] = (int32_t)rec_vals[base + j];
+        uint32_t m = valid ? lane_coverage(L, cnt, lane) : 0u;
+        while (m) {
+            const int j = __ffs(m) - 1;
+            m &= m - 1u;
+            if (base + j >= last) break;
+            if (k < max_per_pixel) out_ids[(int64_t)pix * max_per_pixel + k] = (int32_t)recs[base + j].x;
             k++;
         }
         __syncthreads();
@@ -405,25 +400,25 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
         for (; k < max_per_pixel; k++) out_ids[(int64_t)pix * max_per_pixel + k] = -1;
 }
 
-void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                            const uint2* ranges, PixelState ps, hipStream_t s) {
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_render_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, rec_vals, ranges, ps);
+    hipLaunchKernelGGL(k_render_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, recs, ranges, ps);
 }
 
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
-                           const uint32_t* rec_vals, const uint2* ranges, PixelState ps, int use_abs, int mode,
+                           const uint4* recs, const uint2* ranges, PixelState ps, int use_abs, int mode,
                            hipStream_t s) {
     (void)mode;
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_render_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, rec_vals,
+    hipLaunchKernelGGL(k_render_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, recs,
                        ranges, ps, use_abs);
 }
 
-void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* rec_vals,
+void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s) {
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_export_fragments, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, rec_vals, ranges,
+    hipLaunchKernelGGL(k_export_fragments, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, recs, ranges,
                        ps, max_per_pixel, out_ids);
 }
 

@@ -9,44 +9,30 @@ constexpr int kTileEdge = 8;
 constexpr int kBatch = 32;  // records staged per batch (one 32-bit coverage mask per pixel)
 
 struct RecordLds {
-    float p[12][kBatch];   // tet vertex positions, [3*v + c]
-    float c[16][kBatch];   // tet vertex colours, [4*v + ch]
-    float N[12][kBatch];   // face plane normals cross(p1-p0, p2-p0), [3*f + c]
-    float k[4][kBatch];    // dot(N, camPos) - dot(N, p0) per face
-    int32_t A[3][kBatch];  // edge functions of the record's face: E = C + A*ox + B*oy (fixed point)
-    int32_t B[3][kBatch];
-    long long C[3][kBatch];
-    uint32_t meta[kBatch]; // bits 0-2: edge tie flags, bit 3: face normals point outward
+    float p[12][kBatch];      // tet vertex positions, [3*v + c]
+    float c[16][kBatch];      // tet vertex colours, [4*v + ch]
+    float N[12][kBatch];      // face plane normals cross(p1-p0, p2-p0), [3*f + c]
+    float k[4][kBatch];       // dot(N, camPos) - dot(N, p0) per face
+    uint32_t mask[2][kBatch]; // coverage of the tile's pixels 0-31 / 32-63 (pixel = ly * 8 + lx = lane)
+    uint32_t meta[kBatch];    // bit 3: face normals point outward
     int32_t vid[4][kBatch];
 };
 
+// Stages record rec = {tet * 4 + face, coverage mask low, high} in slot i (world-space data only: the coverage
+// was decided at binning).
 __device__ __forceinline__ void setup_record(const ViewUniforms& U, const int32_t* __restrict__ cells,
                                              const float* __restrict__ pos, const float* __restrict__ col,
-                                             uint32_t rec, int tx, int ty, RecordLds& L, int i) {
-    const int64_t tet = rec >> 2;
-    const int f = rec & 3;
-    ProjectedTet PT;
-    project_tet(U, cells, pos, tet, PT);
-    ScreenTri S;
-    face_screen(U, PT, f, S);
-    const long long px0 = 256LL * (tx * kTileEdge) + 128, py0 = 256LL * (ty * kTileEdge) + 128;
-    uint32_t meta = 0;
-#pragma unroll
-    for (int e = 0; e < 3; e++) {
-        const int a = e, b = (e + 1) % 3;
-        const int32_t dx = S.X[b] - S.X[a], dy = S.Y[b] - S.Y[a];
-        L.C[e][i] = (long long)dx * (py0 - S.Y[a]) - (long long)dy * (px0 - S.X[a]);
-        L.A[e][i] = -dy;
-        L.B[e][i] = dx;
-        if (dy < 0 || (dy == 0 && dx > 0)) meta |= 1u << e;
-    }
+                                             uint4 rec, RecordLds& L, int i) {
+    const int64_t tet = rec.x >> 2;
     const int4 idx = *reinterpret_cast<const int4*>(cells + 4 * tet);
     const int32_t vid[4] = {idx.x, idx.y, idx.z, idx.w};
+    float3_ p[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        L.p[3 * v + 0][i] = PT.p[v].x;
-        L.p[3 * v + 1][i] = PT.p[v].y;
-        L.p[3 * v + 2][i] = PT.p[v].z;
+        p[v] = load_pos(pos, vid[v]);
+        L.p[3 * v + 0][i] = p[v].x;
+        L.p[3 * v + 1][i] = p[v].y;
+        L.p[3 * v + 2][i] = p[v].z;
         if (col != nullptr) {
             const float4 cv = *reinterpret_cast<const float4*>(col + 4 * (int64_t)vid[v]);
             L.c[4 * v + 0][i] = cv.x;
@@ -56,30 +42,31 @@ __device__ __forceinline__ void setup_record(const ViewUniforms& U, const int32_
         }
         L.vid[v][i] = vid[v];
     }
+    L.mask[0][i] = rec.y;
+    L.mask[1][i] = rec.z;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
+    uint32_t meta = 0;
 #pragma unroll
     for (int g = 0; g < 4; g++) {
-        const float3_ a = PT.p[kFaceTable[g][0]], b = PT.p[kFaceTable[g][1]], c = PT.p[kFaceTable[g][2]];
+        const float3_ a = p[kFaceTable[g][0]], b = p[kFaceTable[g][1]], c = p[kFaceTable[g][2]];
         const float3_ n = cross(sub(b, a), sub(c, a));
         const float d = -dot(n, a);
         L.N[3 * g + 0][i] = n.x;
         L.N[3 * g + 1][i] = n.y;
         L.N[3 * g + 2][i] = n.z;
         L.k[g][i] = dot(n, ro) + d;
-        if (g == 0 && dot(n, sub(PT.p[3], PT.p[0])) < 0.0f) meta |= 8u;
+        if (g == 0 && dot(n, sub(p[3], p[0])) < 0.0f) meta |= 8u;
     }
     L.meta[i] = meta;
 }
 
-__device__ __forceinline__ bool covers(const RecordLds& L, int j, int ox, int oy) {
-    const uint32_t meta = L.meta[j];
-    bool in = true;
-#pragma unroll
-    for (int e = 0; e < 3; e++) {
-        const long long E = L.C[e][j] + (long long)L.A[e][j] * ox + (long long)L.B[e][j] * oy;
-        in = in && (E > 0 || (E == 0 && ((meta >> e) & 1u)));
-    }
-    return in;
+// 32-bit mask of the batch's records (bit j: record j, j < cnt) that cover this lane's pixel.
+__device__ __forceinline__ uint32_t lane_coverage(const RecordLds& L, int cnt, int lane) {
+    const uint32_t* w = L.mask[lane >> 5];
+    const int b = lane & 31;
+    uint32_t m = 0;
+    for (int j = 0; j < cnt; j++) m |= ((w[j] >> b) & 1u) << j;
+    return m;
 }
 
 __device__ __forceinline__ float3_ lds_p(const RecordLds& L, int v, int j) {

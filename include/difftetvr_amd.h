@@ -167,7 +167,7 @@ int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out);
 #define DTV_STAGE_SCAN 1
 #define DTV_STAGE_EMIT 2
 #define DTV_STAGE_SORT 3
-#define DTV_STAGE_RANGES 4
+#define DTV_STAGE_RANGES 4 /* tile ranges and the record gather (exact coverage masks) */
 #define DTV_STAGE_FORWARD 5
 #define DTV_STAGE_ADJOINT 6
 #define DTV_NUM_STAGES 7
