@@ -306,7 +306,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->keys2.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->rec_id.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
-    DTV_HIP(view->recs.ensure(sizeof(uint4) * (K > 0 ? K : 1)));
+    DTV_HIP(view->recs.ensure(2 * sizeof(uint4) * (K > 0 ? K : 1)));
     view->mark(DTV_STAGE_EMIT, 0, s);
     launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
                       view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), view->rec_id.as<uint32_t>(), s);

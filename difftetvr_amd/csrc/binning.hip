@@ -251,8 +251,9 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint
     }
 }
 
-// Sorted record i = {face id, coverage mask low, coverage mask high, 0}: the face of record sorted_idx[i] is
-// projected again (the same arithmetic as binning) and its exact coverage of the record's tile computed.
+// Sorted record i = out[2i], out[2i + 1] = {the tet's 4 vertex indices}, {face id, coverage mask low, high, 0}:
+// the face of record sorted_idx[i] is projected again (the same arithmetic as binning) and its exact coverage
+// of the record's tile computed.  Carrying the vertex indices saves the renderer one dependent load per record.
 __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restrict__ items, int nitems,
                                                          const uint64_t* __restrict__ sorted_keys,
                                                          const uint32_t* __restrict__ sorted_idx,
@@ -265,12 +266,15 @@ __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restr
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
     const int tile = (int)(gtile - B.tile0);
+    const int64_t tet = id >> 2;
     ProjectedTet PT;
-    project_tet(U, B.cells, B.pos, id >> 2, PT);
+    project_tet(U, B.cells, B.pos, tet, PT);
     ScreenTri S;
     face_screen(U, PT, id & 3, S);
     const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
-    out[i] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
+    const int4 vid = *reinterpret_cast<const int4*>(B.cells + 4 * tet);
+    out[2 * i] = make_uint4((uint32_t)vid.x, (uint32_t)vid.y, (uint32_t)vid.z, (uint32_t)vid.w);
+    out[2 * i + 1] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
 }
 
 int64_t count_pairs_blocks(int64_t T) { return T > 0 ? (T + 255) / 256 : 0; }

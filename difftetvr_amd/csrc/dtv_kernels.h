@@ -36,7 +36,8 @@ void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
                        const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, hipStream_t s);
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s);
-// out[i] = {face id, 8x8 coverage mask low, high, 0} of sorted record i: the tile renderer's record stream.
+// out[2i], out[2i + 1] = {tet vertex indices}, {face id, 8x8 coverage mask low, high, 0} of sorted record i: the
+// tile renderer's record stream.
 void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
                            const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, uint4* out,
                            hipStream_t s);

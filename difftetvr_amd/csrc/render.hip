@@ -60,7 +60,6 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     const uint32_t gtile = global_tile_of_block();
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
-    const int32_t* __restrict__ cells = B.cells;
     const float* __restrict__ pos = B.pos;
     const float* __restrict__ col = B.col;
     float4* __restrict__ out_image = reinterpret_cast<float4*>(B.image);
@@ -87,9 +86,23 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     uint32_t n = 0, last = rg.x, nck = 0, nlast = 0;
     float Tlast = 1.0f;
     bool done = !valid;
+    // Two-stage prefetch: while a batch is shaded, the next batch's vertex data and the record words of the
+    // batch after it are in flight.
+    RecordIn cur, nxt;
+    RecordData cd;
+    if (lane < kBatch && rg.x + lane < rg.y) {
+        cur = load_record(recs, rg.x + lane);
+        fetch_record_data(pos, col, cur, cd);
+    }
+    if (lane < kBatch && rg.x + kBatch + lane < rg.y) nxt = load_record(recs, rg.x + kBatch + lane);
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, cells, pos, col, recs[base + lane], L, lane);
+        if (lane < cnt) store_record(U, cur, cd, L, lane);
+        if (lane < kBatch && base + kBatch + lane < rg.y) {
+            cur = nxt;
+            fetch_record_data(pos, col, cur, cd);
+        }
+        if (lane < kBatch && base + 2 * kBatch + lane < rg.y) nxt = load_record(recs, base + 2 * kBatch + lane);
         __syncthreads();
         const uint32_t mask = done ? 0u : lane_coverage(L, cnt, lane);
         uint32_t total;
@@ -167,7 +180,6 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
     const ViewUniforms& U = B.U;
-    const int32_t* __restrict__ cells = B.cells;
     const float* __restrict__ pos = B.pos;
     const float* __restrict__ col = B.col;
     const float4* __restrict__ dimage = reinterpret_cast<const float4*>(targets[item].dimage);
@@ -213,10 +225,22 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     if (ballot(!done) == 0ull) return;
     const uint32_t nrec = rg.y - rg.x;
     const int nbatch = (int)((nrec + kBatch - 1) / kBatch);
+    // Batches run back to front.  The next batch's record words are loaded when the current batch starts and its
+    // vertex data just before the current batch's gradient atomics, so that staging it never waits on memory.
+    RecordIn nxt;
+    RecordData nd;
+    {
+        const uint32_t b0 = rg.x + (uint32_t)(nbatch - 1) * kBatch;
+        if (lane < kBatch && b0 + lane < rg.y) {
+            nxt = load_record(recs, b0 + lane);
+            fetch_record_data(pos, col, nxt, nd);
+        }
+    }
     for (int bi = nbatch - 1; bi >= 0; bi--) {
         const uint32_t base = rg.x + (uint32_t)bi * kBatch;
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, cells, pos, col, recs[base + lane], L, lane);
+        if (lane < cnt) store_record(U, nxt, nd, L, lane);
+        if (bi > 0 && lane < kBatch) nxt = load_record(recs, base - kBatch + lane);
         // lanes j and j + 32 own record j's 28 values (first / second half of 14) for this batch
         float acc[14];
 #pragma unroll
@@ -336,6 +360,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
             }
             __syncthreads();
         }
+        if (bi > 0 && lane < kBatch) fetch_record_data(pos, col, nxt, nd);
         // flush: lanes j and j + 32 share record j (first / second half of its 28 values)
         {
             const int j = lane & 31, half = lane >> 5;
@@ -368,7 +393,6 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
     const uint32_t gtile = global_tile_of_block();
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
-    const int32_t* __restrict__ cells = B.cells;
     const float* __restrict__ pos = B.pos;
     const PixelState ps = item_pixel_state(ps_all, B);
     int32_t* __restrict__ out_ids = out_ids_all + B.pixel0 * max_per_pixel;
@@ -384,14 +408,15 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
     int k = 0;
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, cells, pos, nullptr, recs[base + lane], L, lane);
+        if (lane < cnt) setup_record(U, pos, nullptr, load_record(recs, base + lane), L, lane);
         __syncthreads();
         uint32_t m = valid ? lane_coverage(L, cnt, lane) : 0u;
         while (m) {
             const int j = __ffs(m) - 1;
             m &= m - 1u;
             if (base + j >= last) break;
-            if (k < max_per_pixel) out_ids[(int64_t)pix * max_per_pixel + k] = (int32_t)recs[base + j].x;
+            if (k < max_per_pixel)
+                out_ids[(int64_t)pix * max_per_pixel + k] = (int32_t)recs[2 * (int64_t)(base + j) + 1].x;
             k++;
         }
         __syncthreads();

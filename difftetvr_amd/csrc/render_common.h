@@ -18,28 +18,53 @@ struct RecordLds {
     int32_t vid[4][kBatch];
 };
 
-// Stages record rec = {tet * 4 + face, coverage mask low, high} in slot i (world-space data only: the coverage
-// was decided at binning).
-__device__ __forceinline__ void setup_record(const ViewUniforms& U, const int32_t* __restrict__ cells,
-                                             const float* __restrict__ pos, const float* __restrict__ col,
-                                             uint4 rec, RecordLds& L, int i) {
-    const int64_t tet = rec.x >> 2;
-    const int4 idx = *reinterpret_cast<const int4*>(cells + 4 * tet);
-    const int32_t vid[4] = {idx.x, idx.y, idx.z, idx.w};
+// One record of the tile renderer's stream (two uint4 per record): the tet's vertex indices, and
+// {tet * 4 + face, coverage mask low, high, 0}.
+struct RecordIn {
+    uint4 vid;
+    uint4 rec;
+};
+
+__device__ __forceinline__ RecordIn load_record(const uint4* __restrict__ recs, uint32_t i) {
+    RecordIn r;
+    r.vid = recs[2 * (int64_t)i];
+    r.rec = recs[2 * (int64_t)i + 1];
+    return r;
+}
+
+// Stages record r in slot i (world-space data only: the coverage was decided at binning).
+// Vertex data of one record, fetched ahead of its staging so that the loads overlap other work.  (On CDNA one
+// counter tracks loads and stores alike: a load issued after the gradient atomics would wait for them too.)
+struct RecordData {
     float3_ p[4];
+    float4 c[4];
+};
+
+__device__ __forceinline__ void fetch_record_data(const float* __restrict__ pos, const float* __restrict__ col,
+                                                  const RecordIn& r, RecordData& d) {
+    const int32_t vid[4] = {(int32_t)r.vid.x, (int32_t)r.vid.y, (int32_t)r.vid.z, (int32_t)r.vid.w};
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        p[v] = load_pos(pos, vid[v]);
+        d.p[v] = load_pos(pos, vid[v]);
+        d.c[v] = col != nullptr ? *reinterpret_cast<const float4*>(col + 4 * (int64_t)vid[v])
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__device__ __forceinline__ void store_record(const ViewUniforms& U, const RecordIn& r, const RecordData& d,
+                                             RecordLds& L, int i) {
+    const uint4 rec = r.rec;
+    const int32_t vid[4] = {(int32_t)r.vid.x, (int32_t)r.vid.y, (int32_t)r.vid.z, (int32_t)r.vid.w};
+    const float3_* p = d.p;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
         L.p[3 * v + 0][i] = p[v].x;
         L.p[3 * v + 1][i] = p[v].y;
         L.p[3 * v + 2][i] = p[v].z;
-        if (col != nullptr) {
-            const float4 cv = *reinterpret_cast<const float4*>(col + 4 * (int64_t)vid[v]);
-            L.c[4 * v + 0][i] = cv.x;
-            L.c[4 * v + 1][i] = cv.y;
-            L.c[4 * v + 2][i] = cv.z;
-            L.c[4 * v + 3][i] = cv.w;
-        }
+        L.c[4 * v + 0][i] = d.c[v].x;
+        L.c[4 * v + 1][i] = d.c[v].y;
+        L.c[4 * v + 2][i] = d.c[v].z;
+        L.c[4 * v + 3][i] = d.c[v].w;
         L.vid[v][i] = vid[v];
     }
     L.mask[0][i] = rec.y;
@@ -58,6 +83,13 @@ __device__ __forceinline__ void setup_record(const ViewUniforms& U, const int32_
         if (g == 0 && dot(n, sub(p[3], p[0])) < 0.0f) meta |= 8u;
     }
     L.meta[i] = meta;
+}
+
+__device__ __forceinline__ void setup_record(const ViewUniforms& U, const float* __restrict__ pos,
+                                             const float* __restrict__ col, const RecordIn& r, RecordLds& L, int i) {
+    RecordData d;
+    fetch_record_data(pos, col, r, d);
+    store_record(U, r, d, L, i);
 }
 
 // 32-bit mask of the batch's records (bit j: record j, j < cnt) that cover this lane's pixel.
