@@ -196,6 +196,131 @@ def bench_multiview(args):
     print(json.dumps(out), flush=True)
 
 
+def _delaunay_job(args):
+    n, seed = args
+    from difftetvr_amd import meshgen
+    pts, cells = meshgen.delaunay_ball(n, seed=seed)
+    return pts, cells
+
+
+def bench_serve(args):
+    """Serving: a Poisson stream of render requests at a fixed rate over 32 resident meshes (16 cube grids of
+    16^3..80^3 cells x 6 tets, 16 Delaunay meshes of 10k..500k points in the unit ball), each request a seeded
+    random camera on a sphere of radius 1.5..3 with a viewport drawn from {256^2, 512^2, 1024^2}; 90% forward
+    renders, 10% forward + MSE loss + adjoint.  Reports p50/p99 latency (submit to completion) and frames/s."""
+    import multiprocessing
+    import threading
+
+    import numpy as np
+    import torch
+
+    import difftetvr_amd as d
+    from difftetvr_amd import meshgen
+    from difftetvr_amd.serve import Server
+
+    rng = np.random.default_rng(0)
+    grid_ns = np.linspace(16, 80, 16).round().astype(int)
+    del_ns = np.geomspace(10_000, 500_000, 16).round().astype(int)
+    t0 = time.perf_counter()
+    with multiprocessing.get_context("spawn").Pool(min(8, os.cpu_count() or 1)) as pool:
+        dels = pool.map(_delaunay_job, [(int(n), 100 + i) for i, n in enumerate(del_ns)])
+    meshes, tets = [], []
+    for i, n in enumerate(grid_ns):
+        w = meshgen.grid_workload(int(n), seed=i)
+        m = d.TetMesh()
+        m.set_use_gradients(True)
+        m.set_tet_mesh_data(w["cells"], w["pos"], w["col"])
+        meshes.append(m)
+        tets.append(int(w["cells"].shape[0]))
+    for i, (pts, cells) in enumerate(dels):
+        col = meshgen.blob_colors(pts, seed=200 + i)
+        col[:, 3] = meshgen.value_noise(pts, seed=300 + i, vmax=40.0)
+        m = d.TetMesh()
+        m.set_use_gradients(True)
+        m.set_tet_mesh_data(cells, pts, col)
+        meshes.append(m)
+        tets.append(int(cells.shape[0]))
+    t_setup = time.perf_counter() - t0
+    srv = Server(num_workers=2, max_batch=16, memory_budget=16 << 30, attenuation=1.0, fovy=math.pi / 4.0)
+    ids = [srv.add_mesh(m) for m in meshes]
+    sizes = [256, 512, 1024]
+    nreq = int(args.qps * args.duration)
+    plan = []
+    t = 0.0
+    for _ in range(nreq):
+        t += rng.exponential(1.0 / args.qps)
+        mi = int(rng.integers(len(meshes)))
+        e = rng.normal(size=3)
+        e = e / np.linalg.norm(e) * rng.uniform(1.5, 3.0)
+        res = sizes[int(rng.integers(3))]
+        plan.append((t, mi, meshgen.look_at_view_matrix(tuple(e)), res, bool(rng.random() < 0.1)))
+    # output and target buffers allocated up front (the server writes them from its own streams)
+    outs = {s: [torch.empty((s, s, 4), device="cuda") for _ in range(32)] for s in sizes}
+    targets = {s: torch.zeros((s, s, 4), device="cuda") for s in sizes}
+    torch.cuda.synchronize()
+    # warm-up: every mesh once at every size
+    for mi in range(len(meshes)):
+        for s in sizes:
+            srv.wait(srv.submit(ids[mi], plan[0][2], s, s, out=outs[s][0]))
+    tickets = []
+    lock = threading.Lock()
+    results = []
+
+    def collector():
+        k = 0
+        while k < nreq:
+            with lock:
+                have = len(tickets)
+            if k >= have:
+                time.sleep(0.0002)
+                continue
+            tk, res = tickets[k]
+            _, lat, q = srv.wait(tk)
+            results.append((lat, q, res))
+            k += 1
+
+    th = threading.Thread(target=collector)
+    th.start()
+    start = time.perf_counter()
+    for i, (ta, mi, vm, res, adj) in enumerate(plan):
+        now = time.perf_counter() - start
+        if ta > now:
+            time.sleep(ta - now)
+        tk = srv.submit(ids[mi], vm, res, res, target=targets[res] if adj else None, out=outs[res][i % 32])
+        with lock:
+            tickets.append((tk, res))
+    th.join()
+    elapsed = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r[0] for r in results])
+    lat1k = np.array([r[0] for r in results if r[2] == 1024])
+    out = {
+        "metric": "serve p50/p99 ms (Poisson stream, 32 resident meshes)",
+        "value": float(np.percentile(lat, 99)),
+        "unit": "ms (p99 latency)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "p99_ms_1024": float(np.percentile(lat1k, 99)) if lat1k.size else None,
+        "p50_ms_1024": float(np.percentile(lat1k, 50)) if lat1k.size else None,
+        "mean_queue_ms": float(np.mean([r[1] for r in results])),
+        "frames_per_s": len(results) / elapsed,
+        "offered_qps": args.qps,
+        "config": {"workload": "Poisson render requests over 16 grid meshes (16^3..80^3 x6) and 16 Delaunay meshes "
+                               "(10k..500k points), viewports {256,512,1024}^2, 10% with loss + adjoint, 2 workers, "
+                               "up to 16 requests per batch, 16 GiB in-flight budget",
+                   "requests": nreq, "meshes": len(meshes), "tets_total": int(sum(tets)),
+                   "mesh_setup_s": round(t_setup, 1), "batches": st["batches"],
+                   "max_batch_seen": st["max_batch_seen"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,8 +328,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--res", type=int, default=None, help="viewport edge (default 1024; 512 for multiview)")
-    ap.add_argument("--workload", choices=["single", "multiview"], default="single",
-                    help="single: the plain single-view step (default); multiview: 16 views per batched step")
+    ap.add_argument("--workload", choices=["single", "multiview", "serve"], default="single",
+                    help="single: the plain single-view step (default); multiview: 16 views per batched step; "
+                         "serve: Poisson request stream over 32 resident meshes")
+    ap.add_argument("--qps", type=float, default=150.0, help="serve: offered requests per second")
+    ap.add_argument("--duration", type=float, default=8.0, help="serve: seconds of request arrivals")
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=32)
@@ -228,6 +356,9 @@ def main():
 
     if args.workload == "multiview":
         bench_multiview(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
 
     import numpy as np

@@ -49,6 +49,15 @@ class DtvAdjointItem(ctypes.Structure):
     _fields_ = [("dimage", c_void_p), ("grad_positions", c_void_p), ("grad_colors", c_void_p)]
 
 
+class DtvServerRequest(ctypes.Structure):
+    _fields_ = [("mesh_id", c_int32), ("camera", DtvCamera), ("out_image", c_void_p), ("target_image", c_void_p),
+                ("grad_positions", c_void_p), ("grad_colors", c_void_p)]
+
+
+class DtvServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("batches", c_int64), ("max_batch_seen", c_int64)]
+
+
 class DtvAdamGroup(ctypes.Structure):
     _fields_ = [("param", c_void_p), ("grad", c_void_p), ("exp_avg", c_void_p), ("exp_avg_sq", c_void_p),
                 ("numel", c_int64), ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
@@ -61,6 +70,7 @@ SIGNATURES = [
     ("dtv_last_error", ctypes.c_char_p, []),
     ("dtv_ctx_create", ctypes.c_int, [c_int32, ctypes.POINTER(c_void_p)]),
     ("dtv_ctx_destroy", ctypes.c_int, [c_void_p]),
+    ("dtv_ctx_get_device", ctypes.c_int, [c_void_p, ctypes.POINTER(c_int32)]),
     ("dtv_view_create", ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     ("dtv_view_destroy", ctypes.c_int, [c_void_p]),
     ("dtv_camera_uniforms", ctypes.c_int, [ctypes.POINTER(DtvCamera), ctypes.POINTER(c_float)]),
@@ -81,6 +91,15 @@ SIGNATURES = [
     ("dtv_tet_regularizer_grad", ctypes.c_int, [ctypes.POINTER(DtvMesh), c_float, c_float, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad_batch", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
+    ("dtv_server_last_error", ctypes.c_char_p, []),
+    ("dtv_server_create", ctypes.c_int, [c_void_p, c_int32, c_int32, c_int64, ctypes.POINTER(DtvRenderParams),
+                                         ctypes.POINTER(c_void_p)]),
+    ("dtv_server_destroy", ctypes.c_int, [c_void_p]),
+    ("dtv_server_add_mesh", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), ctypes.POINTER(c_int32)]),
+    ("dtv_server_submit", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvServerRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    ("dtv_server_wait", ctypes.c_int, [c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]),
+    ("dtv_server_get_stats", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvServerStats)]),
 ]
 
 _lib = None
@@ -107,7 +126,7 @@ def load():
     return lib
 
 
-def check(status, what=""):
+def check(status, what="", server=False):
     if status != DTV_OK:
-        msg = load().dtv_last_error()
+        msg = load().dtv_server_last_error() if server else load().dtv_last_error()
         raise DtvError(f"{what}: {msg.decode() if msg else 'error'} (status {status})")

@@ -365,7 +365,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     if (adjoint_mode != DTV_ADJOINT_EXACT) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": unsupported adjoint_mode");
     if (n != view->nitems()) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": item count differs from the render");
     for (int i = 0; i < n; i++) {
-        if (!tg[i].dimage || !tg[i].grad_pos || !tg[i].grad_col) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
+        if (!tg[i].dimage) continue;  // item not differentiated
+        if (!tg[i].grad_pos || !tg[i].grad_col) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
         if (!aligned16(tg[i].dimage)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": dimage must be 16-byte aligned");
     }
     DTV_HIP(hipSetDevice(view->ctx->device));
@@ -417,6 +418,12 @@ int dtv_ctx_create(int32_t device, dtv_ctx** out_ctx) {
 
 int dtv_ctx_destroy(dtv_ctx* ctx) {
     delete ctx;
+    return DTV_OK;
+}
+
+int dtv_ctx_get_device(const dtv_ctx* ctx, int32_t* out_device) {
+    if (!ctx || !out_device) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_ctx_get_device: NULL argument");
+    *out_device = ctx->device;
     return DTV_OK;
 }
 

@@ -183,6 +183,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     const float* __restrict__ pos = B.pos;
     const float* __restrict__ col = B.col;
     const float4* __restrict__ dimage = reinterpret_cast<const float4*>(targets[item].dimage);
+    if (dimage == nullptr) return;  // item not differentiated
     float* __restrict__ grad_pos = targets[item].grad_pos;
     float* __restrict__ grad_col = targets[item].grad_col;
     const PixelState ps = item_pixel_state(ps_all, B);

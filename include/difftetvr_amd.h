@@ -112,6 +112,7 @@ const char* dtv_last_error(void);
 /* Replaces ApplicationState creation (src/Module/Module.cpp:192-489, ensureStateExists :646-650). */
 int dtv_ctx_create(int32_t device, dtv_ctx** out_ctx);
 int dtv_ctx_destroy(dtv_ctx* ctx);
+int dtv_ctx_get_device(const dtv_ctx* ctx, int32_t* out_device);
 
 /* Replaces the renderer object's intermediate buffers (TetMeshRendererIntersection.cpp:424-487). */
 int dtv_view_create(dtv_ctx* ctx, dtv_view** out_view);
@@ -136,7 +137,7 @@ int dtv_render_batch(dtv_view* view, const dtv_render_item* items, int32_t num_i
                      const dtv_render_params* params, void* stream);
 
 /* Adjoint of the last dtv_render_batch of this view, items[i] belonging to its render item i (same count).
- * One launch over all tiles of the batch. */
+ * One launch over all tiles of the batch; an item with dimage == NULL is not differentiated. */
 int dtv_render_adjoint_batch(dtv_view* view, const dtv_adjoint_item* items, int32_t num_items, int32_t use_abs_grad,
                              int32_t adjoint_mode, void* stream);
 
@@ -194,6 +195,43 @@ int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float*
  * The multi-view loss of a batched step (the reference sums per-view losses, train.py:325-346). */
 int dtv_mse_loss_grad_batch(const float* image, const float* target, int64_t n_per_item, int32_t num_items,
                             float* dimage, float* loss_out, void* stream);
+
+/* ---- render server ------------------------------------------------------------------------------------
+ * Meshes resident in device memory and a request queue served by worker threads, each owning a view and a
+ * stream: a free worker takes the pending requests in arrival order (up to max_batch, within memory_budget
+ * bytes of estimated transient device memory for all batches in flight; 0 = unbounded) and renders them as
+ * one dtv_render_batch, so requests that arrive while other batches are on the GPU are admitted as workers
+ * and memory free up, and their work overlaps on the device.  A request with a target image is also
+ * differentiated: MSE loss against the target, then the adjoint accumulating into its gradient buffers.
+ * Every result equals what dtv_render gives for the request alone.  No counterpart in the reference (it
+ * renders one view per Python call); errors of these calls are reported by dtv_server_last_error(). */
+typedef struct dtv_server dtv_server;
+
+typedef struct dtv_server_request {
+    int32_t mesh_id;            /* from dtv_server_add_mesh */
+    dtv_camera camera;
+    float* out_image;           /* device [height, width, 4], written when the request completes */
+    const float* target_image;  /* optional device [height, width, 4]: also run loss + adjoint */
+    float* grad_positions;      /* with target_image: device [V,3], accumulated */
+    float* grad_colors;         /* with target_image: device [V,4], accumulated */
+} dtv_server_request;
+
+typedef struct dtv_server_stats {
+    int64_t submitted, completed, batches, max_batch_seen;
+} dtv_server_stats;
+
+const char* dtv_server_last_error(void);
+int dtv_server_create(dtv_ctx* ctx, int32_t num_workers, int32_t max_batch, int64_t memory_budget,
+                      const dtv_render_params* params, dtv_server** out_server);
+/* Waits for every worker to finish its batch; requests still queued are rendered first. */
+int dtv_server_destroy(dtv_server* server);
+/* The mesh's device buffers are borrowed and must outlive the server. */
+int dtv_server_add_mesh(dtv_server* server, const dtv_mesh* mesh, int32_t* out_mesh_id);
+/* Thread-safe, non-blocking. */
+int dtv_server_submit(dtv_server* server, const dtv_server_request* request, uint64_t* out_ticket);
+/* Blocks until the request completed; latency = submit to completion, queue = submit to batch start (ms). */
+int dtv_server_wait(dtv_server* server, uint64_t ticket, double* out_latency_ms, double* out_queue_ms);
+int dtv_server_get_stats(dtv_server* server, dtv_server_stats* out);
 
 #ifdef __cplusplus
 }
