@@ -362,7 +362,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
 int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_abs_grad, int32_t adjoint_mode,
                   hipStream_t s, const char* fn) {
     const std::string f(fn);
-    if (adjoint_mode != DTV_ADJOINT_EXACT) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": unsupported adjoint_mode");
+    if (adjoint_mode != DTV_ADJOINT_EXACT && adjoint_mode != DTV_ADJOINT_REFERENCE)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": unsupported adjoint_mode");
     if (n != view->nitems()) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": item count differs from the render");
     for (int i = 0; i < n; i++) {
         if (!tg[i].dimage) continue;  // item not differentiated

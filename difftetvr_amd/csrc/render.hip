@@ -167,6 +167,11 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     }
 }
 
+// kRef = false: exact derivative of the forward.  kRef = true: the reference's adjoint as written
+// (AdjointIntersectRasterization.glsl:111-362): per pixel from the blended output image, back to front, the
+// 'over' operator is inverted per sub-segment 3, 2, 1 (sub-segment 0 is skipped) with no checkpoints, and
+// each fragment's parameter gradients follow frag_backward_reference.
+template <bool kRef>
 __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restrict__ items,
                                                         const AdjointTarget* __restrict__ targets, int nitems,
                                                         const uint4* __restrict__ recs,
@@ -176,6 +181,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     __shared__ FragListLds F;
     __shared__ float s_w[kRoundAdj][29];  // per fragment: 4 vertices x (3 position + 4 colour), padded row
     __shared__ uint8_t s_hit[kRoundAdj];
+    __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
     const uint32_t gtile = global_tile_of_block();
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
@@ -220,7 +226,15 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
         dC[0] = di.x;
         dC[1] = di.y;
         dC[2] = di.z;
-        dC[3] = di.w * (1.0f - U.clear[3]) - (di.x * U.clear[0] + di.y * U.clear[1] + di.z * U.clear[2]);
+        if constexpr (kRef) {
+            // the reference starts from the blended output image and the raw image adjoint
+            const float T = 1.0f - C.w;
+            C = make_float4(C.x + T * U.clear[0], C.y + T * U.clear[1], C.z + T * U.clear[2], C.w + T * U.clear[3]);
+            dC[3] = di.w;
+            jck = -1;
+        } else {
+            dC[3] = di.w * (1.0f - U.clear[3]) - (di.x * U.clear[0] + di.y * U.clear[1] + di.z * U.clear[2]);
+        }
     }
     bool done = !valid || n == 0;
     if (ballot(!done) == 0ull) return;
@@ -267,12 +281,35 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
                     eval_fragment(L, j, ro, rdp, att, E);
                     F.val[lane] = E.F;
                     s_hit[lane] = E.hit ? 1 : 0;
+                    if constexpr (kRef) {
+#pragma unroll
+                        for (int s = 1; s < 4; s++) s_seg[s - 1][lane] = E.seg[s];
+                    }
                 }
             }
             __syncthreads();
             // B: per pixel, back to front: undo the composite, dL/dF of each fragment
             const uint32_t e0 = max(off, r0), e1 = min(mine_end, r0 + kRoundAdj);
             for (uint32_t e = e0; e < e1; e++) {
+                if constexpr (kRef) {
+                    // reference: invert sub-segments 3..1 of the fragment; dL/d(sub-segment colour) replaces
+                    // the sub-segment colour in s_seg (AdjointIntersectRasterization.glsl:204-228)
+                    if (s_hit[e - r0]) {
+#pragma unroll
+                        for (int s = 3; s > 0; s--) {
+                            const float4 acc = s_seg[s - 1][e - r0];
+                            const float ain = (acc.w - C.w) / (acc.w - 1.0f);
+                            const float4 cin = make_float4(C.x - (1.0f - ain) * acc.x, C.y - (1.0f - ain) * acc.y,
+                                                           C.z - (1.0f - ain) * acc.z, ain);
+                            s_seg[s - 1][e - r0] = make_float4((1.0f - ain) * dC[0], (1.0f - ain) * dC[1],
+                                                               (1.0f - ain) * dC[2], (1.0f - ain) * dC[3]);
+                            dC[3] = dC[3] * (1.0f - acc.w) - (dC[0] * acc.x + dC[1] * acc.y + dC[2] * acc.z);
+                            C = cin;
+                        }
+                    }
+                    n--;
+                    continue;
+                }
                 if (jck >= 0 && ck_n == n) {
                     C = ps.ckpt_color[(int64_t)jck * P + pix];
                     jck--;
@@ -309,26 +346,31 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
                     const uint32_t ent = F.entry[e];
                     const int pl = ent >> 5, j = ent & 31;
                     const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
-                    float lam0[4], lam1[4], dcol0[4], dcol1[4];
-                    float3_ Q0, Q1;
-                    frag_backward_exact(E, L, j, rdp, att, F.val[lane], lam0, lam1, dcol0, dcol1, Q0, Q1);
-                    if (use_abs) {
+                    if constexpr (kRef) {
+                        const float4 dacc[3] = {s_seg[0][lane], s_seg[1][lane], s_seg[2][lane]};
+                        frag_backward_reference(E, L, j, ro, rdp, att, dacc, use_abs != 0, g);
+                    } else {
+                        float lam0[4], lam1[4], dcol0[4], dcol1[4];
+                        float3_ Q0, Q1;
+                        frag_backward_exact(E, L, j, rdp, att, F.val[lane], lam0, lam1, dcol0, dcol1, Q0, Q1);
+                        if (use_abs) {
 #pragma unroll
-                        for (int ch = 0; ch < 4; ch++) {
-                            dcol0[ch] = fabsf(dcol0[ch]);
-                            dcol1[ch] = fabsf(dcol1[ch]);
+                            for (int ch = 0; ch < 4; ch++) {
+                                dcol0[ch] = fabsf(dcol0[ch]);
+                                dcol1[ch] = fabsf(dcol1[ch]);
+                            }
+                            Q0 = f3(fabsf(Q0.x), fabsf(Q0.y), fabsf(Q0.z));
+                            Q1 = f3(fabsf(Q1.x), fabsf(Q1.y), fabsf(Q1.z));
                         }
-                        Q0 = f3(fabsf(Q0.x), fabsf(Q0.y), fabsf(Q0.z));
-                        Q1 = f3(fabsf(Q1.x), fabsf(Q1.y), fabsf(Q1.z));
-                    }
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const float l0 = lam0[v], l1 = lam1[v];
-                        g[7 * v + 0] = l0 * Q0.x + l1 * Q1.x;
-                        g[7 * v + 1] = l0 * Q0.y + l1 * Q1.y;
-                        g[7 * v + 2] = l0 * Q0.z + l1 * Q1.z;
+                        for (int v = 0; v < 4; v++) {
+                            const float l0 = lam0[v], l1 = lam1[v];
+                            g[7 * v + 0] = l0 * Q0.x + l1 * Q1.x;
+                            g[7 * v + 1] = l0 * Q0.y + l1 * Q1.y;
+                            g[7 * v + 2] = l0 * Q0.z + l1 * Q1.z;
 #pragma unroll
-                        for (int ch = 0; ch < 4; ch++) g[7 * v + 3 + ch] = l0 * dcol0[ch] + l1 * dcol1[ch];
+                            for (int ch = 0; ch < 4; ch++) g[7 * v + 3 + ch] = l0 * dcol0[ch] + l1 * dcol1[ch];
+                        }
                     }
                     key = j;
                 }
@@ -435,10 +477,13 @@ void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, c
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint4* recs, const uint2* ranges, PixelState ps, int use_abs, int mode,
                            hipStream_t s) {
-    (void)mode;
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_render_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, recs,
-                       ranges, ps, use_abs);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_render_adjoint<true>, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems,
+                           recs, ranges, ps, use_abs);
+    else
+        hipLaunchKernelGGL(k_render_adjoint<false>, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems,
+                           recs, ranges, ps, use_abs);
 }
 
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,

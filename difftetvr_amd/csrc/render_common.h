@@ -370,4 +370,147 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// The reference's own adjoint of one fragment (AdjointIntersectRasterization.glsl:111-362 with
+// Common/BackwardCommon.glsl:116-251), quirks kept: the caller supplies dL/d(sub-segment colour) for
+// sub-segments 3, 2, 1 only (sub-segment 0 is skipped), and the hit points move with fixed barycentrics.
+
+// accumulateLinearConstAdjoint (BackwardCommon.glsl:116-133) for one sub-segment of length t.
+__device__ __forceinline__ void ref_alc_adjoint(float t, float tf, float3_ c0, float3_ c1, float a, float A,
+                                                float4 dC, float& dOut_dt, float3_& dc0, float3_& dc1, float& da) {
+    const float dd = (dC.x * ((a * c0.x + (a * c1.x - a * c0.x) * t) * A) +
+                      dC.y * ((a * c0.y + (a * c1.y - a * c0.y) * t) * A)) +
+                     dC.z * ((a * c0.z + (a * c1.z - a * c0.z) * t) * A);
+    dOut_dt += tf * (dd + dC.w * a * A);
+    if (a < 1e-6f) {
+        dc0 = f3(0.f, 0.f, 0.f);
+        dc1 = f3(0.f, 0.f, 0.f);
+        const float s = (dC.x * (t * c0.x + 0.5f * t * t * (c1.x - c0.x)) +
+                         dC.y * (t * c0.y + 0.5f * t * t * (c1.y - c0.y))) +
+                        dC.z * (t * c0.z + 0.5f * t * t * (c1.z - c0.z));
+        da = s + dC.w * t;
+    } else {
+        const float inva = 1.0f / a;
+        const float k0 = (1.0f - A) + ((t + inva) * A - inva), k1 = -((t + inva) * A - inva);
+        dc0 = f3(k0 * dC.x, k0 * dC.y, k0 * dC.z);
+        dc1 = f3(k1 * dC.x, k1 * dC.y, k1 * dC.z);
+        const float kq = inva * inva - (t * t + inva * inva + t * inva) * A;
+        const float s = (dC.x * (t * A * c0.x + kq * (c0.x - c1.x)) + dC.y * (t * A * c0.y + kq * (c0.y - c1.y))) +
+                        dC.z * (t * A * c0.z + kq * (c0.z - c1.z));
+        da = s + dC.w * t * A;
+    }
+}
+
+// Derivatives of the unsigned-area barycentrics u = |(P-p1) x (p1-p2)| / |(p0-p2) x (p1-p2)| and
+// v = |(P-p0) x (p0-p2)| / |...| with respect to the triangle's vertices at a fixed hit point P
+// (baryAdjoint, BackwardCommon.glsl:183-251).
+__device__ __forceinline__ void ref_bary_grads(float3_ P, float3_ p0, float3_ p1, float3_ p2, float3_ du[3],
+                                               float3_ dv[3]) {
+    const float3_ A0 = sub(p0, p2), B0 = sub(p1, p2), n0 = cross(A0, B0);
+    const float3_ A1 = sub(P, p1), B1 = sub(p1, p2), n1 = cross(A1, B1);
+    const float3_ A2 = sub(P, p0), B2 = sub(p0, p2), n2 = cross(A2, B2);
+    const float f0 = dot(n0, n0), f1 = dot(n1, n1), f2 = dot(n2, n2);
+    const float den0 = fmaxf(f0 * sqrtf(f0), 1e-6f);
+    const float den1 = fmaxf(sqrtf(f1), 1e-6f);
+    const float den2 = fmaxf(sqrtf(f2), 1e-6f);
+    // halves of the gradients of |A x B|^2: w.r.t. A is B x (A x B), w.r.t. B is (A x B) x A
+    const float3_ gA0 = cross(B0, n0), gB0 = cross(n0, A0);
+    const float3_ gA1 = cross(B1, n1), gB1 = cross(n1, A1);
+    const float3_ gA2 = cross(B2, n2), gB2 = cross(n2, A2);
+    const float3_ hf0[3] = {gA0, gB0, scale(add(gA0, gB0), -1.0f)};
+    const float3_ z = f3(0.f, 0.f, 0.f);
+    const float3_ hf1[3] = {z, sub(gB1, gA1), scale(gB1, -1.0f)};
+    const float3_ hf2[3] = {sub(gB2, gA2), z, scale(gB2, -1.0f)};
+    du[0] = scale(hf0[0], -den1 / den0);
+    dv[1] = scale(hf0[1], -den2 / den0);
+    const float r1 = 1.0f / (den1 * den0), r2 = 1.0f / (den2 * den0);
+#pragma unroll
+    for (int k = 1; k < 3; k++) du[k] = scale(sub(scale(hf1[k], f0), scale(hf0[k], f1)), r1);
+#pragma unroll
+    for (int k = 0; k < 3; k += 2) dv[k] = scale(sub(scale(hf2[k], f0), scale(hf0[k], f2)), r2);
+}
+
+// Parameter gradients of one fragment from dacc[s - 1] = dL/d(colour of sub-segment s), s = 1..3.
+// g[7 v + c]: tet-local vertex v, c = 0..2 position, 3..6 colour.
+__device__ __forceinline__ void frag_backward_reference(const FragEval& E, const RecordLds& L, int j, float3_ ro,
+                                                        float3_ rd, float att, const float4 dacc[3], bool use_abs,
+                                                        float g[28]) {
+    const float tSeg = (E.t1 - E.t0) / 4.0f;
+    float dOut_dt = 0.f;
+    float dcf0[4] = {0.f, 0.f, 0.f, 0.f}, dcf1[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 3; s > 0; s--) {
+        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
+        const float3_ c0 = f3(mixf(E.col0.x, E.col1.x, fb), mixf(E.col0.y, E.col1.y, fb), mixf(E.col0.z, E.col1.z, fb));
+        const float3_ c1 = f3(mixf(E.col0.x, E.col1.x, fe), mixf(E.col0.y, E.col1.y, fe), mixf(E.col0.z, E.col1.z, fe));
+        const float al = mixf(E.col0.w, E.col1.w, fm);
+        const float A = expf(-(al * att) * tSeg);
+        float3_ dc0, dc1;
+        float da;
+        ref_alc_adjoint(tSeg, 0.25f, c0, c1, al * att, A, dacc[s - 1], dOut_dt, dc0, dc1, da);
+        da *= att;
+        dcf0[0] += (1.0f - fb) * dc0.x + (1.0f - fe) * dc1.x;
+        dcf0[1] += (1.0f - fb) * dc0.y + (1.0f - fe) * dc1.y;
+        dcf0[2] += (1.0f - fb) * dc0.z + (1.0f - fe) * dc1.z;
+        dcf1[0] += fb * dc0.x + fe * dc1.x;
+        dcf1[1] += fb * dc0.y + fe * dc1.y;
+        dcf1[2] += fb * dc0.z + fe * dc1.z;
+        dcf0[3] += (1.0f - fm) * da;
+        dcf1[3] += fm * da;
+    }
+    // segmentLengthAdjoint: t = |P0 - P1| with P0, P1 on fixed barycentrics of the entry / exit faces
+    const int fv[2][3] = {{face_vertex(E.f0, 0), face_vertex(E.f0, 1), face_vertex(E.f0, 2)},
+                          {face_vertex(E.f1, 0), face_vertex(E.f1, 1), face_vertex(E.f1, 2)}};
+    float3_ q[2][3];
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) q[sd][k] = lds_p(L, fv[sd][k], j);
+    const float uu[2] = {E.u0, E.u1}, vv[2] = {E.v0, E.v1};
+    const float ww[2] = {1.0f - E.u0 - E.v0, 1.0f - E.u1 - E.v1};
+    const float3_ D = sub(add(add(scale(q[0][0], uu[0]), scale(q[0][1], vv[0])), scale(q[0][2], ww[0])),
+                          add(add(scale(q[1][0], uu[1]), scale(q[1][1], vv[1])), scale(q[1][2], ww[1])));
+    const float invDen = 1.0f / sqrtf(fmaxf(dot(D, D), 1e-6f));
+#pragma unroll
+    for (int q28 = 0; q28 < 28; q28++) g[q28] = 0.f;
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+        const float sgn = sd == 0 ? 1.0f : -1.0f;
+        const float du_s = dOut_dt * (sgn * dot(sub(q[sd][0], q[sd][2]), D) * invDen);
+        const float dv_s = dOut_dt * (sgn * dot(sub(q[sd][1], q[sd][2]), D) * invDen);
+        const float wts[3] = {uu[sd], vv[sd], ww[sd]};
+        const float* dcf = sd == 0 ? dcf0 : dcf1;
+        const float3_ Pp = add(ro, scale(rd, sd == 0 ? E.t0 : E.t1));
+        const float4 c0 = lds_c(L, fv[sd][0], j), c1 = lds_c(L, fv[sd][1], j), c2 = lds_c(L, fv[sd][2], j);
+        const float du = du_s + (((dcf[0] * (c0.x - c2.x) + dcf[1] * (c0.y - c2.y)) + dcf[2] * (c0.z - c2.z)) +
+                                 dcf[3] * (c0.w - c2.w));
+        const float dv = dv_s + (((dcf[0] * (c1.x - c2.x) + dcf[1] * (c1.y - c2.y)) + dcf[2] * (c1.z - c2.z)) +
+                                 dcf[3] * (c1.w - c2.w));
+        float3_ gu[3], gv[3];
+        ref_bary_grads(Pp, q[sd][0], q[sd][1], q[sd][2], gu, gv);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float3_ gp = scale(D, dOut_dt * sgn * wts[k] * invDen);
+            gp = add(gp, add(scale(gu[k], du), scale(gv[k], dv)));
+            float gc[4] = {dcf[0] * wts[k], dcf[1] * wts[k], dcf[2] * wts[k], dcf[3] * wts[k]};
+            if (use_abs) {
+                gp = f3(fabsf(gp.x), fabsf(gp.y), fabsf(gp.z));
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) gc[ch] = fabsf(gc[ch]);
+            }
+            const int v = fv[sd][k];
+            // v is a runtime face-table lookup: select its 7 slots without dynamic register indexing
+#pragma unroll
+            for (int vv2 = 0; vv2 < 4; vv2++) {
+                if (vv2 != v) continue;
+                g[7 * vv2 + 0] += gp.x;
+                g[7 * vv2 + 1] += gp.y;
+                g[7 * vv2 + 2] += gp.z;
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) g[7 * vv2 + 3 + ch] += gc[ch];
+            }
+        }
+    }
+}
+
 }  // namespace dtv
