@@ -203,6 +203,112 @@ def _delaunay_job(args):
     return pts, cells
 
 
+def bench_uneven(args):
+    """Uneven batch: 8 Delaunay meshes of N in {2k, 8k, 32k, 64k, 128k, 256k, 384k, 512k} seeded points in the
+    unit ball (3-octave value-noise density in [0, 40]), one 768 x 768 view each, rendered and differentiated as
+    one batch (loss = sum of per-view MSEs against renders of a second colour field) with Adam on every mesh;
+    against the same work as 8 single-view steps."""
+    import multiprocessing
+
+    import numpy as np
+    import torch
+
+    import difftetvr_amd as d
+    from difftetvr_amd import meshgen
+    from difftetvr_amd.optim import FusedAdam, mse_loss_grad, mse_loss_grad_views
+
+    W = H = args.res or 768
+    ns = [2_000, 8_000, 32_000, 64_000, 128_000, 256_000, 384_000, 512_000]
+    t0 = time.perf_counter()
+    with multiprocessing.get_context("spawn").Pool(min(8, os.cpu_count() or 1)) as pool:
+        dels = pool.map(_delaunay_job, [(n, 500 + i) for i, n in enumerate(ns)])
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(1)
+    views, meshes, targets, tets = [], [], [], 0
+    for i, (pts, cells) in enumerate(dels):
+        e = rng.normal(size=3)
+        views.append(meshgen.look_at_view_matrix(tuple(e / np.linalg.norm(e) * 2.5)))
+        col = meshgen.blob_colors(pts, seed=600 + i)
+        col[:, 3] = meshgen.value_noise(pts, seed=700 + i, vmax=40.0)
+        gt = meshgen.blob_colors(pts, seed=800 + i)
+        gt[:, 3] = col[:, 3]
+        m = d.TetMesh()
+        m.set_use_gradients(True)
+        m.set_tet_mesh_data(cells, pts, col)
+        mt = d.TetMesh()
+        mt.set_tet_mesh_data(cells, pts, gt)
+        meshes.append(m)
+        targets.append(mt)
+        tets += int(cells.shape[0])
+
+    def batch_of(ms):
+        br = d.BatchRenderer()
+        br.set_attenuation(1.0)
+        br.set_camera_fovy(math.pi / 4.0)
+        br.set_items([(m, v, W, H) for m, v in zip(ms, views)])
+        return br
+
+    target_imgs = batch_of(targets).render().detach().clone()
+    br = batch_of(meshes)
+    opts = [FusedAdam([{"params": [m.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
+                       {"params": [m.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True) for m in meshes]
+
+    def batched_step():
+        imgs = br.render()
+        _, dimg = mse_loss_grad_views(imgs, target_imgs)
+        br.render_adjoint(dimg, False)
+        for o in opts:
+            o.step()
+
+    singles = []
+    for m, v in zip(meshes, views):
+        r = d.Renderer(d.RendererType.INTERSECTION)
+        r.set_tet_mesh(m)
+        r.set_attenuation(1.0)
+        r.set_viewport_size(W, H)
+        r.set_camera_fovy(math.pi / 4.0)
+        r.set_view_matrix(v)
+        singles.append(r)
+
+    def single_steps():
+        for i, r in enumerate(singles):
+            img = r.render()
+            _, dimg = mse_loss_grad(img, target_imgs[i])
+            r.render_adjoint(dimg, False)
+            opts[i].step()
+
+    for _ in range(args.warmup):
+        batched_step()
+        single_steps()
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for _ in range(args.steps):
+        batched_step()
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    for _ in range(args.steps):
+        single_steps()
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    br.set_timing(True)
+    batched_step()
+    stage_ms = br.get_timings()
+    br.set_timing(False)
+    st = br.get_stats()
+    ms_b = (tb - ta) * 1000.0 / args.steps
+    ms_s = (tc - tb) * 1000.0 / args.steps
+    out = {
+        "metric": "uneven 8-mesh batch fwd+adjoint+Adam ms/step, Delaunay 2k..512k points @768x768",
+        "value": ms_b, "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": False, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "8 Delaunay meshes in the unit ball (value-noise density [0,40]), one 768^2 view "
+                               "each, one batch per step", "tets": tets, "tile_pairs": st["num_pairs"],
+                   "single_view_ms_per_step": ms_s, "speedup_vs_single_view": ms_s / ms_b,
+                   "mesh_setup_s": round(t_setup, 1), "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_serve(args):
     """Serving: a Poisson stream of render requests at a fixed rate over 32 resident meshes (16 cube grids of
     16^3..80^3 cells x 6 tets, 16 Delaunay meshes of 10k..500k points in the unit ball), each request a seeded
@@ -328,9 +434,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--res", type=int, default=None, help="viewport edge (default 1024; 512 for multiview)")
-    ap.add_argument("--workload", choices=["single", "multiview", "serve"], default="single",
+    ap.add_argument("--workload", choices=["single", "multiview", "uneven", "serve"], default="single",
                     help="single: the plain single-view step (default); multiview: 16 views per batched step; "
-                         "serve: Poisson request stream over 32 resident meshes")
+                         "uneven: 8 Delaunay meshes in one batch; serve: Poisson request stream over 32 meshes")
     ap.add_argument("--qps", type=float, default=150.0, help="serve: offered requests per second")
     ap.add_argument("--duration", type=float, default=8.0, help="serve: seconds of request arrivals")
     ap.add_argument("--views", type=int, default=16)
@@ -359,6 +465,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "uneven":
+        bench_uneven(args)
         return
 
     import numpy as np
