@@ -34,6 +34,30 @@ struct FragListLds {
     float rd[3][64];            // ray direction of each pixel
 };
 
+// Wave-wide comparisons of 5-bit keys from their bit ballots kb[b] = lanes whose key has bit b (valid lanes
+// only).  key_equal_mask: the valid lanes whose key equals k.  key_rank_start: the number of valid lanes whose
+// key is below k; `count` receives the number equal to k.
+__device__ __forceinline__ uint64_t key_equal_mask(const uint64_t kb[5], uint64_t valid, int k) {
+    uint64_t eq = valid;
+#pragma unroll
+    for (int b = 0; b < 5; b++) eq &= ((k >> b) & 1) ? kb[b] : ~kb[b];
+    return eq;
+}
+__device__ __forceinline__ uint32_t key_rank_start(const uint64_t kb[5], uint64_t valid, int k, uint32_t& count) {
+    uint64_t lt = 0, eq = valid;
+#pragma unroll
+    for (int b = 4; b >= 0; b--) {
+        if ((k >> b) & 1) {
+            lt |= eq & ~kb[b];
+            eq &= kb[b];
+        } else {
+            eq &= ~kb[b];
+        }
+    }
+    count = (uint32_t)popc64(eq);
+    return (uint32_t)popc64(lt);
+}
+
 // Coverage masks -> dense fragment list.  Each lane's fragments are contiguous, ordered by record index
 // (ascending, or descending when `reverse`).  Returns the lane's first entry; `total` = all fragments.
 __device__ __forceinline__ uint32_t build_fragment_list(FragListLds& F, uint32_t mask, bool reverse, int lane,
@@ -172,7 +196,8 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
 // 'over' operator is inverted per sub-segment 3, 2, 1 (sub-segment 0 is skipped) with no checkpoints, and
 // each fragment's parameter gradients follow frag_backward_reference.
 template <bool kRef>
-__global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restrict__ items,
+// At least 2 waves per SIMD: the register budget is capped at 256 (the unified VGPR/AGPR file holds 512).
+__global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __restrict__ items,
                                                         const AdjointTarget* __restrict__ targets, int nitems,
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all,
@@ -215,13 +240,20 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
     float dC[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t n = 0, last = 0;
     int jck = -1;                // next checkpoint slot (back to front)
-    uint32_t ck_n = 0xFFFFFFFFu;  // its fragment count, cached: no global load per fragment
+    // Checkpoint jck's fragment count and colour, and the count of the one before it, are held in registers:
+    // the next checkpoint is always loaded well before it is needed.
+    uint32_t ck_n = 0xFFFFFFFFu, ck_n2 = 0xFFFFFFFFu;
+    float4 ck_c = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid) {
         C = ps.ray_color[pix];
         n = ps.frag_count[pix];
         last = ps.last_rec[pix];
         jck = (int)ps.num_ckpt[pix] - 1;
-        if (jck >= 0) ck_n = ps.ckpt_n[(int64_t)jck * P + pix];
+        if (jck >= 0) {
+            ck_n = ps.ckpt_n[(int64_t)jck * P + pix];
+            ck_c = ps.ckpt_color[(int64_t)jck * P + pix];
+            if (jck >= 1) ck_n2 = ps.ckpt_n[(int64_t)(jck - 1) * P + pix];
+        }
         const float4 di = dimage[pix];
         dC[0] = di.x;
         dC[1] = di.y;
@@ -311,15 +343,19 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
                     continue;
                 }
                 if (jck >= 0 && ck_n == n) {
-                    C = ps.ckpt_color[(int64_t)jck * P + pix];
+                    C = ck_c;
                     jck--;
-                    ck_n = jck >= 0 ? ps.ckpt_n[(int64_t)jck * P + pix] : 0xFFFFFFFFu;
+                    ck_n = ck_n2;
+                    if (jck >= 0) {
+                        ck_c = ps.ckpt_color[(int64_t)jck * P + pix];
+                        ck_n2 = jck >= 1 ? ps.ckpt_n[(int64_t)(jck - 1) * P + pix] : 0xFFFFFFFFu;
+                    }
                 }
                 if (s_hit[e - r0]) {
                     const float4 f = F.val[e - r0];
                     float4 Cin;
                     if (jck >= 0 && ck_n == n - 1) {
-                        Cin = ps.ckpt_color[(int64_t)jck * P + pix];
+                        Cin = ck_c;
                     } else {
                         const float ain = (f.w - C.w) / (f.w - 1.0f);
                         Cin = make_float4(C.x - (1.0f - ain) * f.x, C.y - (1.0f - ain) * f.y,
@@ -375,19 +411,18 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
                     key = j;
                 }
             }
-            // counting sort of the 64 fragments by record (ballots): row = start[key] + rank in record
-            uint32_t run = 0, row = 0, seg0 = 0, segn = 0;
-            const uint64_t lt = lanemask_lt();
-#pragma unroll 4
-            for (int jj = 0; jj < kBatch; jj++) {
-                const uint64_t b = ballot(key == jj);
-                const uint32_t c = (uint32_t)popc64(b);
-                if (key == jj) row = run + (uint32_t)popc64(b & lt);
-                if ((lane & 31) == jj) {
-                    seg0 = run;
-                    segn = c;
-                }
-                run += c;
+            // counting sort of the 64 fragments by record: row = (fragments of lower records) + (rank among the
+            // same record's fragments in lane order); the owner lanes of record j get its segment [seg0, +segn)
+            uint32_t row = 0, seg0 = 0, segn = 0;
+            {
+                uint64_t kb[5];
+#pragma unroll
+                for (int b = 0; b < 5; b++) kb[b] = ballot(key < kBatch && ((key >> b) & 1));
+                const uint64_t valid = ballot(key < kBatch);
+                uint32_t own_count;
+                row = key_rank_start(kb, valid, key < kBatch ? key : 0, own_count);
+                row += (uint32_t)popc64(key_equal_mask(kb, valid, key < kBatch ? key : 0) & lanemask_lt());
+                seg0 = key_rank_start(kb, valid, lane & 31, segn);
             }
             if (key < kBatch) {
 #pragma unroll
@@ -403,18 +438,73 @@ __global__ __launch_bounds__(64) void k_render_adjoint(const BatchItem* __restri
             }
             __syncthreads();
         }
-        if (bi > 0 && lane < kBatch) fetch_record_data(pos, col, nxt, nd);
-        // flush: lanes j and j + 32 share record j (first / second half of its 28 values)
+        // Flush.  Lanes j and j + 32 hold record j's tet vertices 0, 1 and 2, 3 (7 values each): 128 vertex
+        // slots, slot s = v * 32 + j.  Depth-adjacent tets share most vertices (typically 6-7 slots per distinct
+        // vertex in a batch), so the slots of one vertex are summed in LDS, in slot order, by the slot that
+        // comes first, and only that slot issues the global atomics: far fewer memory-side atomics.
         {
             const int j = lane & 31, half = lane >> 5;
-            if (j < cnt) {
+            float(*s_acc)[7] = reinterpret_cast<float(*)[7]>(&s_w[0][0]);  // 128 x 7 floats, free after the rounds
+            const int s0 = (2 * half) * 32 + j, s1 = s0 + 32;
 #pragma unroll
-                for (int q = 0; q < 14; q++) {
-                    const int qq = half * 14 + q;
-                    const int v = qq / 7, c = qq % 7;
-                    const float val = acc[q];
+            for (int c = 0; c < 7; c++) {
+                s_acc[s0][c] = acc[c];
+                s_acc[s1][c] = acc[7 + c];
+            }
+            __syncthreads();
+            const int32_t* vids = &L.vid[0][0];
+            int32_t v0 = 0, v1 = 0;
+            float sum[2][7];
+            bool lead[2] = {false, false};
+            if (j < cnt) {
+                const int4* vids4 = reinterpret_cast<const int4*>(vids);
+                v0 = vids[s0];
+                v1 = vids[s1];
+                // 128-bit masks of the slots holding v0 / v1 (branch-free, broadcast LDS reads)
+                uint32_t m0[4] = {0u, 0u, 0u, 0u}, m1[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int q = 0; q < kBatch; q++) {
+                    const int4 w = vids4[q];  // slots 4q .. 4q + 3
+                    const uint32_t b0 = (uint32_t)(w.x == v0) | ((uint32_t)(w.y == v0) << 1) |
+                                        ((uint32_t)(w.z == v0) << 2) | ((uint32_t)(w.w == v0) << 3);
+                    const uint32_t b1 = (uint32_t)(w.x == v1) | ((uint32_t)(w.y == v1) << 1) |
+                                        ((uint32_t)(w.z == v1) << 2) | ((uint32_t)(w.w == v1) << 3);
+                    m0[q >> 3] |= b0 << (4 * (q & 7));
+                    m1[q >> 3] |= b1 << (4 * (q & 7));
+                }
+                const uint32_t live = cnt >= 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u);  // slots of records j < cnt
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const uint32_t* m = k == 0 ? m0 : m1;
+                    const int sk = k == 0 ? s0 : s1;
+                    int first = -1;
+#pragma unroll
+                    for (int c = 0; c < 7; c++) sum[k][c] = 0.f;
+#pragma unroll
+                    for (int wd = 0; wd < 4; wd++) {
+                        uint32_t bits = m[wd] & live;
+                        while (bits) {
+                            const int i = 32 * wd + __ffs(bits) - 1;
+                            bits &= bits - 1u;
+                            if (first < 0) first = i;
+#pragma unroll
+                            for (int c = 0; c < 7; c++) sum[k][c] += s_acc[i][c];
+                        }
+                    }
+                    lead[k] = first == sk;  // else another slot of this vertex issues its atomics
+                }
+            }
+            // The next batch's vertex data is requested before the atomics, so staging it does not wait for them.
+            // Only lanes 0..31 hold that batch's record words.
+            if (bi > 0 && lane < kBatch) fetch_record_data(pos, col, nxt, nd);
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                if (!lead[k]) continue;
+                const int64_t g = k == 0 ? v0 : v1;
+#pragma unroll
+                for (int c = 0; c < 7; c++) {
+                    const float val = sum[k][c];
                     if (val == 0.f) continue;
-                    const int64_t g = L.vid[v][j];
                     if (c < 3)
                         atomic_add_f32(&grad_pos[3 * g + c], val);
                     else

@@ -8,7 +8,7 @@ namespace dtv {
 constexpr int kTileEdge = 8;
 constexpr int kBatch = 32;  // records staged per batch (one 32-bit coverage mask per pixel)
 
-struct RecordLds {
+struct alignas(16) RecordLds {  // 16-byte aligned: vid[] is also read as int4
     float p[12][kBatch];      // tet vertex positions, [3*v + c]
     float c[16][kBatch];      // tet vertex colours, [4*v + ch]
     float N[12][kBatch];      // face plane normals cross(p1-p0, p2-p0), [3*f + c]
