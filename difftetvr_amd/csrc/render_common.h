@@ -125,8 +125,9 @@ __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1
 __device__ __forceinline__ void bary_ref(float3_ p0, float3_ p1, float3_ p2, float3_ P, float& u, float& v) {
     const float3_ d20 = sub(p2, p0), d21 = sub(p2, p1);
     const float area = fmaxf(length(cross(d20, d21)), 1e-5f);
-    u = length(cross(d21, sub(P, p1))) / area;
-    v = length(cross(sub(P, p0), d20)) / area;
+    // continuous quantities (no discrete decision depends on them): fast division
+    u = __fdividef(length(cross(d21, sub(P, p1))), area);
+    v = __fdividef(length(cross(sub(P, p0), d20)), area);
 }
 
 // Inside test of rayTriangleIntersect (RayIntersectionTests.glsl:45-68) for face g at ray parameter t (the t
@@ -244,8 +245,9 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
         const float a = mixf(E.col0.w, E.col1.w, fm) * att;
         float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!(a < 1e-6f)) {
-            const float A = expf(-a * tSeg);
-            const float kk = (tSeg + 1.0f / a) * A - 1.0f / a;
+            const float A = __expf(-a * tSeg);
+            const float ia = __fdividef(1.0f, a);
+            const float kk = (tSeg + ia) * A - ia;
             cur = make_float4((1.0f - A) * c0x + kk * (c0x - c1x), (1.0f - A) * c0y + kk * (c0y - c1y),
                               (1.0f - A) * c0z + kk * (c0z - c1z), 1.0f - A);
         }
@@ -287,7 +289,7 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
     const float4 ca = lds_c(L, ia, j), cb = lds_c(L, ib, j), cc = lds_c(L, ic, j);
     const float3_ e1 = sub(b, a), e2 = sub(c, a);
     const float3_ n = cross(e1, e2);
-    const float inn = 1.0f / dot(n, n);
+    const float inn = __fdividef(1.0f, dot(n, n));
     const float3_ gb = scale(cross(e2, n), inn), gc = scale(cross(n, e1), inn);
     // G^T dC and dC . (G d)
     const float db[4] = {cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w};
@@ -328,8 +330,8 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
         const float a = mixf(E.col0.w, E.col1.w, fm) * att;
         float dc0[3] = {0.f, 0.f, 0.f}, dc1[3] = {0.f, 0.f, 0.f}, da = 0.f, dts = 0.f;
         if (!(a < 1e-6f)) {
-            const float A = expf(-a * tSeg);
-            const float ia = 1.0f / a;
+            const float A = __expf(-a * tSeg);
+            const float ia = __fdividef(1.0f, a);
             const float K = (tSeg + ia) * A - ia;
             const float dKda = ia * ia * (1.0f - A) - tSeg * tSeg * A - tSeg * ia * A;
 #pragma unroll
