@@ -362,6 +362,19 @@ class Renderer:
     def set_checkpoint_interval(self, k):
         self._checkpoint_interval = int(k)
 
+    def set_record_capacity(self, capacity):
+        """Planned rendering (``dtv_view_set_record_capacity``): > 0 sizes the (tile, face) record buffers for
+        ``capacity`` records and removes the per-render host synchronisation, so that render / adjoint / Adam
+        can be captured in a CUDA/HIP graph (after one ordinary call).  0 = off."""
+        _lib.check(self._lib.dtv_view_set_record_capacity(self._view, int(capacity)), "set_record_capacity")
+
+    def records_overflowed(self):
+        """(overflowed, records) of the last render; synchronises the device."""
+        o, k = ctypes.c_int32(), ctypes.c_int64()
+        _lib.check(self._lib.dtv_view_records_overflowed(self._view, ctypes.byref(o), ctypes.byref(k)),
+                   "records_overflowed")
+        return bool(o.value), int(k.value)
+
     def _camera(self):
         cam = _lib.DtvCamera()
         for i in range(16):

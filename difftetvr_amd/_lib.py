@@ -85,9 +85,12 @@ SIGNATURES = [
     ("dtv_view_fragment_counts", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("dtv_view_export_fragments", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_int32, c_void_p, c_void_p]),
     ("dtv_view_get_stats", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvViewStats)]),
+    ("dtv_view_set_record_capacity", ctypes.c_int, [c_void_p, c_int64]),
+    ("dtv_view_records_overflowed", ctypes.c_int, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int64)]),
     ("dtv_view_set_timing", ctypes.c_int, [c_void_p, c_int32]),
     ("dtv_view_get_timings", ctypes.c_int, [c_void_p, ctypes.POINTER(c_float)]),
     ("dtv_adam_step", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p]),
+    ("dtv_adam_step_device", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p, c_void_p]),
     ("dtv_tet_regularizer_grad", ctypes.c_int, [ctypes.POINTER(DtvMesh), c_float, c_float, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad_batch", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),

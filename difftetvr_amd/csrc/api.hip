@@ -150,6 +150,7 @@ struct dtv_view {
         recs, hist,
         lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
+    int64_t planned_cap = 0;  // > 0: planned (synchronisation-free) rendering with this record capacity
     // state of the last render
     bool rendered = false;
     int64_t total_blocks = 0, total_tiles = 0, total_pixels = 0;
@@ -292,13 +293,28 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     launch_reduce_bin_stats(view->block_stats.as<uint32_t>(), blocks, scan_total, dstats, s);
     view->mark(DTV_STAGE_SCAN, 1, s);
     DTV_HIP(hipGetLastError());
-    DTV_HIP(hipMemcpyAsync(view->pinnedStats, dstats, sizeof(BinStats), hipMemcpyDeviceToHost, s));
-    DTV_HIP(hipStreamSynchronize(s));
-    const BinStats st = *view->pinnedStats;
-    if (st.total_pairs >= (1ull << 30)) return fail(DTV_ERR_OUT_OF_MEMORY, f + ": more than 2^30 (tile, face) records");
-    const int64_t K = (int64_t)st.total_pairs;
-    view->num_pairs = K;
-    view->num_tris = st.num_tris;
+    // The record count: read back after one synchronisation, or, in planned mode, a fixed capacity with the
+    // count left in device memory (scan_total) for the kernels to read.
+    const bool planned = view->planned_cap > 0;
+    BinStats st{};
+    int64_t K;
+    if (!planned) {
+        DTV_HIP(hipMemcpyAsync(view->pinnedStats, dstats, sizeof(BinStats), hipMemcpyDeviceToHost, s));
+        DTV_HIP(hipStreamSynchronize(s));
+        st = *view->pinnedStats;
+        if (st.total_pairs >= (1ull << 30))
+            return fail(DTV_ERR_OUT_OF_MEMORY, f + ": more than 2^30 (tile, face) records");
+        K = (int64_t)st.total_pairs;
+        view->num_pairs = K;
+        view->num_tris = st.num_tris;
+    } else {
+        K = view->planned_cap;
+        view->num_pairs = -1;
+        view->num_tris = -1;
+    }
+    const uint64_t* n_dev = planned ? scan_total : nullptr;
+    uint32_t* overflow = reinterpret_cast<uint32_t*>(view->small.as<char>() + 192);
+    fill_u32(overflow, 0u, 1, s);
 
     // 3. emit, 4. sort, 5. tile ranges
     DTV_HIP(view->keys.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
@@ -309,10 +325,12 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->recs.ensure(2 * sizeof(uint4) * (K > 0 ? K : 1)));
     view->mark(DTV_STAGE_EMIT, 0, s);
     launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
-                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), view->rec_id.as<uint32_t>(), s);
+                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), view->rec_id.as<uint32_t>(), K, overflow,
+                      s);
     view->mark(DTV_STAGE_EMIT, 1, s);
     uint32_t mask = 0;
-    const uint32_t varying = st.key_and ^ st.key_or;
+    // planned: every depth digit is sorted (which of them vary is not known on the host)
+    const uint32_t varying = planned ? 0xFFFFFFFFu : (st.key_and ^ st.key_or);
     for (int p = 0; p < 4; p++)
         if ((varying >> (8 * p)) & 255u) mask |= 1u << p;
     int tile_bits = 0;
@@ -325,7 +343,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                                   (size_t)(sort_partitions(K) + 1)));
     view->mark(DTV_STAGE_SORT, 0, s);
     view->sort_passes = radix_sort_pairs(view->keys.as<uint64_t>(), view->vals.as<uint32_t>(),
-                                         view->keys2.as<uint64_t>(), view->vals2.as<uint32_t>(), K, mask,
+                                         view->keys2.as<uint64_t>(), view->vals2.as<uint32_t>(), K, n_dev, mask,
                                          view->hist.as<uint32_t>(), view->lookback.as<uint32_t>(),
                                          view->hist.as<uint32_t>() + 8 * 256, s);
     view->mark(DTV_STAGE_SORT, 1, s);
@@ -333,9 +351,10 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     const uint64_t* sorted_keys = view->result_in_alt ? view->keys2.as<uint64_t>() : view->keys.as<uint64_t>();
     DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
     view->mark(DTV_STAGE_RANGES, 0, s);
-    launch_tile_ranges(sorted_keys, K, tiles, view->ranges.as<uint2>(), s);
+    launch_tile_ranges(sorted_keys, K, n_dev, tiles, view->ranges.as<uint2>(), s);
     const uint32_t* sorted_idx = view->result_in_alt ? view->vals2.as<uint32_t>() : view->vals.as<uint32_t>();
-    launch_gather_records(dI, n, sorted_keys, sorted_idx, view->rec_id.as<uint32_t>(), K, view->recs.as<uint4>(), s);
+    launch_gather_records(dI, n, sorted_keys, sorted_idx, view->rec_id.as<uint32_t>(), K, n_dev,
+                          view->recs.as<uint4>(), s);
     view->mark(DTV_STAGE_RANGES, 1, s);
 
     // 6. render
@@ -372,8 +391,12 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     }
     DTV_HIP(hipSetDevice(view->ctx->device));
     if (!view->targets_copied) DTV_HIP(hipEventCreateWithFlags(&view->targets_copied, hipEventDisableTiming));
-    // the pinned staging of the previous call must have been consumed before it is rewritten
-    if (view->targets_pending) DTV_HIP(hipEventSynchronize(view->targets_copied));
+    // Under stream capture the upload becomes a graph node that reads the staging at every replay (nothing may
+    // wait on the device then); otherwise the staging of the previous call must have been consumed first.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    DTV_HIP(hipStreamIsCapturing(s, &capture));
+    const bool capturing = capture != hipStreamCaptureStatusNone;
+    if (view->targets_pending && !capturing) DTV_HIP(hipEventSynchronize(view->targets_copied));
     view->targets_pending = false;
     if (view->pinned_targets_cap < n) {
         if (view->pinnedTargets) DTV_HIP(hipHostFree(view->pinnedTargets));
@@ -386,8 +409,10 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     DTV_HIP(view->dTargets.ensure(sizeof(AdjointTarget) * (size_t)n));
     DTV_HIP(hipMemcpyAsync(view->dTargets.p, view->pinnedTargets, sizeof(AdjointTarget) * (size_t)n,
                            hipMemcpyHostToDevice, s));
-    DTV_HIP(hipEventRecord(view->targets_copied, s));
-    view->targets_pending = true;
+    if (!capturing) {
+        DTV_HIP(hipEventRecord(view->targets_copied, s));
+        view->targets_pending = true;
+    }
     view->mark(DTV_STAGE_ADJOINT, 0, s);
     launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
                           view->sorted_recs(), view->ranges.as<uint2>(), view->pixel_state(), use_abs_grad ? 1 : 0,
@@ -543,8 +568,37 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
     return DTV_OK;
 }
 
+int dtv_view_set_record_capacity(dtv_view* view, int64_t record_capacity) {
+    if (!view || record_capacity < 0 || record_capacity >= (int64_t)(1u << 30))
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_set_record_capacity: capacity must be 0..2^30-1");
+    view->planned_cap = record_capacity;
+    return DTV_OK;
+}
+
+int dtv_view_records_overflowed(dtv_view* view, int32_t* out_overflowed, int64_t* out_records) {
+    if (!view || !out_overflowed) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_records_overflowed: NULL argument");
+    if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_view_records_overflowed: view has not been rendered");
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    DTV_HIP(hipDeviceSynchronize());
+    uint64_t total = 0;
+    DTV_HIP(hipMemcpy(&total, view->small.as<char>() + 128, sizeof(total), hipMemcpyDeviceToHost));
+    BinStats st{};
+    DTV_HIP(hipMemcpy(&st, view->small.p, sizeof(st), hipMemcpyDeviceToHost));
+    view->num_pairs = (int64_t)total;
+    view->num_tris = st.num_tris;
+    const int64_t cap = view->planned_cap > 0 ? view->planned_cap : (int64_t)total;
+    *out_overflowed = (int64_t)total > cap ? 1 : 0;
+    if (out_records) *out_records = (int64_t)total;
+    return DTV_OK;
+}
+
 int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out) {
     if (!view || !out) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_get_stats: NULL argument");
+    if (view->rendered && view->num_pairs < 0) {  // planned render: the counts are still on the device
+        int32_t o = 0;
+        const int rc = dtv_view_records_overflowed(view, &o, nullptr);
+        if (rc != DTV_OK) return rc;
+    }
     out->num_triangles = view->num_tris;
     out->num_pairs = view->num_pairs;
     out->num_tiles = view->total_tiles;
@@ -582,19 +636,24 @@ int dtv_view_get_timings(dtv_view* view, float* out_ms) {
     return DTV_OK;
 }
 
-int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream) {
-    if (!groups || num_groups <= 0 || num_groups > 8)
-        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: 1..8 groups required");
+namespace {
+
+int adam(const dtv_adam_group* groups, int32_t num_groups, int64_t* step_dev, hipStream_t s, const char* fn) {
+    const std::string f(fn);
+    if (!groups || num_groups <= 0 || num_groups > 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": 1..8 groups required");
     AdamGroupDev g[8];
     for (int i = 0; i < num_groups; i++) {
         const dtv_adam_group& G = groups[i];
         if (G.numel < 0 || (G.numel > 0 && (!G.param || !G.grad || !G.exp_avg || !G.exp_avg_sq)))
-            return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: invalid group");
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid group");
         if (!aligned16(G.param) || !aligned16(G.grad) || !aligned16(G.exp_avg) || !aligned16(G.exp_avg_sq))
-            return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: buffers must be 16-byte aligned");
-        if (G.step < 1) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step: step must be >= 1");
-        const double bc1 = 1.0 - std::pow((double)G.beta1, (double)G.step);
-        const double bc2 = 1.0 - std::pow((double)G.beta2, (double)G.step);
+            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": buffers must be 16-byte aligned");
+        const int64_t step = step_dev ? 1 : G.step;  // with a device step count the host value is unused
+        if (step < 1) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": step must be >= 1");
+        const double bc1 = 1.0 - std::pow((double)G.beta1, (double)step);
+        const double bc2 = 1.0 - std::pow((double)G.beta2, (double)step);
+        g[i].lr = G.lr;
+        g[i].beta1 = G.beta1;
         g[i].param = G.param;
         g[i].grad = G.grad;
         g[i].m = G.exp_avg;
@@ -609,9 +668,20 @@ int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream
         g[i].clamp_nonneg = G.clamp_nonneg;
         g[i].zero_grad = G.zero_grad;
     }
-    launch_adam(g, num_groups, reinterpret_cast<hipStream_t>(stream));
+    launch_adam(g, num_groups, step_dev, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;
+}
+
+}  // namespace
+
+int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream) {
+    return adam(groups, num_groups, nullptr, reinterpret_cast<hipStream_t>(stream), "dtv_adam_step");
+}
+
+int dtv_adam_step_device(const dtv_adam_group* groups, int32_t num_groups, int64_t* step_state, void* stream) {
+    if (!step_state) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_adam_step_device: NULL step_state");
+    return adam(groups, num_groups, step_state, reinterpret_cast<hipStream_t>(stream), "dtv_adam_step_device");
 }
 
 int dtv_tet_regularizer_grad(const dtv_mesh* mesh, float lambda, float beta, float* grad_positions, void* stream) {

@@ -208,11 +208,13 @@ __global__ __launch_bounds__(256) void k_reduce_bin_stats(const uint4* __restric
 
 // Same grid as k_count_pairs: writes the tet's records at offsets[slot], faces in order, tiles row-major.
 // keys/vals are the sort input (vals = record index); rec_id[i] = tet * 4 + face of record i.
+// Records at index >= cap (the capacity of keys/vals/rec_id) are not written; *overflow is then set to 1.
 __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict__ items, int nitems,
                                                      const uint32_t* __restrict__ offsets,
                                                      const uint32_t* __restrict__ counts,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                     uint32_t* __restrict__ rec_id) {
+                                                     uint32_t* __restrict__ rec_id, uint64_t cap,
+                                                     uint32_t* __restrict__ overflow) {
     const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
     const ViewUniforms& U = B.U;
     const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
@@ -234,16 +236,22 @@ __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict_
         for (int ty = b.ty0; ty <= b.ty1 && left > 0; ty++)
             for (int tx = b.tx0; tx <= b.tx1 && left > 0; tx++) {
                 if (test && !tile_may_cover(b.S, tx, ty)) continue;
-                keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
-                vals[o] = o;
-                rec_id[o] = id;
+                if (o < cap) {
+                    keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
+                    vals[o] = o;
+                    rec_id[o] = id;
+                } else {
+                    *overflow = 1u;
+                }
                 o++;
                 left--;
             }
     }
 }
 
-__global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint2* __restrict__ ranges) {
+__global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n_cap, const uint64_t* __restrict__ n_dev,
+                              uint2* __restrict__ ranges) {
+    const int64_t n = run_count(n_cap, n_dev);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t t = (uint32_t)(keys[i] >> 32);
         if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != t) ranges[t].x = (uint32_t)i;
@@ -257,10 +265,11 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n, uint
 __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restrict__ items, int nitems,
                                                          const uint64_t* __restrict__ sorted_keys,
                                                          const uint32_t* __restrict__ sorted_idx,
-                                                         const uint32_t* __restrict__ rec_id, int64_t n,
+                                                         const uint32_t* __restrict__ rec_id, int64_t n_cap,
+                                                         const uint64_t* __restrict__ n_dev,
                                                          uint4* __restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= run_count(n_cap, n_dev)) return;
     const uint32_t id = rec_id[sorted_idx[i]];
     const uint32_t gtile = (uint32_t)(sorted_keys[i] >> 32);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
@@ -293,26 +302,28 @@ void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const
 }
 
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
-                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, hipStream_t s) {
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, int64_t cap,
+                       uint32_t* overflow, hipStream_t s) {
     if (nblocks <= 0) return;
     hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, offsets, counts, keys,
-                       vals, rec_id);
+                       vals, rec_id, (uint64_t)cap, overflow);
 }
 
-void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s) {
+void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
+                        uint2* ranges, hipStream_t s) {
     fill_u32(reinterpret_cast<uint32_t*>(ranges), 0u, 2 * num_tiles, s);
     if (n <= 0) return;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, sorted_keys, n, ranges);
+    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, sorted_keys, n, n_dev, ranges);
 }
 
 void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
-                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, uint4* out,
-                           hipStream_t s) {
+                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, const uint64_t* n_dev,
+                           uint4* out, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items, nitems,
-                       sorted_keys, sorted_idx, rec_id, n, out);
+                       sorted_keys, sorted_idx, rec_id, n, n_dev, out);
 }
 
 }  // namespace dtv

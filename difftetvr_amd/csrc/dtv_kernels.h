@@ -13,8 +13,11 @@ int64_t scan_partitions(int64_t n);
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* status, uint32_t* counter,
                         uint64_t* total, hipStream_t s);
 int64_t sort_partitions(int64_t n);
+// n_dev (may be NULL): the element count in device memory, clamped to n (the capacity the launches are sized
+// for); NULL sorts exactly n.
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                     uint32_t digit_mask, uint32_t* hist, uint32_t* lookback, uint32_t* counters, hipStream_t s);
+                     const uint64_t* n_dev, uint32_t digit_mask, uint32_t* hist, uint32_t* lookback,
+                     uint32_t* counters, hipStream_t s);
 
 // binning.hip
 struct BinStats {
@@ -33,14 +36,18 @@ void launch_count_pairs(const BatchItem* items, int nitems, int64_t nblocks, uin
 void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const uint64_t* scan_total,
                              BinStats* stats, hipStream_t s);
 // keys/vals: sort input (vals = record index); rec_id[i] = tet * 4 + face of record i.
+// Writes only records below cap (the buffers' capacity) and sets *overflow = 1 if any record lies beyond.
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
-                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, hipStream_t s);
-void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, int64_t num_tiles, uint2* ranges, hipStream_t s);
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, int64_t cap,
+                       uint32_t* overflow, hipStream_t s);
+// n: capacity the launch is sized for; n_dev (may be NULL): actual count in device memory (see run_count).
+void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
+                        uint2* ranges, hipStream_t s);
 // out[2i], out[2i + 1] = {tet vertex indices}, {face id, 8x8 coverage mask low, high, 0} of sorted record i: the
 // tile renderer's record stream.
 void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
-                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, uint4* out,
-                           hipStream_t s);
+                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, const uint64_t* n_dev,
+                           uint4* out, hipStream_t s);
 
 // render.hip
 // Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's
@@ -76,8 +83,11 @@ struct AdamGroupDev {
     float eps;
     int32_t clamp_nonneg;
     int32_t zero_grad;
+    float lr, beta1;    // for bias corrections computed on the device
 };
-void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s);
+// step_dev (may be NULL): steps taken so far in device memory; then the bias corrections use step_dev + 1
+// (step_size / bc2_sqrt of the groups are ignored) and a second launch increments it.
+void launch_adam(const AdamGroupDev* groups, int ngroups, int64_t* step_dev, hipStream_t s);
 void launch_mse(const float* img, const float* tgt, int64_t n, int64_t denom, float* dimg, float* loss,
                 hipStream_t s);
 

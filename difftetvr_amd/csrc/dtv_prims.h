@@ -12,6 +12,14 @@ __device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u,
 __device__ __forceinline__ uint64_t lanemask_lt() { return (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id())); }
 __device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
 
+// Element count of a launch sized for n_cap: n_dev (may be NULL) holds the actual count in device memory, read
+// at run time and clamped to n_cap, so that a launch sequence can be recorded before the count is known.
+__device__ __forceinline__ int64_t run_count(int64_t n_cap, const uint64_t* n_dev) {
+    if (n_dev == nullptr) return n_cap;
+    const uint64_t d = *n_dev;
+    return d < (uint64_t)n_cap ? (int64_t)d : n_cap;
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     const int lane = lane_id();

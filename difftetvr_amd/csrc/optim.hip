@@ -16,6 +16,7 @@ struct AdamLaunch {
     AdamGroupDev g[kMaxAdamGroups];
     int64_t block_start[kMaxAdamGroups + 1];  // first block of each group (in float4 units of work)
     int ngroups;
+    const int64_t* step_dev;  // non-NULL: steps taken so far, in device memory (the bias corrections use +1)
 };
 
 __device__ __forceinline__ float adam_one(float p, float gr, float& m, float& v, const AdamGroupDev& G) {
@@ -30,7 +31,13 @@ __device__ __forceinline__ float adam_one(float p, float gr, float& m, float& v,
 __global__ __launch_bounds__(256) void k_adam(AdamLaunch A) {
     int gi = 0;
     while (gi + 1 < A.ngroups && (int64_t)blockIdx.x >= A.block_start[gi + 1]) gi++;
-    const AdamGroupDev& G = A.g[gi];
+    AdamGroupDev G = A.g[gi];
+    if (A.step_dev != nullptr) {
+        // bias corrections of step t = taken + 1, in double as on the host
+        const double t = (double)(*A.step_dev + 1);
+        G.step_size = (float)((double)G.lr / (1.0 - pow((double)G.beta1, t)));
+        G.bc2_sqrt = (float)sqrt(1.0 - pow((double)G.beta2, t));
+    }
     const int64_t i4 = ((int64_t)blockIdx.x - A.block_start[gi]) * 256 + threadIdx.x;
     const int64_t n4 = G.n / 4;
     if (i4 < n4) {
@@ -60,9 +67,12 @@ __global__ __launch_bounds__(256) void k_adam(AdamLaunch A) {
     }
 }
 
-void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s) {
+__global__ void k_step_increment(int64_t* step) { *step += 1; }
+
+void launch_adam(const AdamGroupDev* groups, int ngroups, int64_t* step_dev, hipStream_t s) {
     AdamLaunch A{};
     A.ngroups = ngroups;
+    A.step_dev = step_dev;
     int64_t blocks = 0;
     for (int i = 0; i < ngroups; i++) {
         A.g[i] = groups[i];
@@ -71,8 +81,8 @@ void launch_adam(const AdamGroupDev* groups, int ngroups, hipStream_t s) {
         blocks += (work + 255) / 256;
     }
     A.block_start[ngroups] = blocks;
-    if (blocks == 0) return;
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, s, A);
+    if (blocks > 0) hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(256), 0, s, A);
+    if (step_dev != nullptr) hipLaunchKernelGGL(k_step_increment, dim3(1), dim3(1), 0, s, step_dev);
 }
 
 // dimg = 2 (img - tgt) / denom ; loss += sum((img - tgt)^2) / denom  (denom = n: the mean; denom = n / items:

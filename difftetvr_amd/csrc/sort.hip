@@ -101,8 +101,10 @@ int64_t sort_partitions(int64_t n) { return (n + kSortPart - 1) / kSortPart; }
 
 // Histograms of the digit positions selected in digit_mask (bit p -> bits [8p, 8p+8)).  hist[p][256].
 // One LDS histogram per wave keeps the atomics of a single heavy bin spread over four copies.
-__global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* __restrict__ keys, int64_t n,
-                                                          uint32_t digit_mask, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* __restrict__ keys, int64_t n_cap,
+                                                          const uint64_t* __restrict__ n_dev, uint32_t digit_mask,
+                                                          uint32_t* __restrict__ hist) {
+    const int64_t n = run_count(n_cap, n_dev);
     __shared__ uint32_t s_h[4][8][256];
     for (int i = threadIdx.x; i < 4 * 8 * 256; i += 256) (&s_h[0][0][0])[i] = 0;
     __syncthreads();
@@ -137,8 +139,10 @@ __global__ __launch_bounds__(256) void k_digit_offsets(uint32_t* __restrict__ hi
 
 __global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
         const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-        uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ digit_offsets,
-        uint32_t* __restrict__ lookback, uint32_t* __restrict__ part_counter) {
+        uint32_t* __restrict__ vout, int64_t n_cap, const uint64_t* __restrict__ n_dev, int shift,
+        const uint32_t* __restrict__ digit_offsets, uint32_t* __restrict__ lookback,
+        uint32_t* __restrict__ part_counter) {
+    const int64_t n = run_count(n_cap, n_dev);
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wcnt[kSortWaves][256];
     __shared__ uint32_t s_local_start[256];
@@ -246,7 +250,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
 // (keys_alt, vals_alt).  hist must hold 8*256 u32; lookback (#passes * partitions * 256) u32; counters 8 u32.
 // Returns the number of passes; the result is in the alternate buffers when that number is odd.
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
-                     uint32_t digit_mask, uint32_t* hist, uint32_t* lookback, uint32_t* counters, hipStream_t s) {
+                     const uint64_t* n_dev, uint32_t digit_mask, uint32_t* hist, uint32_t* lookback,
+                     uint32_t* counters, hipStream_t s) {
     if (n <= 1 || digit_mask == 0) return 0;
     const int64_t parts = sort_partitions(n);
     int npass = 0;
@@ -256,7 +261,8 @@ int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_
     fill_u32(lookback, 0u, (int64_t)npass * parts * 256, s);
     int64_t hblocks = (n + 255) / 256;
     if (hblocks > 1024) hblocks = 1024;
-    hipLaunchKernelGGL(k_digit_histogram, dim3((unsigned)hblocks), dim3(256), 0, s, keys, n, digit_mask, hist);
+    hipLaunchKernelGGL(k_digit_histogram, dim3((unsigned)hblocks), dim3(256), 0, s, keys, n, n_dev, digit_mask,
+                       hist);
     hipLaunchKernelGGL(k_digit_offsets, dim3(1), dim3(256), 0, s, hist, digit_mask);
     uint64_t* ka = keys;
     uint32_t* va = vals;
@@ -265,8 +271,8 @@ int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_
     int pass = 0;
     for (int p = 0; p < 8; p++) {
         if (!((digit_mask >> p) & 1)) continue;
-        hipLaunchKernelGGL(k_onesweep_pass, dim3((unsigned)parts), dim3(kSortBlock), 0, s, ka, va, kb, vb, n, 8 * p,
-                           hist + p * 256, lookback + (int64_t)pass * parts * 256, counters + pass);
+        hipLaunchKernelGGL(k_onesweep_pass, dim3((unsigned)parts), dim3(kSortBlock), 0, s, ka, va, kb, vb, n, n_dev,
+                           8 * p, hist + p * 256, lookback + (int64_t)pass * parts * 256, counters + pass);
         uint64_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
         pass++;

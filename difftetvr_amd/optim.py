@@ -13,9 +13,15 @@ from . import _lib
 
 
 class FusedAdam(torch.optim.Optimizer):
-    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clamp_nonneg=False, zero_grad_after=False):
+    """``device_step=True`` keeps the step count in device memory (``dtv_adam_step_device``), so that ``step()``
+    can be captured in a CUDA/HIP graph and replayed: every replay advances the bias corrections."""
+
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clamp_nonneg=False, zero_grad_after=False,
+                 device_step=False):
         defaults = dict(lr=lr, betas=betas, eps=eps, clamp_nonneg=clamp_nonneg, zero_grad_after=zero_grad_after)
         super().__init__(params, defaults)
+        self._device_step = bool(device_step)
+        self._step_state = []  # one device counter per launch of up to 8 groups
 
     @torch.no_grad()
     def step(self, closure=None):
@@ -49,10 +55,16 @@ class FusedAdam(torch.optim.Optimizer):
                 groups.append(g)
         lib = _lib.load()
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        for i in range(0, len(groups), 8):
+        for k, i in enumerate(range(0, len(groups), 8)):
             chunk = groups[i:i + 8]
             arr = (_lib.DtvAdamGroup * len(chunk))(*chunk)
-            _lib.check(lib.dtv_adam_step(arr, len(chunk), stream), "dtv_adam_step")
+            if self._device_step:
+                if len(self._step_state) <= k:
+                    self._step_state.append(torch.zeros(1, dtype=torch.int64, device=torch.device("cuda")))
+                _lib.check(lib.dtv_adam_step_device(arr, len(chunk), ctypes.c_void_p(self._step_state[k].data_ptr()),
+                                                    stream), "dtv_adam_step_device")
+            else:
+                _lib.check(lib.dtv_adam_step(arr, len(chunk), stream), "dtv_adam_step")
         return loss
 
 

@@ -160,6 +160,19 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
 
 int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out);
 
+/* Planned (synchronisation-free) rendering.  With record_capacity > 0, dtv_render / dtv_render_batch size the
+ * (tile, face) record buffers for that many records instead of synchronising on the count, sort all four
+ * depth-key digits, and let the kernels read the record count from device memory: nothing in a render (or an
+ * adjoint) waits for the device, so a whole training step can be captured in a HIP graph and replayed (after
+ * one ordinary call has sized every buffer).  If a render produces more records than the capacity, the extra
+ * records are dropped (that image is wrong; nothing outside the buffers is touched) and
+ * dtv_view_records_overflowed says so.  0 (default) = size exactly, with one synchronisation per render.
+ * No counterpart in the reference (its GPU work is recorded in Vulkan command buffers). */
+int dtv_view_set_record_capacity(dtv_view* view, int64_t record_capacity);
+/* Synchronises the device.  *out_overflowed = 1 if the last render produced more records than the capacity;
+ * *out_records (may be NULL) = the records it produced. */
+int dtv_view_records_overflowed(dtv_view* view, int32_t* out_overflowed, int64_t* out_records);
+
 /* Per-stage GPU timers (hipEvents recorded on the caller's stream around each stage of dtv_render /
  * dtv_render_adjoint).  No reference counterpart (the reference only has wall-clock opt_time,
  * pytests/train.py:366,434).  dtv_view_get_timings synchronises on the recorded events and returns the
@@ -178,6 +191,11 @@ int dtv_view_get_timings(dtv_view* view, float* out_ms);
 /* One fused launch over up to 8 parameter groups.  Replaces optimizer.step() + clamp (train.py:351-356)
  * and the GLSL Adam pass (Optimizer.glsl:136-169, OptimizerPass.cpp:120-177). */
 int dtv_adam_step(const dtv_adam_group* groups, int32_t num_groups, void* stream);
+
+/* The same update with the step count in device memory, for captured graphs: the groups' step fields are
+ * ignored; *step_state (device int64, caller-owned, 0 before the first step) counts the steps taken, the bias
+ * corrections use *step_state + 1, and the call increments it on the stream.  No reference counterpart. */
+int dtv_adam_step_device(const dtv_adam_group* groups, int32_t num_groups, int64_t* step_state, void* stream);
 
 /* Tet-quality regulariser: accumulates (+=) the gradient of sum over tets of the mirrored softplus
  * (lambda / beta) log(1 + exp(-beta q)), q = 6 sqrt(2) V / l_rms^3, into grad_positions [V,3].
