@@ -440,6 +440,8 @@ def main():
     ap.add_argument("--qps", type=float, default=150.0, help="serve: offered requests per second")
     ap.add_argument("--duration", type=float, default=8.0, help="serve: seconds of request arrivals")
     ap.add_argument("--views", type=int, default=16)
+    ap.add_argument("--graph", action="store_true",
+                    help="single: capture the step in a HIP graph (planned rendering) and replay it")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -501,33 +503,70 @@ def main():
     target = r_gt.render().detach().clone()
     mesh, r = make(w["pos"], True)
     opt = FusedAdam([{"params": [mesh.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
-                     {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True)
+                     {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True,
+                    device_step=args.graph)
+    if args.graph:
+        # planned rendering: record buffers sized from an ordinary render, no host synchronisation per step
+        r.render()
+        r.set_record_capacity(int(r.get_stats()["num_pairs"] * 1.25) + 4096)
 
-    def step():
+    # HIP events around each stage on the stream the kernels run on (torch's current stream), read after the
+    # timed region: no synchronisation inside it
+    ev = []
+
+    def step(record=False):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if record else None
+        if record:
+            e[0].record()
         img = r.render()
+        if record:
+            e[1].record()
         _, dimg = mse_loss_grad(img, target)
+        if record:
+            e[2].record()
         r.render_adjoint(dimg, False)
+        if record:
+            e[3].record()
         opt.step()
+        if record:
+            ev.append(e)
 
     for _ in range(args.warmup):
         step()
+    graph = None
+    if args.graph:
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step()
     torch.cuda.synchronize()
-    r.set_timing(True)
-    acc = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
-        for k, v in r.get_timings().items():
-            acc[k] = acc.get(k, 0.0) + v
+        if graph is not None:
+            graph.replay()
+        else:
+            step(record=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    r.set_timing(False)
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    stage_ms = {k: v / args.steps for k, v in acc.items()}
+    # per-stage breakdown of the pipeline (library timers) on a few further steps
+    r.set_timing(True)
+    acc = {}
+    for _ in range(3):
+        step(record=graph is not None)
+        for k, v in r.get_timings().items():
+            acc[k] = acc.get(k, 0.0) + v / 3
+    r.set_timing(False)
+    stage_ms = dict(acc)
+    if ev:  # the timed region's own events (the graph path measures its eager steps above)
+        stage_ms["render"] = sum(e[0].elapsed_time(e[1]) for e in ev) / len(ev)
+        stage_ms["adjoint"] = sum(e[2].elapsed_time(e[3]) for e in ev) / len(ev)
     dominant = max(("forward", "adjoint"), key=lambda k: stage_ms.get(k, 0.0))
     abytes = algorithmic_bytes(dominant, T, V, P)
     achieved = abytes / (stage_ms[dominant] * 1e-3) / 1e9
     stats = r.get_stats()
+    if args.graph and r.records_overflowed()[0]:
+        raise RuntimeError("bench: record capacity exceeded during the graph replays")
 
     out = {
         "metric": METRIC,
@@ -547,6 +586,7 @@ def main():
                                f"dir (0.3,0.4,1), attenuation 1, ERT on",
                    "tets": T, "vertices": V, "pixels": P, "tile_pairs": stats["num_pairs"],
                    "front_faces": stats["num_triangles"], "sort_passes": stats["sort_passes"],
+                   "hip_graph": bool(args.graph),
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
         "roofline": {"bound": "hbm", "kernel": f"k_render_{dominant}", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
