@@ -347,7 +347,8 @@ def bench_serve(args):
         meshes.append(m)
         tets.append(int(cells.shape[0]))
     t_setup = time.perf_counter() - t0
-    srv = Server(num_workers=2, max_batch=16, memory_budget=16 << 30, attenuation=1.0, fovy=math.pi / 4.0)
+    srv = Server(num_workers=args.workers, max_batch=args.max_batch, memory_budget=16 << 30, attenuation=1.0,
+                 fovy=math.pi / 4.0)
     ids = [srv.add_mesh(m) for m in meshes]
     sizes = [256, 512, 1024]
     nreq = int(args.qps * args.duration)
@@ -380,9 +381,9 @@ def bench_serve(args):
             if k >= have:
                 time.sleep(0.0002)
                 continue
-            tk, res = tickets[k]
+            tk, res, adj = tickets[k]
             _, lat, q = srv.wait(tk)
-            results.append((lat, q, res))
+            results.append((lat, q, res, adj))
             k += 1
 
     th = threading.Thread(target=collector)
@@ -394,13 +395,21 @@ def bench_serve(args):
             time.sleep(ta - now)
         tk = srv.submit(ids[mi], vm, res, res, target=targets[res] if adj else None, out=outs[res][i % 32])
         with lock:
-            tickets.append((tk, res))
+            tickets.append((tk, res, adj))
     th.join()
     elapsed = time.perf_counter() - start
     st = srv.stats()
     srv.close()
     lat = np.array([r[0] for r in results])
     lat1k = np.array([r[0] for r in results if r[2] == 1024])
+
+    def pct(sel):
+        v = np.array([r[0] for r in results if sel(r)])
+        return {"n": int(v.size), "p50_ms": float(np.percentile(v, 50)) if v.size else None,
+                "p99_ms": float(np.percentile(v, 99)) if v.size else None}
+
+    classes = {f"render_{s}": pct(lambda r, s=s: r[2] == s and not r[3]) for s in sizes}
+    classes.update({f"train_{s}": pct(lambda r, s=s: r[2] == s and r[3]) for s in sizes})
     out = {
         "metric": "serve p50/p99 ms (Poisson stream, 32 resident meshes)",
         "value": float(np.percentile(lat, 99)),
@@ -414,12 +423,15 @@ def bench_serve(args):
         "p99_ms": float(np.percentile(lat, 99)),
         "p99_ms_1024": float(np.percentile(lat1k, 99)) if lat1k.size else None,
         "p50_ms_1024": float(np.percentile(lat1k, 50)) if lat1k.size else None,
+        "p99_ms_1024_render": classes["render_1024"]["p99_ms"],
+        "by_class": classes,
         "mean_queue_ms": float(np.mean([r[1] for r in results])),
         "frames_per_s": len(results) / elapsed,
         "offered_qps": args.qps,
         "config": {"workload": "Poisson render requests over 16 grid meshes (16^3..80^3 x6) and 16 Delaunay meshes "
-                               "(10k..500k points), viewports {256,512,1024}^2, 10% with loss + adjoint, 2 workers, "
-                               "up to 16 requests per batch, 16 GiB in-flight budget",
+                               f"(10k..500k points), viewports {{256,512,1024}}^2, 10% with loss + adjoint, "
+                               f"{args.workers} workers, up to {args.max_batch} requests per batch, "
+                               "16 GiB in-flight budget",
                    "requests": nreq, "meshes": len(meshes), "tets_total": int(sum(tets)),
                    "mesh_setup_s": round(t_setup, 1), "batches": st["batches"],
                    "max_batch_seen": st["max_batch_seen"]},
@@ -440,6 +452,8 @@ def main():
     ap.add_argument("--qps", type=float, default=150.0, help="serve: offered requests per second")
     ap.add_argument("--duration", type=float, default=8.0, help="serve: seconds of request arrivals")
     ap.add_argument("--views", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=2, help="serve: worker threads (one stream and view each)")
+    ap.add_argument("--max-batch", type=int, default=16, help="serve: most requests per batch")
     ap.add_argument("--graph", action="store_true",
                     help="single: capture the step in a HIP graph (planned rendering) and replay it")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -21,8 +21,8 @@ from . import _lib
 from . import meshgen
 from ._lib import DtvError
 
-__all__ = ["vec3", "vec4", "uvec3", "AABB3", "ColorStorage", "TestCase", "RendererType", "TetMesh", "Renderer",
-           "BatchRenderer", "TetRegularizer", "set_device_type", "DtvError", "meshgen"]
+__all__ = ["vec3", "vec4", "uvec3", "AABB3", "ColorStorage", "TestCase", "RendererType", "SplitGradientType",
+           "TetMesh", "Renderer", "BatchRenderer", "TetRegularizer", "set_device_type", "DtvError", "meshgen"]
 
 
 class vec3:
@@ -78,6 +78,15 @@ class ColorStorage(enum.IntEnum):
 class TestCase(enum.IntEnum):
     SINGLE_TETRAHEDRON = 0
     CUBE_CENTRAL_GRADIENT = 1
+
+
+class SplitGradientType(enum.IntEnum):
+    """Which gradient ranks the vertices for coarse-to-fine splitting (OptimizerDefines.hpp:50-52).  The ABS_
+    variants read the same buffers; the caller accumulates them with ``use_abs_grad`` (train.py:369-404)."""
+    POSITION = 0
+    COLOR = 1
+    ABS_POSITION = 2
+    ABS_COLOR = 3
 
 
 class RendererType(enum.IntEnum):
@@ -257,6 +266,29 @@ class TetMesh:
         p = self._positions.detach()[self._cells.long()]
         o = torch.sum(torch.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], dim=1) * (p[:, 3] - p[:, 0]), dim=1)
         return bool((o >= 0).any())
+
+    def set_force_use_ovm_representation(self):
+        """Accepted for compatibility (Module.cpp:794): refinement works on the plain cell array."""
+
+    def split_by_largest_gradient_magnitudes(self, volume_renderer, split_gradient_type, splits_ratio):
+        """Coarse-to-fine step (Module.cpp:820-832, TetMesh.cpp:1786-1879): subdivides the ceil(splits_ratio*V)
+        vertices with the longest position (POSITION, ABS_POSITION) or colour (COLOR, ABS_COLOR) gradient
+        rows, then re-creates the mesh tensors (new tensors: optimisers must be given them, as train.py:413-420
+        does) and hands the mesh to ``volume_renderer``."""
+        from . import refine
+        self._require()
+        kind = SplitGradientType(split_gradient_type)
+        src = self._positions if kind in (SplitGradientType.POSITION, SplitGradientType.ABS_POSITION) else self._colors
+        if src.grad is None:
+            raise RuntimeError("split_by_largest_gradient_magnitudes: the mesh has no gradients "
+                               "(set_use_gradients(True) and accumulate them first).")
+        torch.cuda.synchronize()
+        cells, pos, col = refine.split_by_largest_gradient_magnitudes(
+            self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(), self._colors.detach().cpu().numpy(),
+            src.grad.detach().cpu().numpy(), float(splits_ratio))
+        self.set_tet_mesh_data(cells, pos, col)
+        if volume_renderer is not None:
+            volume_renderer.set_tet_mesh(self)
 
     def set_vertices_changed(self, vertices_changed=True):
         """Device tensors are the render inputs, so there is nothing to upload (TetMesh.cpp:788-818)."""

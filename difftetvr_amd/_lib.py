@@ -92,6 +92,13 @@ SIGNATURES = [
     ("dtv_adam_step", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p]),
     ("dtv_adam_step_device", ctypes.c_int, [ctypes.POINTER(DtvAdamGroup), c_int32, c_void_p, c_void_p]),
     ("dtv_tet_regularizer_grad", ctypes.c_int, [ctypes.POINTER(DtvMesh), c_float, c_float, c_void_p, c_void_p]),
+    ("dtv_split_by_largest_gradient_magnitudes", ctypes.c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                                                c_void_p, c_int32, c_float, ctypes.POINTER(c_void_p)]),
+    ("dtv_subdivide_vertices", ctypes.c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int64,
+                                              ctypes.POINTER(c_void_p)]),
+    ("dtv_split_result_sizes", ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    ("dtv_split_result_copy", ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("dtv_split_result_destroy", ctypes.c_int, [c_void_p]),
     ("dtv_mse_loss_grad", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     ("dtv_mse_loss_grad_batch", ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
     ("dtv_server_last_error", ctypes.c_char_p, []),

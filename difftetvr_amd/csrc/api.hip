@@ -21,6 +21,13 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+// Error reporting for the other host sources of the library (dtv_last_error).
+int dtv::report_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define DTV_HIP(expr)                                                                                   \
     do {                                                                                                \
         hipError_t e_ = (expr);                                                                         \

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "dtv_device.h"
 
 namespace dtv {
@@ -94,5 +96,8 @@ void launch_mse(const float* img, const float* tgt, int64_t n, int64_t denom, fl
 // regularizer.hip
 void launch_tet_regularizer(const int32_t* cells, const float* pos, int64_t T, float lambda, float beta,
                             float* grad_pos, hipStream_t s);
+
+// api.hip: sets the message dtv_last_error() returns and passes the status code through.
+int report_error(int code, const std::string& msg);
 
 }  // namespace dtv

@@ -67,12 +67,13 @@ void worker_main(dtv_server* S) {
     (void)hipSetDevice(S->device);
     dtv_view* view = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t done_ev = nullptr;
+    hipEvent_t done_ev = nullptr, fwd_ev = nullptr;
     float* scratch = nullptr;  // dimage of differentiated requests
     int64_t scratch_cap = 0;
     const int init_ok = dtv_view_create(S->ctx, &view) == DTV_OK &&
                         hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
-                        hipEventCreateWithFlags(&done_ev, hipEventDisableTiming) == hipSuccess;
+                        hipEventCreateWithFlags(&done_ev, hipEventDisableTiming) == hipSuccess &&
+                        hipEventCreateWithFlags(&fwd_ev, hipEventDisableTiming) == hipSuccess;
     std::vector<Request*> batch;
     std::vector<dtv_render_item> items;
     std::vector<dtv_adjoint_item> adj;
@@ -107,21 +108,25 @@ void worker_main(dtv_server* S) {
         std::string err = init_ok ? "" : "server worker initialisation failed";
         items.assign(batch.size(), dtv_render_item{});
         int64_t adj_pixels = 0;
-        bool any_adj = false;
+        size_t n_adj = 0;
         for (size_t i = 0; i < batch.size(); i++) {
             const dtv_server_request& q = batch[i]->req;
             items[i].mesh = S->meshes[(size_t)q.mesh_id];
             items[i].camera = q.camera;
             items[i].out_image = q.out_image;
             if (q.target_image) {
-                any_adj = true;
+                n_adj++;
                 adj_pixels += (int64_t)q.camera.width * q.camera.height;
             }
         }
+        const bool any_adj = n_adj > 0;
         if (rc == DTV_OK) {
             rc = dtv_render_batch(view, items.data(), (int32_t)items.size(), &S->params, stream);
             if (rc != DTV_OK) err = dtv_last_error();
         }
+        // A batch that also differentiates releases its forward-only requests when the forward has finished.
+        const bool early_release = rc == DTV_OK && any_adj && init_ok && n_adj < batch.size() &&
+                                   hipEventRecord(fwd_ev, stream) == hipSuccess;
         if (rc == DTV_OK && any_adj) {
             if (scratch_cap < 4 * adj_pixels) {
                 if (scratch) (void)hipFree(scratch);
@@ -148,8 +153,30 @@ void worker_main(dtv_server* S) {
             if (rc == DTV_OK) rc = dtv_render_adjoint_batch(view, adj.data(), (int32_t)adj.size(), 0, DTV_ADJOINT_EXACT, stream);
             if (rc != DTV_OK && err.empty()) err = dtv_last_error();
         }
+        if (init_ok) (void)hipEventRecord(done_ev, stream);
+        if (early_release) {
+            const bool fwd_ok = hipEventSynchronize(fwd_ev) == hipSuccess;
+            const Clock::time_point t_fwd = Clock::now();
+            {
+                // a released request belongs to its waiter (who frees it), so it leaves the batch here
+                std::lock_guard<std::mutex> lk(S->mu);
+                size_t keep = 0;
+                for (Request* r : batch) {
+                    if (r->req.target_image) {
+                        batch[keep++] = r;
+                        continue;
+                    }
+                    r->t_start = t_start;
+                    r->t_done = t_fwd;
+                    r->status = fwd_ok ? 0 : DTV_ERR_HIP;
+                    r->error = fwd_ok ? "" : "server: batch failed on the device";
+                    S->stats.completed++;
+                }
+                batch.resize(keep);
+            }
+            S->cv_done.notify_all();
+        }
         if (init_ok) {
-            (void)hipEventRecord(done_ev, stream);
             if (hipEventSynchronize(done_ev) != hipSuccess && rc == DTV_OK) {
                 rc = DTV_ERR_HIP;
                 err = "server: batch failed on the device";
@@ -172,6 +199,7 @@ void worker_main(dtv_server* S) {
     }
     if (scratch) (void)hipFree(scratch);
     if (done_ev) (void)hipEventDestroy(done_ev);
+    if (fwd_ev) (void)hipEventDestroy(fwd_ev);
     if (view) (void)dtv_view_destroy(view);
     if (stream) (void)hipStreamDestroy(stream);
 }

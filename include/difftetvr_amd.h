@@ -203,6 +203,28 @@ int dtv_adam_step_device(const dtv_adam_group* groups, int32_t num_groups, int64
  * Data/Shaders/Optimizer/TetRegularizer.glsl:108-166). */
 int dtv_tet_regularizer_grad(const dtv_mesh* mesh, float lambda, float beta, float* grad_positions, void* stream);
 
+/* ---- Coarse-to-fine refinement (host arrays; no device work) ----
+ * Subdivides, one after another, the ceil(splits_ratio * V) vertices whose gradient rows ([V,3] position or
+ * [V,4] colour gradients, gradient_components 3 or 4) have the largest length, ties to the larger index: each
+ * incident edge gets its midpoint (position and colour at t = 0.5), each incident tet becomes a corner tet and
+ * a prism split into three tets, the prism diagonals chosen by the smallest-vertex-index rule.  The result is
+ * read with dtv_split_result_sizes / dtv_split_result_copy and freed with dtv_split_result_destroy.
+ * Replaces TetMesh::splitByLargestGradientMagnitudes (src/Tet/TetMesh.cpp:1786-1879, bound at
+ * Module.cpp:820-832) and TetMesh::subdivideVertices (TetMesh.cpp:1757-1784). */
+typedef struct dtv_split_result dtv_split_result;
+int dtv_split_by_largest_gradient_magnitudes(const int32_t* cells, int64_t num_cells, const float* positions,
+                                             const float* colors, int64_t num_vertices, const float* gradients,
+                                             int32_t gradient_components, float splits_ratio,
+                                             dtv_split_result** out_result);
+/* The same with the magnitudes given and the number of splits explicit (TetMesh::subdivideVertices). */
+int dtv_subdivide_vertices(const int32_t* cells, int64_t num_cells, const float* positions, const float* colors,
+                           int64_t num_vertices, const float* gradient_magnitudes, int64_t num_splits,
+                           dtv_split_result** out_result);
+int dtv_split_result_sizes(const dtv_split_result* result, int64_t* num_cells, int64_t* num_vertices);
+/* cells [num_cells,4] int32, positions [num_vertices,3], colors [num_vertices,4], host memory. */
+int dtv_split_result_copy(const dtv_split_result* result, int32_t* cells, float* positions, float* colors);
+int dtv_split_result_destroy(dtv_split_result* result);
+
 /* MSE loss and its image adjoint: dimage = 2 (image - target) / n, *loss_out (device float, may be NULL)
  * = mean squared error.  Replaces torch.nn.MSELoss in train.py:325,346 and Loss.glsl:55-95. */
 int dtv_mse_loss_grad(const float* image, const float* target, int64_t n, float* dimage, float* loss_out,

@@ -93,14 +93,17 @@ def test_server_differentiated_requests(gpu):
         m.get_vertex_positions().grad.zero_()
         m.get_vertex_colors().grad.zero_()
     torch.cuda.synchronize()
-    # mixed with forward-only requests
+    # mixed with forward-only requests (released when the batch's forward is done, before its adjoint)
     tickets = []
     for (mi, vm, W, H), t in zip(reqs, targets):
-        tickets.append(srv.submit(ids[mi], vm, W, H, target=t))
-        tickets.append(srv.submit(ids[(mi + 1) % 3], vm, W, H))
-    for t in tickets:
-        srv.wait(t)
+        tickets.append((srv.submit(ids[mi], vm, W, H, target=t), None))
+        tickets.append((srv.submit(ids[(mi + 1) % 3], vm, W, H), ((mi + 1) % 3, vm, W, H)))
+    fwd_only = [(srv.wait(t)[0], spec) for t, spec in tickets]
     srv.close()
+    for img, spec in fwd_only:
+        if spec is not None:
+            mi, vm, W, H = spec
+            assert torch.equal(img, _single(d, meshes[mi], vm, W, H))
     for (gp, gc), m in zip(g1, meshes):
         assert float(gc.abs().sum()) > 0.0
         assert rel_l2(m.get_vertex_positions().grad.cpu().numpy(), gp.cpu().numpy()) <= 1e-5
