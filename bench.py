@@ -452,7 +452,7 @@ def main():
     ap.add_argument("--qps", type=float, default=150.0, help="serve: offered requests per second")
     ap.add_argument("--duration", type=float, default=8.0, help="serve: seconds of request arrivals")
     ap.add_argument("--views", type=int, default=16)
-    ap.add_argument("--workers", type=int, default=2, help="serve: worker threads (one stream and view each)")
+    ap.add_argument("--workers", type=int, default=4, help="serve: worker threads (one stream and view each)")
     ap.add_argument("--max-batch", type=int, default=16, help="serve: most requests per batch")
     ap.add_argument("--graph", action="store_true",
                     help="single: capture the step in a HIP graph (planned rendering) and replay it")
@@ -600,6 +600,8 @@ def main():
                                f"dir (0.3,0.4,1), attenuation 1, ERT on",
                    "tets": T, "vertices": V, "pixels": P, "tile_pairs": stats["num_pairs"],
                    "front_faces": stats["num_triangles"], "sort_passes": stats["sort_passes"],
+                   "segments": int(r.get_fragment_counts().to(torch.int64).sum()),
+                   "gradient_atomics": stats["num_gradient_atomics"],
                    "hip_graph": bool(args.graph),
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
         "roofline": {"bound": "hbm", "kernel": f"k_render_{dominant}", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -506,7 +506,7 @@ class Renderer:
         _lib.check(self._lib.dtv_view_get_stats(self._view, ctypes.byref(st)), "get_stats")
         return {"num_triangles": st.num_triangles, "num_pairs": st.num_pairs, "num_tiles": st.num_tiles,
                 "sort_passes": st.sort_passes, "width": st.width, "height": st.height, "num_items": st.num_items,
-                "num_pixels": st.num_pixels}
+                "num_pixels": st.num_pixels, "num_gradient_atomics": st.num_gradient_atomics}
 
 
 class BatchRenderer(Renderer):

@@ -38,7 +38,7 @@ class DtvRenderParams(ctypes.Structure):
 class DtvViewStats(ctypes.Structure):
     _fields_ = [("num_triangles", c_int64), ("num_pairs", c_int64), ("num_tiles", c_int64),
                 ("sort_passes", c_int32), ("width", c_int32), ("height", c_int32), ("num_items", c_int32),
-                ("num_pixels", c_int64)]
+                ("num_pixels", c_int64), ("num_gradient_atomics", c_int64)]
 
 
 class DtvRenderItem(ctypes.Structure):

@@ -153,7 +153,7 @@ struct dtv_view {
     hipEvent_t targets_copied = nullptr;   // the last upload out of pinnedTargets
     bool targets_pending = false;
     BinStats* pinnedStats = nullptr;
-    DevBuf dItems, dTargets, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2, rec_id,
+    DevBuf dItems, dTargets, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2,
         recs, hist,
         lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
@@ -183,8 +183,10 @@ struct dtv_view {
         ps.num_ckpt = num_ckpt.as<uint32_t>();
         ps.ckpt_color = ckpt_color.as<float4>();
         ps.ckpt_n = ckpt_n.as<uint32_t>();
+        ps.counters = nullptr;
         return ps;
     }
+    bool atomics_counted = false;  // the last adjoint ran with timing on and counted its atomics
 };
 
 namespace {
@@ -286,7 +288,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->counts.ensure(sizeof(uint32_t) * nslots));
     DTV_HIP(view->offsets.ensure(sizeof(uint32_t) * nslots));
     DTV_HIP(view->scan_status.ensure(sizeof(uint64_t) * (scan_partitions(nslots) + 1)));
-    DTV_HIP(view->small.ensure(256));
+    DTV_HIP(view->small.ensure(320));  // stats 0, scan counter 64, total 128, overflow 192, atomics 256
     BinStats* dstats = view->small.as<BinStats>();
     uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
     uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
@@ -328,12 +330,10 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->vals.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->keys2.ensure(sizeof(uint64_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
-    DTV_HIP(view->rec_id.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->recs.ensure(2 * sizeof(uint4) * (K > 0 ? K : 1)));
     view->mark(DTV_STAGE_EMIT, 0, s);
     launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
-                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), view->rec_id.as<uint32_t>(), K, overflow,
-                      s);
+                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), K, overflow, s);
     view->mark(DTV_STAGE_EMIT, 1, s);
     uint32_t mask = 0;
     // planned: every depth digit is sorted (which of them vary is not known on the host)
@@ -359,9 +359,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
     view->mark(DTV_STAGE_RANGES, 0, s);
     launch_tile_ranges(sorted_keys, K, n_dev, tiles, view->ranges.as<uint2>(), s);
-    const uint32_t* sorted_idx = view->result_in_alt ? view->vals2.as<uint32_t>() : view->vals.as<uint32_t>();
-    launch_gather_records(dI, n, sorted_keys, sorted_idx, view->rec_id.as<uint32_t>(), K, n_dev,
-                          view->recs.as<uint4>(), s);
+    const uint32_t* sorted_ids = view->result_in_alt ? view->vals2.as<uint32_t>() : view->vals.as<uint32_t>();
+    launch_gather_records(dI, n, sorted_keys, sorted_ids, K, n_dev, view->recs.as<uint4>(), s);
     view->mark(DTV_STAGE_RANGES, 1, s);
 
     // 6. render
@@ -420,10 +419,15 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         DTV_HIP(hipEventRecord(view->targets_copied, s));
         view->targets_pending = true;
     }
+    PixelState ps = view->pixel_state();
+    view->atomics_counted = view->timing;
+    if (view->timing) {  // count the gradient atomics along with the stage timers
+        ps.counters = reinterpret_cast<unsigned long long*>(view->small.as<char>() + 256);
+        fill_u32(reinterpret_cast<uint32_t*>(ps.counters), 0u, 2, s);
+    }
     view->mark(DTV_STAGE_ADJOINT, 0, s);
     launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
-                          view->sorted_recs(), view->ranges.as<uint2>(), view->pixel_state(), use_abs_grad ? 1 : 0,
-                          adjoint_mode, s);
+                          view->sorted_recs(), view->ranges.as<uint2>(), ps, use_abs_grad ? 1 : 0, adjoint_mode, s);
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;
@@ -480,7 +484,7 @@ int dtv_view_destroy(dtv_view* v) {
     (void)hipDeviceSynchronize();
     DevBuf* bufs[] = {&v->dItems, &v->dTargets, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
                       &v->vals, &v->keys2,
-                      &v->vals2, &v->rec_id, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
+                      &v->vals2, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
                       &v->num_ckpt, &v->ckpt_color, &v->ckpt_n};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < DTV_NUM_STAGES; i++)
@@ -614,6 +618,14 @@ int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out) {
     out->height = view->rendered ? view->items[0].U.height : 0;
     out->num_items = view->rendered ? view->nitems() : 0;
     out->num_pixels = view->total_pixels;
+    out->num_gradient_atomics = -1;
+    if (view->atomics_counted) {
+        DTV_HIP(hipSetDevice(view->ctx->device));
+        DTV_HIP(hipDeviceSynchronize());
+        unsigned long long c = 0;
+        DTV_HIP(hipMemcpy(&c, view->small.as<char>() + 256, sizeof(c), hipMemcpyDeviceToHost));
+        out->num_gradient_atomics = (int64_t)c;
+    }
     return DTV_OK;
 }
 

@@ -207,14 +207,14 @@ __global__ __launch_bounds__(256) void k_reduce_bin_stats(const uint4* __restric
 }
 
 // Same grid as k_count_pairs: writes the tet's records at offsets[slot], faces in order, tiles row-major.
-// keys/vals are the sort input (vals = record index); rec_id[i] = tet * 4 + face of record i.
-// Records at index >= cap (the capacity of keys/vals/rec_id) are not written; *overflow is then set to 1.
+// keys/vals are the sort input, vals = the record's face id tet * 4 + face (the stable sort keeps equal keys in
+// emit order, so the payload need not encode it).
+// Records at index >= cap (the capacity of keys/vals) are not written; *overflow is then set to 1.
 __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict__ items, int nitems,
                                                      const uint32_t* __restrict__ offsets,
                                                      const uint32_t* __restrict__ counts,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                     uint32_t* __restrict__ rec_id, uint64_t cap,
-                                                     uint32_t* __restrict__ overflow) {
+                                                     uint64_t cap, uint32_t* __restrict__ overflow) {
     const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
     const ViewUniforms& U = B.U;
     const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
@@ -238,8 +238,7 @@ __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict_
                 if (test && !tile_may_cover(b.S, tx, ty)) continue;
                 if (o < cap) {
                     keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
-                    vals[o] = o;
-                    rec_id[o] = id;
+                    vals[o] = id;
                 } else {
                     *overflow = 1u;
                 }
@@ -260,17 +259,16 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n_cap, 
 }
 
 // Sorted record i = out[2i], out[2i + 1] = {the tet's 4 vertex indices}, {face id, coverage mask low, high, 0}:
-// the face of record sorted_idx[i] is projected again (the same arithmetic as binning) and its exact coverage
-// of the record's tile computed.  Carrying the vertex indices saves the renderer one dependent load per record.
+// the face sorted_ids[i] is projected again (the same arithmetic as binning) and its exact coverage of the
+// record's tile computed.  Carrying the vertex indices saves the renderer one dependent load per record.
 __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restrict__ items, int nitems,
                                                          const uint64_t* __restrict__ sorted_keys,
-                                                         const uint32_t* __restrict__ sorted_idx,
-                                                         const uint32_t* __restrict__ rec_id, int64_t n_cap,
+                                                         const uint32_t* __restrict__ sorted_ids, int64_t n_cap,
                                                          const uint64_t* __restrict__ n_dev,
                                                          uint4* __restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= run_count(n_cap, n_dev)) return;
-    const uint32_t id = rec_id[sorted_idx[i]];
+    const uint32_t id = sorted_ids[i];
     const uint32_t gtile = (uint32_t)(sorted_keys[i] >> 32);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
@@ -302,11 +300,11 @@ void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const
 }
 
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
-                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, int64_t cap,
-                       uint32_t* overflow, hipStream_t s) {
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, int64_t cap, uint32_t* overflow,
+                       hipStream_t s) {
     if (nblocks <= 0) return;
     hipLaunchKernelGGL(k_emit_pairs, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, offsets, counts, keys,
-                       vals, rec_id, (uint64_t)cap, overflow);
+                       vals, (uint64_t)cap, overflow);
 }
 
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
@@ -319,11 +317,10 @@ void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* 
 }
 
 void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
-                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, const uint64_t* n_dev,
-                           uint4* out, hipStream_t s) {
+                           const uint32_t* sorted_ids, int64_t n, const uint64_t* n_dev, uint4* out, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items, nitems,
-                       sorted_keys, sorted_idx, rec_id, n, n_dev, out);
+                       sorted_keys, sorted_ids, n, n_dev, out);
 }
 
 }  // namespace dtv

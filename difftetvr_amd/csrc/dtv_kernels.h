@@ -37,19 +37,18 @@ void launch_count_pairs(const BatchItem* items, int nitems, int64_t nblocks, uin
 // stats.total_pairs = *scan_total; the other fields reduce the per-block partials of launch_count_pairs.
 void launch_reduce_bin_stats(const uint32_t* block_stats, int64_t nblocks, const uint64_t* scan_total,
                              BinStats* stats, hipStream_t s);
-// keys/vals: sort input (vals = record index); rec_id[i] = tet * 4 + face of record i.
+// keys/vals: sort input, vals = face id (tet * 4 + face) of the record.
 // Writes only records below cap (the buffers' capacity) and sets *overflow = 1 if any record lies beyond.
 void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* offsets,
-                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, uint32_t* rec_id, int64_t cap,
-                       uint32_t* overflow, hipStream_t s);
+                       const uint32_t* counts, uint64_t* keys, uint32_t* vals, int64_t cap, uint32_t* overflow,
+                       hipStream_t s);
 // n: capacity the launch is sized for; n_dev (may be NULL): actual count in device memory (see run_count).
 void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
                         uint2* ranges, hipStream_t s);
 // out[2i], out[2i + 1] = {tet vertex indices}, {face id, 8x8 coverage mask low, high, 0} of sorted record i: the
 // tile renderer's record stream.
 void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
-                           const uint32_t* sorted_idx, const uint32_t* rec_id, int64_t n, const uint64_t* n_dev,
-                           uint4* out, hipStream_t s);
+                           const uint32_t* sorted_ids, int64_t n, const uint64_t* n_dev, uint4* out, hipStream_t s);
 
 // render.hip
 // Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's
@@ -61,6 +60,7 @@ struct PixelState {
     uint32_t* num_ckpt;     // [P]
     float4* ckpt_color;     // [max_ckpt][P]
     uint32_t* ckpt_n;       // [max_ckpt][P]
+    unsigned long long* counters;  // optional (NULL): [0] += gradient float atomics issued by the adjoint
 };
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                            const uint2* ranges, PixelState ps, hipStream_t s);

@@ -207,6 +207,7 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
     __shared__ float s_w[kRoundAdj][29];  // per fragment: 4 vertices x (3 position + 4 colour), padded row
     __shared__ uint8_t s_hit[kRoundAdj];
     __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
+    __shared__ uint32_t s_natom;                         // atomics issued by this tile (when counted)
     const uint32_t gtile = global_tile_of_block();
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
@@ -236,6 +237,7 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
     const int P = U.width * U.height;
     const int pix = valid ? pixel_index(U, x, y) : 0;
 
+    if (ps.counters != nullptr && lane == 0) s_natom = 0u;
     float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
     float dC[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t n = 0, last = 0;
@@ -497,6 +499,7 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
             // The next batch's vertex data is requested before the atomics, so staging it does not wait for them.
             // Only lanes 0..31 hold that batch's record words.
             if (bi > 0 && lane < kBatch) fetch_record_data(pos, col, nxt, nd);
+            uint32_t issued = 0;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 if (!lead[k]) continue;
@@ -505,15 +508,21 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                 for (int c = 0; c < 7; c++) {
                     const float val = sum[k][c];
                     if (val == 0.f) continue;
+                    issued++;
                     if (c < 3)
                         atomic_add_f32(&grad_pos[3 * g + c], val);
                     else
                         atomic_add_f32(&grad_col[4 * g + (c - 3)], val);
                 }
             }
+            if (ps.counters != nullptr && issued) atomicAdd(&s_natom, issued);
         }
         if (ballot(!done) == 0ull) break;
         __syncthreads();
+    }
+    if (ps.counters != nullptr) {
+        __syncthreads();
+        if (lane == 0 && s_natom) atomicAdd(ps.counters, (unsigned long long)s_natom);
     }
 }
 

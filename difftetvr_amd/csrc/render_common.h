@@ -275,6 +275,7 @@ __device__ __forceinline__ PixelState item_pixel_state(const PixelState& all, co
     ps.num_ckpt = all.num_ckpt + B.pixel0;
     ps.ckpt_color = all.ckpt_color + (int64_t)B.U.max_ckpt * B.pixel0;
     ps.ckpt_n = all.ckpt_n + (int64_t)B.U.max_ckpt * B.pixel0;
+    ps.counters = all.counters;
     return ps;
 }
 

@@ -73,6 +73,7 @@ typedef struct dtv_view_stats {
     int32_t width, height;    /* viewport of the first item */
     int32_t num_items;        /* renders in the last batch (1 after dtv_render) */
     int64_t num_pixels;       /* pixels of all items */
+    int64_t num_gradient_atomics;  /* float atomics of the last adjoint if it ran with timing on, else -1 */
 } dtv_view_stats;
 
 /* One render of a batch: a mesh seen through a camera into out_image [height, width, 4]. */
