@@ -100,15 +100,36 @@ __device__ __forceinline__ uint64_t tile_coverage(const ScreenTri& S, int tx, in
         if (emax < 0 || (emax == 0 && !t.tie)) return 0ull;
         if (emin > 0 || (emin == 0 && t.tie)) continue;  // the edge admits every pixel of the tile
         uint64_t em = 0;
-        long long Er = t.C;
-#pragma unroll 1
-        for (int ly = 0; ly < 8; ly++) {
-            long long E = Er;
-            for (int lx = 0; lx < 8; lx++) {
-                if (E > 0 || (E == 0 && t.tie)) em |= 1ull << (8 * ly + lx);
-                E += t.sx;
+        // sx = -256 dy and sy = 256 dx, so E = 256 W + r with W = q + ly dx - lx dy, q = floor(C / 256),
+        // r = C mod 256: E > 0 or (E == 0 on a tie edge)  <=>  W >= wmin.  The edge crosses the tile, so
+        // emin <= 0 <= emax bounds |E| by 7 (|sx| + |sy|) and |W| by 7 (|dx| + |dy|) + 1 < 2^31 for screen
+        // coordinates below 2^26 (the guard band keeps them below 2^25.2): exact in 32-bit integers.
+        const long long dx = t.sy >> 8, dy = -(t.sx >> 8);
+        if (dx > -(1ll << 27) && dx < (1ll << 27) && dy > -(1ll << 27) && dy < (1ll << 27)) {
+            const int dxi = (int)dx, dyi = (int)dy;
+            const int r = (int)(t.C & 255);
+            const int wmin = (r > 0 || t.tie) ? 0 : 1;
+            const int q = (int)(t.C >> 8);
+            // only values at the tile's 64 pixel centres are formed (no step past the tile can overflow)
+#pragma unroll
+            for (int ly = 0; ly < 8; ly++) {
+                const int Wr = q + ly * dxi;
+                uint32_t rm = 0;
+#pragma unroll
+                for (int lx = 0; lx < 8; lx++) rm |= (uint32_t)(Wr - lx * dyi >= wmin) << lx;
+                em |= (uint64_t)rm << (8 * ly);
             }
-            Er += t.sy;
+        } else {
+            long long Er = t.C;
+#pragma unroll 1
+            for (int ly = 0; ly < 8; ly++) {
+                long long E = Er;
+                for (int lx = 0; lx < 8; lx++) {
+                    if (E > 0 || (E == 0 && t.tie)) em |= 1ull << (8 * ly + lx);
+                    E += t.sx;
+                }
+                Er += t.sy;
+            }
         }
         m &= em;
         if (m == 0) return 0ull;

@@ -109,11 +109,23 @@ __global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* __restr
     for (int i = threadIdx.x; i < 4 * 8 * 256; i += 256) (&s_h[0][0][0])[i] = 0;
     __syncthreads();
     const int w = threadIdx.x / 64;
+    const int lane = lane_id();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t k = keys[i];
+        const uint64_t act = ballot(true);
 #pragma unroll
-        for (int p = 0; p < 8; p++)
-            if (digit_mask & (1u << p)) atomicAdd(&s_h[w][p][(k >> (8 * p)) & 255], 1u);
+        for (int p = 0; p < 8; p++) {
+            if (!(digit_mask & (1u << p))) continue;
+            // digits shared by the whole wave (the depth exponent, the high tile bits) are added once: 64 LDS
+            // atomics on one address would serialise
+            const uint32_t d = (uint32_t)(k >> (8 * p)) & 255u;
+            const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+            if (ballot(d == d0) == act) {
+                if (lane == __ffsll((unsigned long long)act) - 1) atomicAdd(&s_h[w][p][d0], (uint32_t)popc64(act));
+            } else {
+                atomicAdd(&s_h[w][p][d], 1u);
+            }
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 8 * 256; i += 256) {
