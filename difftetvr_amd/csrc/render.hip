@@ -28,9 +28,11 @@ constexpr int kRoundAdj = 64;               // fragments per round in the adjoin
 constexpr int kMaxFrags = kBatch * 64;      // fragments a batch can produce
 constexpr float kMissMarker = -1.0f;        // alpha of a fragment whose ray missed the tet
 
+// kVals = fragments per round.  The adjoint's 64 keep its LDS below 20 KB: eight waves fit in a CU's 160 KB.
+template <int kVals>
 struct FragListLds {
     uint16_t entry[kMaxFrags];  // (pixel lane << 5) | record index
-    float4 val[kRound];         // forward: fragment colour F; adjoint: dL/dF
+    float4 val[kVals];          // forward: fragment colour F; adjoint: dL/dF
     float rd[3][64];            // ray direction of each pixel
 };
 
@@ -60,7 +62,8 @@ __device__ __forceinline__ uint32_t key_rank_start(const uint64_t kb[5], uint64_
 
 // Coverage masks -> dense fragment list.  Each lane's fragments are contiguous, ordered by record index
 // (ascending, or descending when `reverse`).  Returns the lane's first entry; `total` = all fragments.
-__device__ __forceinline__ uint32_t build_fragment_list(FragListLds& F, uint32_t mask, bool reverse, int lane,
+template <int kVals>
+__device__ __forceinline__ uint32_t build_fragment_list(FragListLds<kVals>& F, uint32_t mask, bool reverse, int lane,
                                                         uint32_t& total) {
     const uint32_t nm = (uint32_t)__popc(mask);
     const uint32_t inc = wave_inclusive_scan(nm);
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ RecordLds L;
-    __shared__ FragListLds F;
+    __shared__ FragListLds<kRound> F;
     const uint32_t gtile = global_tile_of_block();
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                                                         const uint2* __restrict__ ranges, PixelState ps_all,
                                                         int use_abs) {
     __shared__ RecordLds L;
-    __shared__ FragListLds F;
+    __shared__ FragListLds<kRoundAdj> F;
     __shared__ float s_w[kRoundAdj][29];  // per fragment: 4 vertices x (3 position + 4 colour), padded row
     __shared__ uint8_t s_hit[kRoundAdj];
     __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
