@@ -163,7 +163,6 @@ struct dtv_view {
     int64_t total_blocks = 0, total_tiles = 0, total_pixels = 0;
     int64_t num_pairs = 0, num_tris = 0;
     int sort_passes = 0;
-    bool result_in_alt = false;
     // per-stage timers
     bool timing = false;
     hipEvent_t ev[DTV_NUM_STAGES][2] = {};
@@ -288,7 +287,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->counts.ensure(sizeof(uint32_t) * nslots));
     DTV_HIP(view->offsets.ensure(sizeof(uint32_t) * nslots));
     DTV_HIP(view->scan_status.ensure(sizeof(uint64_t) * (scan_partitions(nslots) + 1)));
-    DTV_HIP(view->small.ensure(320));  // stats 0, scan counter 64, total 128, overflow 192, atomics 256
+    // stats 0, scan counter 64, total 128, overflow 192, atomics 256, sort route 320 (48 B)
+    DTV_HIP(view->small.ensure(384));
     BinStats* dstats = view->small.as<BinStats>();
     uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
     uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
@@ -336,7 +336,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                       view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), K, overflow, s);
     view->mark(DTV_STAGE_EMIT, 1, s);
     uint32_t mask = 0;
-    // planned: every depth digit is sorted (which of them vary is not known on the host)
+    // planned: a pass per depth digit (which of them vary is not known on the host; constant ones exit at once)
     const uint32_t varying = planned ? 0xFFFFFFFFu : (st.key_and ^ st.key_or);
     for (int p = 0; p < 4; p++)
         if ((varying >> (8 * p)) & 255u) mask |= 1u << p;
@@ -348,19 +348,18 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->hist.ensure(sizeof(uint32_t) * 8 * 256 + 64));
     DTV_HIP(view->lookback.ensure(sizeof(uint32_t) * 256 * (size_t)(npass > 0 ? npass : 1) *
                                   (size_t)(sort_partitions(K) + 1)));
+    // planned: a depth digit all keys share costs one launch that exits at once (the sort routes on the device)
+    SortRoute* route = reinterpret_cast<SortRoute*>(view->small.as<char>() + 320);
     view->mark(DTV_STAGE_SORT, 0, s);
     view->sort_passes = radix_sort_pairs(view->keys.as<uint64_t>(), view->vals.as<uint32_t>(),
                                          view->keys2.as<uint64_t>(), view->vals2.as<uint32_t>(), K, n_dev, mask,
                                          view->hist.as<uint32_t>(), view->lookback.as<uint32_t>(),
-                                         view->hist.as<uint32_t>() + 8 * 256, s);
+                                         view->hist.as<uint32_t>() + 8 * 256, route, s);
     view->mark(DTV_STAGE_SORT, 1, s);
-    view->result_in_alt = (view->sort_passes & 1) != 0;
-    const uint64_t* sorted_keys = view->result_in_alt ? view->keys2.as<uint64_t>() : view->keys.as<uint64_t>();
     DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
     view->mark(DTV_STAGE_RANGES, 0, s);
-    launch_tile_ranges(sorted_keys, K, n_dev, tiles, view->ranges.as<uint2>(), s);
-    const uint32_t* sorted_ids = view->result_in_alt ? view->vals2.as<uint32_t>() : view->vals.as<uint32_t>();
-    launch_gather_records(dI, n, sorted_keys, sorted_ids, K, n_dev, view->recs.as<uint4>(), s);
+    launch_tile_ranges(route, K, n_dev, tiles, view->ranges.as<uint2>(), s);
+    launch_gather_records(dI, n, route, K, n_dev, view->recs.as<uint4>(), s);
     view->mark(DTV_STAGE_RANGES, 1, s);
 
     // 6. render

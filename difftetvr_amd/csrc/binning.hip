@@ -5,8 +5,8 @@
 // face becomes one (tile, face) record per 8x8 screen tile of its pixel-centre bounding box that the face's edge
 // functions do not exclude (tile_may_cover), keyed by (tile << 32 | depth-key bits).  Counting and emission are
 // two passes over the tets separated by a device-wide scan, so records are written in (tet, face, tile) order
-// and the stable sort keeps that order for equal depth keys.  The sort moves record indices; k_gather_records
-// then lays the sorted records out for the tile renderer as {face id, exact 64-bit coverage mask of the tile}
+// and the stable sort keeps that order for equal depth keys.  The sort moves face ids; k_gather_records then
+// lays the sorted records out for the tile renderer as {face id, exact 64-bit coverage mask of the tile}
 // (one thread per record, so the per-pixel edge tests are spread evenly over the GPU).
 #include "dtv_kernels.h"
 #include "dtv_prims.h"
@@ -269,8 +269,9 @@ __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict_
     }
 }
 
-__global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n_cap, const uint64_t* __restrict__ n_dev,
+__global__ void k_tile_ranges(const SortRoute* __restrict__ route, int64_t n_cap, const uint64_t* __restrict__ n_dev,
                               uint2* __restrict__ ranges) {
+    const uint64_t* __restrict__ keys = route->keys_out;
     const int64_t n = run_count(n_cap, n_dev);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t t = (uint32_t)(keys[i] >> 32);
@@ -283,12 +284,13 @@ __global__ void k_tile_ranges(const uint64_t* __restrict__ keys, int64_t n_cap, 
 // the face sorted_ids[i] is projected again (the same arithmetic as binning) and its exact coverage of the
 // record's tile computed.  Carrying the vertex indices saves the renderer one dependent load per record.
 __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restrict__ items, int nitems,
-                                                         const uint64_t* __restrict__ sorted_keys,
-                                                         const uint32_t* __restrict__ sorted_ids, int64_t n_cap,
+                                                         const SortRoute* __restrict__ route, int64_t n_cap,
                                                          const uint64_t* __restrict__ n_dev,
                                                          uint4* __restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= run_count(n_cap, n_dev)) return;
+    const uint64_t* __restrict__ sorted_keys = route->keys_out;
+    const uint32_t* __restrict__ sorted_ids = route->vals_out;
     const uint32_t id = sorted_ids[i];
     const uint32_t gtile = (uint32_t)(sorted_keys[i] >> 32);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
@@ -328,20 +330,20 @@ void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, cons
                        vals, (uint64_t)cap, overflow);
 }
 
-void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
+void launch_tile_ranges(const SortRoute* route, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
                         uint2* ranges, hipStream_t s) {
     fill_u32(reinterpret_cast<uint32_t*>(ranges), 0u, 2 * num_tiles, s);
     if (n <= 0) return;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, sorted_keys, n, n_dev, ranges);
+    hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, route, n, n_dev, ranges);
 }
 
-void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
-                           const uint32_t* sorted_ids, int64_t n, const uint64_t* n_dev, uint4* out, hipStream_t s) {
+void launch_gather_records(const BatchItem* items, int nitems, const SortRoute* route, int64_t n,
+                           const uint64_t* n_dev, uint4* out, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items, nitems,
-                       sorted_keys, sorted_ids, n, n_dev, out);
+    hipLaunchKernelGGL(k_gather_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items, nitems, route,
+                       n, n_dev, out);
 }
 
 }  // namespace dtv

@@ -15,11 +15,19 @@ int64_t scan_partitions(int64_t n);
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* status, uint32_t* counter,
                         uint64_t* total, hipStream_t s);
 int64_t sort_partitions(int64_t n);
+// Where the sorted pairs end up, decided on the device: a selected digit that every key shares is skipped
+// (its pass exits at once), so the ping-pong parity is known only there.  pass_src[i]: 0 = pass i reads the
+// first buffers, 1 = the alternates, 2 = skipped.
+struct SortRoute {
+    uint32_t pass_src[8];
+    const uint64_t* keys_out;
+    const uint32_t* vals_out;
+};
 // n_dev (may be NULL): the element count in device memory, clamped to n (the capacity the launches are sized
-// for); NULL sorts exactly n.
+// for); NULL sorts exactly n.  route (device) receives the result buffers; returns the passes launched.
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
                      const uint64_t* n_dev, uint32_t digit_mask, uint32_t* hist, uint32_t* lookback,
-                     uint32_t* counters, hipStream_t s);
+                     uint32_t* counters, SortRoute* route, hipStream_t s);
 
 // binning.hip
 struct BinStats {
@@ -43,12 +51,13 @@ void launch_emit_pairs(const BatchItem* items, int nitems, int64_t nblocks, cons
                        const uint32_t* counts, uint64_t* keys, uint32_t* vals, int64_t cap, uint32_t* overflow,
                        hipStream_t s);
 // n: capacity the launch is sized for; n_dev (may be NULL): actual count in device memory (see run_count).
-void launch_tile_ranges(const uint64_t* sorted_keys, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
+// The sorted keys and face ids are route->keys_out / vals_out (device pointers, set by the sort).
+void launch_tile_ranges(const SortRoute* route, int64_t n, const uint64_t* n_dev, int64_t num_tiles,
                         uint2* ranges, hipStream_t s);
 // out[2i], out[2i + 1] = {tet vertex indices}, {face id, 8x8 coverage mask low, high, 0} of sorted record i: the
 // tile renderer's record stream.
-void launch_gather_records(const BatchItem* items, int nitems, const uint64_t* sorted_keys,
-                           const uint32_t* sorted_ids, int64_t n, const uint64_t* n_dev, uint4* out, hipStream_t s);
+void launch_gather_records(const BatchItem* items, int nitems, const SortRoute* route, int64_t n,
+                           const uint64_t* n_dev, uint4* out, hipStream_t s);
 
 // render.hip
 // Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's

@@ -136,24 +136,52 @@ __global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* __restr
     }
 }
 
-// Exclusive scan of each selected histogram in place: offsets[p][d] = sum_{d' < d} hist[p][d'].
-__global__ __launch_bounds__(256) void k_digit_offsets(uint32_t* __restrict__ hist, uint32_t digit_mask) {
+// Exclusive scan of each selected histogram in place: offsets[p][d] = sum_{d' < d} hist[p][d'].  Also routes
+// the passes: a digit whose histogram has a single non-empty bucket is skipped, and the result buffers are
+// those the last pass that runs writes.
+__global__ __launch_bounds__(256) void k_digit_offsets(uint32_t* __restrict__ hist, uint32_t digit_mask,
+                                                       int64_t n_cap, const uint64_t* __restrict__ n_dev,
+                                                       SortRoute* __restrict__ route, const uint64_t* keys,
+                                                       const uint32_t* vals, const uint64_t* keys_alt,
+                                                       const uint32_t* vals_alt) {
     __shared__ uint32_t s_tmp[256 / 64 + 1];
+    const uint32_t n = (uint32_t)run_count(n_cap, n_dev);
+    uint32_t parity = 0;
+    int pass = 0;
     for (int p = 0; p < 8; p++) {
         if (!(digit_mask & (1u << p))) continue;
         uint32_t v = hist[p * 256 + threadIdx.x];
+        const bool trivial = __syncthreads_or(v == n) != 0;
         uint32_t tot;
         uint32_t e = block_exclusive_scan<256>(v, s_tmp, tot);
         hist[p * 256 + threadIdx.x] = e;
+        if (threadIdx.x == 0) route->pass_src[pass] = trivial ? 2u : parity;
+        if (!trivial) parity ^= 1u;
+        pass++;
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        route->keys_out = parity ? keys_alt : keys;
+        route->vals_out = parity ? vals_alt : vals;
     }
 }
 
+__global__ void k_route_identity(SortRoute* __restrict__ route, const uint64_t* keys, const uint32_t* vals) {
+    route->keys_out = keys;
+    route->vals_out = vals;
+}
+
 __global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
-        const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-        uint32_t* __restrict__ vout, int64_t n_cap, const uint64_t* __restrict__ n_dev, int shift,
+        uint64_t* __restrict__ keys_a, uint32_t* __restrict__ vals_a, uint64_t* __restrict__ keys_b,
+        uint32_t* __restrict__ vals_b, int64_t n_cap, const uint64_t* __restrict__ n_dev, int shift,
         const uint32_t* __restrict__ digit_offsets, uint32_t* __restrict__ lookback,
-        uint32_t* __restrict__ part_counter) {
+        uint32_t* __restrict__ part_counter, const SortRoute* __restrict__ route, int pass) {
+    const uint32_t src = route->pass_src[pass];
+    if (src == 2u) return;  // every key has the same digit here: nothing moves
+    const uint64_t* __restrict__ kin = src ? keys_b : keys_a;
+    const uint32_t* __restrict__ vin = src ? vals_b : vals_a;
+    uint64_t* __restrict__ kout = src ? keys_a : keys_b;
+    uint32_t* __restrict__ vout = src ? vals_a : vals_b;
     const int64_t n = run_count(n_cap, n_dev);
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wcnt[kSortWaves][256];
@@ -260,11 +288,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
 
 // Sorts (keys, vals) of length n by the digits in digit_mask (ascending digit positions), ping-ponging with
 // (keys_alt, vals_alt).  hist must hold 8*256 u32; lookback (#passes * partitions * 256) u32; counters 8 u32.
-// Returns the number of passes; the result is in the alternate buffers when that number is odd.
+// Returns the number of passes launched; route->keys_out / vals_out (device) name the sorted buffers.
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
                      const uint64_t* n_dev, uint32_t digit_mask, uint32_t* hist, uint32_t* lookback,
-                     uint32_t* counters, hipStream_t s) {
-    if (n <= 1 || digit_mask == 0) return 0;
+                     uint32_t* counters, SortRoute* route, hipStream_t s) {
+    if (n <= 1 || digit_mask == 0) {
+        hipLaunchKernelGGL(k_route_identity, dim3(1), dim3(1), 0, s, route, keys, vals);
+        return 0;
+    }
     const int64_t parts = sort_partitions(n);
     int npass = 0;
     for (int p = 0; p < 8; p++) npass += (digit_mask >> p) & 1;
@@ -275,18 +306,14 @@ int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_
     if (hblocks > 1024) hblocks = 1024;
     hipLaunchKernelGGL(k_digit_histogram, dim3((unsigned)hblocks), dim3(256), 0, s, keys, n, n_dev, digit_mask,
                        hist);
-    hipLaunchKernelGGL(k_digit_offsets, dim3(1), dim3(256), 0, s, hist, digit_mask);
-    uint64_t* ka = keys;
-    uint32_t* va = vals;
-    uint64_t* kb = keys_alt;
-    uint32_t* vb = vals_alt;
+    hipLaunchKernelGGL(k_digit_offsets, dim3(1), dim3(256), 0, s, hist, digit_mask, n, n_dev, route, keys, vals,
+                       keys_alt, vals_alt);
     int pass = 0;
     for (int p = 0; p < 8; p++) {
         if (!((digit_mask >> p) & 1)) continue;
-        hipLaunchKernelGGL(k_onesweep_pass, dim3((unsigned)parts), dim3(kSortBlock), 0, s, ka, va, kb, vb, n, n_dev,
-                           8 * p, hist + p * 256, lookback + (int64_t)pass * parts * 256, counters + pass);
-        uint64_t* tk = ka; ka = kb; kb = tk;
-        uint32_t* tv = va; va = vb; vb = tv;
+        hipLaunchKernelGGL(k_onesweep_pass, dim3((unsigned)parts), dim3(kSortBlock), 0, s, keys, vals, keys_alt,
+                           vals_alt, n, n_dev, 8 * p, hist + p * 256, lookback + (int64_t)pass * parts * 256,
+                           counters + pass, route, pass);
         pass++;
     }
     return npass;
