@@ -435,8 +435,20 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
             }
             __syncthreads();
             {
+                // two rows per step: 28 independent LDS reads in flight instead of 14
                 const int half = lane >> 5;
-                for (uint32_t rr = 0; rr < segn; rr++) {
+                uint32_t rr = 0;
+                for (; rr + 1 < segn; rr += 2) {
+                    float a[14], b[14];
+#pragma unroll
+                    for (int q = 0; q < 14; q++) {
+                        a[q] = s_w[seg0 + rr][half * 14 + q];
+                        b[q] = s_w[seg0 + rr + 1][half * 14 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 14; q++) acc[q] += a[q] + b[q];
+                }
+                if (rr < segn) {
 #pragma unroll
                     for (int q = 0; q < 14; q++) acc[q] += s_w[seg0 + rr][half * 14 + q];
                 }
