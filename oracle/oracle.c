@@ -25,8 +25,10 @@
  *                        chord-length invariant on the cube grid
  *   exact adjoint        FP64 central differences of the FP64 forward (h=1e-4, masked to pixels whose fragment
  *                        lists do not change)
- *   reference adjoint    parity unpinned (the reference's own derivative is not the forward's derivative;
- *                        only its colour part is checked against the exact adjoint with the s=0 term removed)
+ *   reference adjoint    on rays through one tet: FP64 central differences of the forward with sub-segment 0
+ *                        frozen (colours, density) and of the two frozen-geometry forwards whose derivatives it
+ *                        adds (positions) (tests/test_reference_adjoint_pinning.py); the chaining of state
+ *                        between fragments (sub-segment 0 never inverted) is parity unpinned
  *   Adam                 golden vectors written by torch.optim.Adam (tests/golden/adam_golden.npz)
  */
 #include <math.h>
