@@ -347,8 +347,9 @@ def bench_serve(args):
         meshes.append(m)
         tets.append(int(cells.shape[0]))
     t_setup = time.perf_counter() - t0
+    # early ray termination on, the reference renderer's default (TetMeshVolumeRenderer.hpp:243-244)
     srv = Server(num_workers=args.workers, max_batch=args.max_batch, memory_budget=16 << 30, attenuation=1.0,
-                 fovy=math.pi / 4.0)
+                 fovy=math.pi / 4.0, use_early_ray_termination=True)
     ids = [srv.add_mesh(m) for m in meshes]
     sizes = [256, 512, 1024]
     nreq = int(args.qps * args.duration)
@@ -431,7 +432,7 @@ def bench_serve(args):
         "config": {"workload": "Poisson render requests over 16 grid meshes (16^3..80^3 x6) and 16 Delaunay meshes "
                                f"(10k..500k points), viewports {{256,512,1024}}^2, 10% with loss + adjoint, "
                                f"{args.workers} workers, up to {args.max_batch} requests per batch, "
-                               "16 GiB in-flight budget",
+                               "16 GiB in-flight budget, early ray termination on",
                    "requests": nreq, "meshes": len(meshes), "tets_total": int(sum(tets)),
                    "mesh_setup_s": round(t_setup, 1), "batches": st["batches"],
                    "max_batch_seen": st["max_batch_seen"]},

@@ -3,8 +3,9 @@
 // every worker owns a view and a stream, so while one batch runs on the GPU the next worker admits the
 // requests that arrived meanwhile and launches them; admission is bounded by a device-memory budget for the
 // batches in flight.  Requests with a target image are also differentiated (MSE loss, then the batched
-// adjoint accumulating into the request's gradient buffers).  No counterpart in the reference, which renders
-// one view per call from Python.
+// adjoint accumulating into the request's gradient buffers); they are batched apart from renders, one such
+// batch at a time, so that renders never wait behind an adjoint.  No counterpart in the reference, which
+// renders one view per call from Python.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,7 +47,8 @@ struct dtv_server {
     std::vector<dtv_mesh> meshes;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<Request*> pending;
+    std::deque<Request*> pending_render, pending_train;  // forward-only requests; requests with a target
+    bool train_in_flight = false;                        // one differentiated batch on the GPU at a time
     std::unordered_map<uint64_t, Request*> requests;
     uint64_t next_ticket = 1;
     int64_t in_flight_mem = 0;
@@ -67,38 +69,56 @@ void worker_main(dtv_server* S) {
     (void)hipSetDevice(S->device);
     dtv_view* view = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t done_ev = nullptr, fwd_ev = nullptr;
+    hipEvent_t done_ev = nullptr;
     float* scratch = nullptr;  // dimage of differentiated requests
     int64_t scratch_cap = 0;
     const int init_ok = dtv_view_create(S->ctx, &view) == DTV_OK &&
                         hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
-                        hipEventCreateWithFlags(&done_ev, hipEventDisableTiming) == hipSuccess &&
-                        hipEventCreateWithFlags(&fwd_ev, hipEventDisableTiming) == hipSuccess;
+                        hipEventCreateWithFlags(&done_ev, hipEventDisableTiming) == hipSuccess;
     std::vector<Request*> batch;
     std::vector<dtv_render_item> items;
     std::vector<dtv_adjoint_item> adj;
-    for (;;) {
+    bool quit = false;
+    while (!quit) {
         batch.clear();
         int64_t mem = 0;
+        bool train = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
-            if (S->stop && S->pending.empty()) break;
+            // Renders and differentiated requests form separate batches.  At most one differentiated batch runs
+            // at a time (its adjoint is several times a render's work), so the other workers keep rendering
+            // beside it; between the two classes the older request goes first.
+            for (;;) {
+                const bool t_ok = !S->pending_train.empty() && !S->train_in_flight;
+                const bool r_ok = !S->pending_render.empty();
+                if (t_ok || r_ok) {
+                    train = t_ok && (!r_ok || S->pending_train.front()->ticket < S->pending_render.front()->ticket);
+                    break;
+                }
+                if (S->stop && S->pending_render.empty() && S->pending_train.empty()) {
+                    quit = true;
+                    break;
+                }
+                S->cv_work.wait(lk);
+            }
+            if (quit) break;
+            std::deque<Request*>& queue = train ? S->pending_train : S->pending_render;
             // admit in arrival order while the batch has room and the budget allows (one request always fits)
-            while (!S->pending.empty() && (int32_t)batch.size() < S->max_batch) {
-                Request* r = S->pending.front();
+            while (!queue.empty() && (int32_t)batch.size() < S->max_batch) {
+                Request* r = queue.front();
                 if (!batch.empty() || S->in_flight_mem > 0) {
                     if (S->budget > 0 && S->in_flight_mem + mem + r->mem > S->budget) break;
                 }
-                S->pending.pop_front();
+                queue.pop_front();
                 batch.push_back(r);
                 mem += r->mem;
             }
             if (batch.empty()) {
                 // budget exhausted by batches in flight: wait for one to finish
-                S->cv_work.wait(lk, [&] { return S->stop || S->in_flight_mem == 0 || S->pending.empty(); });
+                S->cv_work.wait(lk, [&] { return S->stop || S->in_flight_mem == 0 || queue.empty(); });
                 continue;
             }
+            if (train) S->train_in_flight = true;
             S->in_flight_mem += mem;
             S->stats.batches++;
             S->stats.max_batch_seen = std::max<int64_t>(S->stats.max_batch_seen, (int64_t)batch.size());
@@ -124,9 +144,6 @@ void worker_main(dtv_server* S) {
             rc = dtv_render_batch(view, items.data(), (int32_t)items.size(), &S->params, stream);
             if (rc != DTV_OK) err = dtv_last_error();
         }
-        // A batch that also differentiates releases its forward-only requests when the forward has finished.
-        const bool early_release = rc == DTV_OK && any_adj && init_ok && n_adj < batch.size() &&
-                                   hipEventRecord(fwd_ev, stream) == hipSuccess;
         if (rc == DTV_OK && any_adj) {
             if (scratch_cap < 4 * adj_pixels) {
                 if (scratch) (void)hipFree(scratch);
@@ -153,30 +170,8 @@ void worker_main(dtv_server* S) {
             if (rc == DTV_OK) rc = dtv_render_adjoint_batch(view, adj.data(), (int32_t)adj.size(), 0, DTV_ADJOINT_EXACT, stream);
             if (rc != DTV_OK && err.empty()) err = dtv_last_error();
         }
-        if (init_ok) (void)hipEventRecord(done_ev, stream);
-        if (early_release) {
-            const bool fwd_ok = hipEventSynchronize(fwd_ev) == hipSuccess;
-            const Clock::time_point t_fwd = Clock::now();
-            {
-                // a released request belongs to its waiter (who frees it), so it leaves the batch here
-                std::lock_guard<std::mutex> lk(S->mu);
-                size_t keep = 0;
-                for (Request* r : batch) {
-                    if (r->req.target_image) {
-                        batch[keep++] = r;
-                        continue;
-                    }
-                    r->t_start = t_start;
-                    r->t_done = t_fwd;
-                    r->status = fwd_ok ? 0 : DTV_ERR_HIP;
-                    r->error = fwd_ok ? "" : "server: batch failed on the device";
-                    S->stats.completed++;
-                }
-                batch.resize(keep);
-            }
-            S->cv_done.notify_all();
-        }
         if (init_ok) {
+            (void)hipEventRecord(done_ev, stream);
             if (hipEventSynchronize(done_ev) != hipSuccess && rc == DTV_OK) {
                 rc = DTV_ERR_HIP;
                 err = "server: batch failed on the device";
@@ -193,13 +188,13 @@ void worker_main(dtv_server* S) {
                 S->stats.completed++;
             }
             S->in_flight_mem -= mem;
+            if (train) S->train_in_flight = false;
         }
         S->cv_done.notify_all();
         S->cv_work.notify_all();
     }
     if (scratch) (void)hipFree(scratch);
     if (done_ev) (void)hipEventDestroy(done_ev);
-    if (fwd_ev) (void)hipEventDestroy(fwd_ev);
     if (view) (void)dtv_view_destroy(view);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -274,7 +269,7 @@ int dtv_server_submit(dtv_server* S, const dtv_server_request* req, uint64_t* ou
         r->mem = request_memory(S->meshes[(size_t)req->mesh_id], req->camera);
         r->t_submit = Clock::now();
         S->requests[r->ticket] = r;
-        S->pending.push_back(r);
+        (req->target_image ? S->pending_train : S->pending_render).push_back(r);
         S->stats.submitted++;
         *out_ticket = r->ticket;
     }
