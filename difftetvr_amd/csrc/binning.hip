@@ -297,12 +297,12 @@ __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restr
     const ViewUniforms& U = B.U;
     const int tile = (int)(gtile - B.tile0);
     const int64_t tet = id >> 2;
+    const int4 vid = *reinterpret_cast<const int4*>(B.cells + 4 * tet);
     ProjectedTet PT;
-    project_tet(U, B.cells, B.pos, tet, PT);
+    project_face(U, vid, B.pos, (int)(id & 3), PT);  // the tet passed culling when its records were emitted
     ScreenTri S;
     face_screen(U, PT, id & 3, S);
     const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
-    const int4 vid = *reinterpret_cast<const int4*>(B.cells + 4 * tet);
     out[2 * i] = make_uint4((uint32_t)vid.x, (uint32_t)vid.y, (uint32_t)vid.z, (uint32_t)vid.w);
     out[2 * i + 1] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
 }

@@ -25,6 +25,9 @@ __device__ __forceinline__ int face_vertex(int f, int k) { return (kFacePacked >
 __device__ __forceinline__ float sel4(int i, float a0, float a1, float a2, float a3) {
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
+__device__ __forceinline__ int32_t sel4i(int i, int32_t a0, int32_t a1, int32_t a2, int32_t a3) {
+    return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
 
 // Per-view uniforms (IntersectUniform.glsl:29-41 plus the raster/tile setup).  Matrices column-major m[c*4+r].
 struct ViewUniforms {
@@ -146,6 +149,35 @@ __device__ __forceinline__ void project_tet(const ViewUniforms& U, const int32_t
         float sc = signf_(dot(cam, n) + d);
         float so = signf_(dot(o, n) + d);
         if (sc != 0.0f && sc != so) T.visible |= 1u << f;
+    }
+}
+
+// NDC x, y of face f's three vertices only, with project_tet's arithmetic (bit-identical): for a record whose
+// tet already passed culling.  T.ndc of the fourth vertex is left 0; face_screen does not read it.
+__device__ __forceinline__ void project_face(const ViewUniforms& U, const int4 idx, const float* __restrict__ pos,
+                                             int f, ProjectedTet& T) {
+    const int32_t vid[4] = {idx.x, idx.y, idx.z, idx.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) T.ndc[i][0] = T.ndc[i][1] = T.ndc[i][2] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int v = face_vertex(f, k);
+        const float3_ p = load_pos(pos, sel4i(v, vid[0], vid[1], vid[2], vid[3]));
+        float c[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            c[r] = ((U.viewProj[0 * 4 + r] * p.x + U.viewProj[1 * 4 + r] * p.y) + U.viewProj[2 * 4 + r] * p.z) +
+                   U.viewProj[3 * 4 + r] * 1.0f;
+        const float nx = c[0] / c[3], ny = c[1] / c[3];
+        // scatter into slot v (register selects; v is 0..3)
+        T.ndc[0][0] = v == 0 ? nx : T.ndc[0][0];
+        T.ndc[0][1] = v == 0 ? ny : T.ndc[0][1];
+        T.ndc[1][0] = v == 1 ? nx : T.ndc[1][0];
+        T.ndc[1][1] = v == 1 ? ny : T.ndc[1][1];
+        T.ndc[2][0] = v == 2 ? nx : T.ndc[2][0];
+        T.ndc[2][1] = v == 2 ? ny : T.ndc[2][1];
+        T.ndc[3][0] = v == 3 ? nx : T.ndc[3][0];
+        T.ndc[3][1] = v == 3 ? ny : T.ndc[3][1];
     }
 }
 
