@@ -457,59 +457,61 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
         }
         // Flush.  Lanes j and j + 32 hold record j's tet vertices 0, 1 and 2, 3 (7 values each): 128 vertex
         // slots, slot s = v * 32 + j.  Depth-adjacent tets share most vertices (typically 6-7 slots per distinct
-        // vertex in a batch), so the slots of one vertex are summed in LDS, in slot order, by the slot that
-        // comes first, and only that slot issues the global atomics: far fewer memory-side atomics.
+        // vertex in a batch), so the slots of one vertex are summed in an LDS hash table keyed by vertex id
+        // (224 entries of {id, 7 sums}: at most 128 ids, so open addressing always finds a free entry), and
+        // only the slot that created the entry issues the global atomics: far fewer memory-side atomics.
         {
             const int j = lane & 31, half = lane >> 5;
-            float(*s_acc)[7] = reinterpret_cast<float(*)[7]>(&s_w[0][0]);  // 128 x 7 floats, free after the rounds
+            constexpr int kTab = 224;                                        // entries of 8 words: 7168 B
+            int32_t* tab = reinterpret_cast<int32_t*>(&s_w[0][0]);          // s_w is free after the rounds
+            float* tabf = reinterpret_cast<float*>(&s_w[0][0]);
+            for (int e = lane; e < kTab; e += 64) tab[8 * e] = -1;
+            __syncthreads();
             const int s0 = (2 * half) * 32 + j, s1 = s0 + 32;
+            const int32_t* vids = &L.vid[0][0];
+            const bool slot_live = j < cnt;
+            const int32_t v0 = slot_live ? vids[s0] : 0, v1 = slot_live ? vids[s1] : 0;
+            int ent[2] = {-1, -1};
+            bool owner[2] = {false, false};
 #pragma unroll
-            for (int c = 0; c < 7; c++) {
-                s_acc[s0][c] = acc[c];
-                s_acc[s1][c] = acc[7 + c];
+            for (int k = 0; k < 2; k++) {
+                if (!slot_live) continue;
+                const int32_t v = k == 0 ? v0 : v1;
+                int e = (int)(((uint32_t)v * 2654435761u) >> 24) * kTab >> 8;  // multiplicative hash -> [0, kTab)
+                for (int probe = 0; probe < kTab; probe++) {
+                    const int32_t old = atomicCAS(&tab[8 * e], -1, v);
+                    if (old == -1 || old == v) {
+                        ent[k] = e;
+                        owner[k] = old == -1;
+                        break;
+                    }
+                    e = e + 1 == kTab ? 0 : e + 1;
+                }
             }
             __syncthreads();
-            const int32_t* vids = &L.vid[0][0];
-            int32_t v0 = 0, v1 = 0;
+            // the entry's creator stores its values; the other slots of that vertex then add theirs
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                if (owner[k]) {
+#pragma unroll
+                    for (int c = 0; c < 7; c++) tabf[8 * ent[k] + 1 + c] = acc[7 * k + c];
+                }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                if (ent[k] >= 0 && !owner[k]) {
+#pragma unroll
+                    for (int c = 0; c < 7; c++) atomicAdd(&tabf[8 * ent[k] + 1 + c], acc[7 * k + c]);
+                }
+            __syncthreads();
             float sum[2][7];
             bool lead[2] = {false, false};
-            if (j < cnt) {
-                const int4* vids4 = reinterpret_cast<const int4*>(vids);
-                v0 = vids[s0];
-                v1 = vids[s1];
-                // 128-bit masks of the slots holding v0 / v1 (branch-free, broadcast LDS reads)
-                uint32_t m0[4] = {0u, 0u, 0u, 0u}, m1[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-                for (int q = 0; q < kBatch; q++) {
-                    const int4 w = vids4[q];  // slots 4q .. 4q + 3
-                    const uint32_t b0 = (uint32_t)(w.x == v0) | ((uint32_t)(w.y == v0) << 1) |
-                                        ((uint32_t)(w.z == v0) << 2) | ((uint32_t)(w.w == v0) << 3);
-                    const uint32_t b1 = (uint32_t)(w.x == v1) | ((uint32_t)(w.y == v1) << 1) |
-                                        ((uint32_t)(w.z == v1) << 2) | ((uint32_t)(w.w == v1) << 3);
-                    m0[q >> 3] |= b0 << (4 * (q & 7));
-                    m1[q >> 3] |= b1 << (4 * (q & 7));
-                }
-                const uint32_t live = cnt >= 32 ? 0xFFFFFFFFu : ((1u << cnt) - 1u);  // slots of records j < cnt
+            for (int k = 0; k < 2; k++) {
+                // a slot that found no entry (not possible with kTab > 128 ids) issues its own values
+                lead[k] = slot_live && (owner[k] || ent[k] < 0);
 #pragma unroll
-                for (int k = 0; k < 2; k++) {
-                    const uint32_t* m = k == 0 ? m0 : m1;
-                    const int sk = k == 0 ? s0 : s1;
-                    int first = -1;
-#pragma unroll
-                    for (int c = 0; c < 7; c++) sum[k][c] = 0.f;
-#pragma unroll
-                    for (int wd = 0; wd < 4; wd++) {
-                        uint32_t bits = m[wd] & live;
-                        while (bits) {
-                            const int i = 32 * wd + __ffs(bits) - 1;
-                            bits &= bits - 1u;
-                            if (first < 0) first = i;
-#pragma unroll
-                            for (int c = 0; c < 7; c++) sum[k][c] += s_acc[i][c];
-                        }
-                    }
-                    lead[k] = first == sk;  // else another slot of this vertex issues its atomics
-                }
+                for (int c = 0; c < 7; c++) sum[k][c] = owner[k] ? tabf[8 * ent[k] + 1 + c] : acc[7 * k + c];
             }
             // The next batch's vertex data is requested before the atomics, so staging it does not wait for them.
             // Only lanes 0..31 hold that batch's record words.
