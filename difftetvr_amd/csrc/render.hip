@@ -28,12 +28,22 @@ constexpr int kRoundAdj = 64;               // fragments per round in the adjoin
 constexpr int kMaxFrags = kBatch * 64;      // fragments a batch can produce
 constexpr float kMissMarker = -1.0f;        // alpha of a fragment whose ray missed the tet
 
-// kVals = fragments per round.  The adjoint's 64 keep its LDS below 20 KB: eight waves fit in a CU's 160 KB.
-template <int kVals>
+// The forward's fragment list and round values.
 struct FragListLds {
     uint16_t entry[kMaxFrags];  // (pixel lane << 5) | record index
-    float4 val[kVals];          // forward: fragment colour F; adjoint: dL/dF
+    float4 val[kRound];         // fragment colour F
     float rd[3][64];            // ray direction of each pixel
+};
+
+// The adjoint's: with the records, 13 KB per wave, so that three waves fit in each SIMD (a CU has 160 KB).  A
+// batch whose fragments exceed kAdjFrags is processed in two parts of 16 records; the per-fragment gradient rows
+// are reduced half a round (32 fragments) at a time; ray directions stay in registers and move between lanes by
+// permute.  The flush's vertex hash table spans entry and w, which are free by then.
+constexpr int kAdjFrags = 1024;
+struct AdjFragLds {
+    uint16_t entry[kAdjFrags];  // (pixel lane << 5) | record index, one part of a batch
+    float w[32][29];            // gradient rows of half a round (29: conflict-free column access)
+    float4 val[kRoundAdj];      // F, then dL/dF; w = kMissMarker when the ray missed the tet
 };
 
 // Wave-wide comparisons of 5-bit keys from their bit ballots kb[b] = lanes whose key has bit b (valid lanes
@@ -62,8 +72,7 @@ __device__ __forceinline__ uint32_t key_rank_start(const uint64_t kb[5], uint64_
 
 // Coverage masks -> dense fragment list.  Each lane's fragments are contiguous, ordered by record index
 // (ascending, or descending when `reverse`).  Returns the lane's first entry; `total` = all fragments.
-template <int kVals>
-__device__ __forceinline__ uint32_t build_fragment_list(FragListLds<kVals>& F, uint32_t mask, bool reverse, int lane,
+__device__ __forceinline__ uint32_t build_fragment_list(uint16_t* entry, uint32_t mask, bool reverse, int lane,
                                                         uint32_t& total) {
     const uint32_t nm = (uint32_t)__popc(mask);
     const uint32_t inc = wave_inclusive_scan(nm);
@@ -73,17 +82,22 @@ __device__ __forceinline__ uint32_t build_fragment_list(FragListLds<kVals>& F, u
     uint32_t m = mask;
     while (m) {
         const int j = reverse ? 31 - __clz(m) : __ffs(m) - 1;
-        F.entry[k++] = (uint16_t)((lane << 5) | j);
+        entry[k++] = (uint16_t)((lane << 5) | j);
         m &= ~(1u << j);
     }
     return off;
+}
+
+// Ray direction of pixel lane pl from the lanes' own (register) directions; every lane must take part.
+__device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
+    return f3(__shfl(mine.x, pl, 64), __shfl(mine.y, pl, 64), __shfl(mine.z, pl, 64));
 }
 
 __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ RecordLds L;
-    __shared__ FragListLds<kRound> F;
+    __shared__ FragListLds F;
     const uint32_t gtile = global_tile_of_block();
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
@@ -133,7 +147,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
         __syncthreads();
         const uint32_t mask = done ? 0u : lane_coverage(L, cnt, lane);
         uint32_t total;
-        const uint32_t off = build_fragment_list(F, mask, false, lane, total);
+        const uint32_t off = build_fragment_list(F.entry, mask, false, lane, total);
         const uint32_t mine_end = off + (uint32_t)__popc(mask);
         __syncthreads();
         for (uint32_t r0 = 0; r0 < total; r0 += kRound) {
@@ -199,18 +213,16 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
 // 'over' operator is inverted per sub-segment 3, 2, 1 (sub-segment 0 is skipped) with no checkpoints, and
 // each fragment's parameter gradients follow frag_backward_reference.
 template <bool kRef>
-// At least 2 waves per SIMD: the register budget is capped at 256 (the unified VGPR/AGPR file holds 512).
-__global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __restrict__ items,
+// Waves per SIMD: 3 for the exact adjoint (168 VGPRs of the 512 per lane, 13 KB of LDS), 2 for the reference
+// one, whose sub-segment arrays need more of both.
+__global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const BatchItem* __restrict__ items,
                                                         const AdjointTarget* __restrict__ targets, int nitems,
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all,
                                                         int use_abs) {
     __shared__ RecordLds L;
-    __shared__ FragListLds<kRoundAdj> F;
-    __shared__ float s_w[kRoundAdj][29];  // per fragment: 4 vertices x (3 position + 4 colour), padded row
-    __shared__ uint8_t s_hit[kRoundAdj];
+    __shared__ AdjFragLds A;
     __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
-    __shared__ uint32_t s_natom;                         // atomics issued by this tile (when counted)
     const uint32_t gtile = global_tile_of_block();
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
@@ -229,18 +241,13 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
     const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
     const bool valid = x < U.width && y < U.height;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
-    {
-        const float3_ rd = ray_direction(U, x, y);
-        F.rd[0][lane] = rd.x;
-        F.rd[1][lane] = rd.y;
-        F.rd[2][lane] = rd.z;
-    }
+    const float3_ my_rd = ray_direction(U, x, y);
     const float att = U.attenuation;
     const uint2 rg = ranges[gtile];
     const int P = U.width * U.height;
     const int pix = valid ? pixel_index(U, x, y) : 0;
 
-    if (ps.counters != nullptr && lane == 0) s_natom = 0u;
+    uint32_t natom = 0;  // gradient atomics this lane issued (counted when ps.counters is set)
     float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
     float dC[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t n = 0, last = 0;
@@ -301,23 +308,32 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
         // records at or after `last` were not composited in the forward (early ray termination)
         const uint32_t nlive = last > base ? min((uint32_t)cnt, last - base) : 0u;
         const uint32_t live = nlive >= 32u ? 0xFFFFFFFFu : ((1u << nlive) - 1u);
-        const uint32_t mask = done ? 0u : (lane_coverage(L, cnt, lane) & live);
+        const uint32_t mask_all = done ? 0u : (lane_coverage(L, cnt, lane) & live);
+        // parts (back to front): all 32 records, or records 16-31 then 0-15 when the list exceeds kAdjFrags
+        int nparts = 1;
+        {
+            const uint32_t inc = wave_inclusive_scan((uint32_t)__popc(mask_all));
+            if ((uint32_t)__builtin_amdgcn_readlane((int)inc, 63) > (uint32_t)kAdjFrags) nparts = 2;
+        }
+        for (int part = 0; part < nparts; part++) {
+        const uint32_t pmask = nparts == 1 ? 0xFFFFFFFFu : (part == 0 ? 0xFFFF0000u : 0x0000FFFFu);
+        const uint32_t mask = done ? 0u : (mask_all & pmask);
         uint32_t total;
-        const uint32_t off = build_fragment_list(F, mask, true, lane, total);
+        const uint32_t off = build_fragment_list(A.entry, mask, true, lane, total);
         const uint32_t mine_end = off + (uint32_t)__popc(mask);
         __syncthreads();
         for (uint32_t r0 = 0; r0 < total; r0 += kRoundAdj) {
             // A: fragment colours (one fragment per lane; E stays in registers for phase C)
             FragEval E;
+            E.hit = false;
             {
                 const uint32_t e = r0 + lane;
+                const uint32_t ent = e < total ? A.entry[e] : (uint32_t)(lane << 5);
+                const int pl = ent >> 5, j = ent & 31;
+                const float3_ rdp = lane_ray(my_rd, pl);
                 if (e < total) {
-                    const uint32_t ent = F.entry[e];
-                    const int pl = ent >> 5, j = ent & 31;
-                    const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
                     eval_fragment(L, j, ro, rdp, att, E);
-                    F.val[lane] = E.F;
-                    s_hit[lane] = E.hit ? 1 : 0;
+                    A.val[lane] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
                     if constexpr (kRef) {
 #pragma unroll
                         for (int s = 1; s < 4; s++) s_seg[s - 1][lane] = E.seg[s];
@@ -331,7 +347,7 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                 if constexpr (kRef) {
                     // reference: invert sub-segments 3..1 of the fragment; dL/d(sub-segment colour) replaces
                     // the sub-segment colour in s_seg (AdjointIntersectRasterization.glsl:204-228)
-                    if (s_hit[e - r0]) {
+                    if (A.val[e - r0].w != kMissMarker) {
 #pragma unroll
                         for (int s = 3; s > 0; s--) {
                             const float4 acc = s_seg[s - 1][e - r0];
@@ -356,8 +372,8 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                         ck_n2 = jck >= 1 ? ps.ckpt_n[(int64_t)(jck - 1) * P + pix] : 0xFFFFFFFFu;
                     }
                 }
-                if (s_hit[e - r0]) {
-                    const float4 f = F.val[e - r0];
+                const float4 f = A.val[e - r0];
+                if (f.w != kMissMarker) {
                     float4 Cin;
                     if (jck >= 0 && ck_n == n - 1) {
                         Cin = ck_c;
@@ -367,7 +383,7 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                                           C.z - (1.0f - ain) * f.z, ain);
                     }
                     const float omA = 1.0f - Cin.w;
-                    F.val[e - r0] = make_float4(omA * dC[0], omA * dC[1], omA * dC[2], omA * dC[3]);
+                    A.val[e - r0] = make_float4(omA * dC[0], omA * dC[1], omA * dC[2], omA * dC[3]);
                     dC[3] = dC[3] * (1.0f - f.w) - (dC[0] * f.x + dC[1] * f.y + dC[2] * f.z);
                     C = Cin;
                 }
@@ -376,24 +392,24 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
             if (n == 0) done = true;
             __syncthreads();
             // C: gradients of the fragments (one per lane).  Each fragment's 28 values are stored, grouped by
-            // record, into s_w; the owner lanes of each record then add its rows.  No LDS atomics.
+            // record, into A.w (half a round at a time); the owner lanes of each record then add its rows.
             int key = kBatch;  // record index of this lane's fragment, kBatch = none
             float g[28];
 #pragma unroll
             for (int q = 0; q < 28; q++) g[q] = 0.f;
             {
                 const uint32_t e = r0 + lane;
-                if (e < total && s_hit[lane]) {
-                    const uint32_t ent = F.entry[e];
-                    const int pl = ent >> 5, j = ent & 31;
-                    const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
+                const uint32_t ent = e < total ? A.entry[e] : (uint32_t)(lane << 5);
+                const int pl = ent >> 5, j = ent & 31;
+                const float3_ rdp = lane_ray(my_rd, pl);
+                if (e < total && E.hit) {
                     if constexpr (kRef) {
                         const float4 dacc[3] = {s_seg[0][lane], s_seg[1][lane], s_seg[2][lane]};
                         frag_backward_reference(E, L, j, ro, rdp, att, dacc, use_abs != 0, g);
                     } else {
                         float lam0[4], lam1[4], dcol0[4], dcol1[4];
                         float3_ Q0, Q1;
-                        frag_backward_exact(E, L, j, rdp, att, F.val[lane], lam0, lam1, dcol0, dcol1, Q0, Q1);
+                        frag_backward_exact(E, L, j, rdp, att, A.val[lane], lam0, lam1, dcol0, dcol1, Q0, Q1);
                         if (use_abs) {
 #pragma unroll
                             for (int ch = 0; ch < 4; ch++) {
@@ -416,55 +432,63 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                     key = j;
                 }
             }
-            // counting sort of the 64 fragments by record: row = (fragments of lower records) + (rank among the
-            // same record's fragments in lane order); the owner lanes of record j get its segment [seg0, +segn)
-            uint32_t row = 0, seg0 = 0, segn = 0;
-            {
-                uint64_t kb[5];
+            // Per half round (the fragments of lanes 0-31, then 32-63): counting sort by record, row = (fragments
+            // of lower records) + (rank among the same record's fragments in lane order); the owner lanes of
+            // record j (j and j + 32) get its segment [seg0, seg0 + segn) and add its rows.
+#pragma unroll 1
+            for (int h = 0; h < 2; h++) {
+                const int keyh = (lane >> 5) == h ? key : kBatch;
+                uint32_t row = 0, seg0 = 0, segn = 0;
+                {
+                    uint64_t kb[5];
 #pragma unroll
-                for (int b = 0; b < 5; b++) kb[b] = ballot(key < kBatch && ((key >> b) & 1));
-                const uint64_t valid = ballot(key < kBatch);
-                uint32_t own_count;
-                row = key_rank_start(kb, valid, key < kBatch ? key : 0, own_count);
-                row += (uint32_t)popc64(key_equal_mask(kb, valid, key < kBatch ? key : 0) & lanemask_lt());
-                seg0 = key_rank_start(kb, valid, lane & 31, segn);
-            }
-            if (key < kBatch) {
+                    for (int b = 0; b < 5; b++) kb[b] = ballot(keyh < kBatch && ((keyh >> b) & 1));
+                    const uint64_t valid = ballot(keyh < kBatch);
+                    uint32_t own_count;
+                    row = key_rank_start(kb, valid, keyh < kBatch ? keyh : 0, own_count);
+                    row += (uint32_t)popc64(key_equal_mask(kb, valid, keyh < kBatch ? keyh : 0) & lanemask_lt());
+                    seg0 = key_rank_start(kb, valid, lane & 31, segn);
+                }
+                if (keyh < kBatch) {
 #pragma unroll
-                for (int q = 0; q < 28; q++) s_w[row][q] = g[q];
-            }
-            __syncthreads();
-            {
-                // two rows per step: 28 independent LDS reads in flight instead of 14
-                const int half = lane >> 5;
-                uint32_t rr = 0;
-                for (; rr + 1 < segn; rr += 2) {
-                    float a[14], b[14];
+                    for (int q = 0; q < 28; q++) A.w[row][q] = g[q];
+                }
+                __syncthreads();
+                {
+                    // two rows per step: 28 independent LDS reads in flight instead of 14
+                    const int half = lane >> 5;
+                    uint32_t rr = 0;
+                    for (; rr + 1 < segn; rr += 2) {
+                        float a[14], b[14];
 #pragma unroll
-                    for (int q = 0; q < 14; q++) {
-                        a[q] = s_w[seg0 + rr][half * 14 + q];
-                        b[q] = s_w[seg0 + rr + 1][half * 14 + q];
+                        for (int q = 0; q < 14; q++) {
+                            a[q] = A.w[seg0 + rr][half * 14 + q];
+                            b[q] = A.w[seg0 + rr + 1][half * 14 + q];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 14; q++) acc[q] += a[q] + b[q];
                     }
+                    if (rr < segn) {
 #pragma unroll
-                    for (int q = 0; q < 14; q++) acc[q] += a[q] + b[q];
+                        for (int q = 0; q < 14; q++) acc[q] += A.w[seg0 + rr][half * 14 + q];
+                    }
                 }
-                if (rr < segn) {
-#pragma unroll
-                    for (int q = 0; q < 14; q++) acc[q] += s_w[seg0 + rr][half * 14 + q];
-                }
+                __syncthreads();
             }
-            __syncthreads();
         }
+        }  // parts
         // Flush.  Lanes j and j + 32 hold record j's tet vertices 0, 1 and 2, 3 (7 values each): 128 vertex
         // slots, slot s = v * 32 + j.  Depth-adjacent tets share most vertices (typically 6-7 slots per distinct
         // vertex in a batch), so the slots of one vertex are summed in an LDS hash table keyed by vertex id
-        // (224 entries of {id, 7 sums}: at most 128 ids, so open addressing always finds a free entry), and
-        // only the slot that created the entry issues the global atomics: far fewer memory-side atomics.
+        // (180 entries of {id, 7 sums}; a slot that finds no free entry within the table issues its own
+        // atomics), and only the slot that created the entry issues the global atomics: far fewer memory-side
+        // atomics.
         {
             const int j = lane & 31, half = lane >> 5;
-            constexpr int kTab = 224;                                        // entries of 8 words: 7168 B
-            int32_t* tab = reinterpret_cast<int32_t*>(&s_w[0][0]);          // s_w is free after the rounds
-            float* tabf = reinterpret_cast<float*>(&s_w[0][0]);
+            constexpr int kTab = (int)((sizeof(A.entry) + sizeof(A.w)) / 32);  // entries of 8 words
+            static_assert(kTab >= 160, "hash table too small for 128 vertex slots");
+            int32_t* tab = reinterpret_cast<int32_t*>(&A.entry[0]);  // entry and w are free after the rounds
+            float* tabf = reinterpret_cast<float*>(&A.entry[0]);
             for (int e = lane; e < kTab; e += 64) tab[8 * e] = -1;
             __syncthreads();
             const int s0 = (2 * half) * 32 + j, s1 = s0 + 32;
@@ -532,14 +556,14 @@ __global__ __launch_bounds__(64, 2) void k_render_adjoint(const BatchItem* __res
                         atomic_add_f32(&grad_col[4 * g + (c - 3)], val);
                 }
             }
-            if (ps.counters != nullptr && issued) atomicAdd(&s_natom, issued);
+            natom += issued;
         }
         if (ballot(!done) == 0ull) break;
         __syncthreads();
     }
     if (ps.counters != nullptr) {
-        __syncthreads();
-        if (lane == 0 && s_natom) atomicAdd(ps.counters, (unsigned long long)s_natom);
+        const uint32_t tot = wave_inclusive_scan(natom);
+        if (lane == 63 && tot) atomicAdd(ps.counters, (unsigned long long)tot);
     }
 }
 
