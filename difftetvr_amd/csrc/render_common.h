@@ -121,6 +121,38 @@ struct FragEval {
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
+// The 4 sub-segments of a segment of length 4 tSeg between the entry and exit colours (RGB + density):
+// seg[s] = accumulateLinearConst of sub-segment s (ForwardCommon.glsl:42-49, colour endpoints mixed at s/4 and
+// (s+1)/4, density at the midpoint, IntersectRasterization.glsl:228-241), pre[s] = their composite before s,
+// F = the composite of all four.  The forward and the exact backward both call this, so the backward can
+// recompute seg/pre instead of keeping them in registers from phase A.
+__device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSeg, float att, float4 seg[4],
+                                             float4 pre[4], float4& F) {
+    float4 rc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
+        const float c0x = mixf(col0.x, col1.x, fb), c0y = mixf(col0.y, col1.y, fb), c0z = mixf(col0.z, col1.z, fb);
+        const float c1x = mixf(col0.x, col1.x, fe), c1y = mixf(col0.y, col1.y, fe), c1z = mixf(col0.z, col1.z, fe);
+        const float a = mixf(col0.w, col1.w, fm) * att;
+        float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!(a < 1e-6f)) {
+            const float A = __expf(-a * tSeg);
+            const float ia = __fdividef(1.0f, a);
+            const float kk = (tSeg + ia) * A - ia;
+            cur = make_float4((1.0f - A) * c0x + kk * (c0x - c1x), (1.0f - A) * c0y + kk * (c0y - c1y),
+                              (1.0f - A) * c0z + kk * (c0z - c1z), 1.0f - A);
+        }
+        pre[s] = rc;
+        seg[s] = cur;
+        rc.x = rc.x + (1.0f - rc.w) * cur.x;
+        rc.y = rc.y + (1.0f - rc.w) * cur.y;
+        rc.z = rc.z + (1.0f - rc.w) * cur.z;
+        rc.w = rc.w + (1.0f - rc.w) * cur.w;
+    }
+    F = rc;
+}
+
 // Barycentric coordinates with unsigned areas and the 1e-5 area floor (BarycentricInterpolation.glsl:29-38).
 __device__ __forceinline__ void bary_ref(float3_ p0, float3_ p1, float3_ p2, float3_ P, float& u, float& v) {
     const float3_ d20 = sub(p2, p0), d21 = sub(p2, p1);
@@ -233,32 +265,7 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
         E.col1 = make_float4((ca.x * u + cb.x * v) + cc.x * w, (ca.y * u + cb.y * v) + cc.y * w,
                              (ca.z * u + cb.z * v) + cc.z * w, (ca.w * u + cb.w * v) + cc.w * w);
     }
-    const float tSeg = (tout - tin) / 4.0f;
-    float4 rc = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
-        const float c0x = mixf(E.col0.x, E.col1.x, fb), c0y = mixf(E.col0.y, E.col1.y, fb),
-                    c0z = mixf(E.col0.z, E.col1.z, fb);
-        const float c1x = mixf(E.col0.x, E.col1.x, fe), c1y = mixf(E.col0.y, E.col1.y, fe),
-                    c1z = mixf(E.col0.z, E.col1.z, fe);
-        const float a = mixf(E.col0.w, E.col1.w, fm) * att;
-        float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!(a < 1e-6f)) {
-            const float A = __expf(-a * tSeg);
-            const float ia = __fdividef(1.0f, a);
-            const float kk = (tSeg + ia) * A - ia;
-            cur = make_float4((1.0f - A) * c0x + kk * (c0x - c1x), (1.0f - A) * c0y + kk * (c0y - c1y),
-                              (1.0f - A) * c0z + kk * (c0z - c1z), 1.0f - A);
-        }
-        E.pre[s] = rc;
-        E.seg[s] = cur;
-        rc.x = rc.x + (1.0f - rc.w) * cur.x;
-        rc.y = rc.y + (1.0f - rc.w) * cur.y;
-        rc.z = rc.z + (1.0f - rc.w) * cur.z;
-        rc.w = rc.w + (1.0f - rc.w) * cur.w;
-    }
-    E.F = rc;
+    sub_segments(E.col0, E.col1, (tout - tin) / 4.0f, att, E.seg, E.pre, E.F);
 }
 
 __device__ __forceinline__ int pixel_index(const ViewUniforms& U, int x, int y) { return y * U.width + x; }
@@ -319,9 +326,11 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     }
     float dLdt = 0.f;
     const float tSeg = (E.t1 - E.t0) / 4.0f;
+    float4 seg[4], pre[4], Fr;
+    sub_segments(E.col0, E.col1, tSeg, att, seg, pre, Fr);  // as in eval_fragment: not kept live since phase A
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
-        const float4 sg = E.seg[s], st = E.pre[s];
+        const float4 sg = seg[s], st = pre[s];
         const float omA = 1.0f - st.w;
         const float dseg[4] = {omA * dR[0], omA * dR[1], omA * dR[2], omA * dR[3]};
         dR[3] = dR[3] * (1.0f - sg.w) - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
