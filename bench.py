@@ -475,8 +475,8 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU per box "
-              "(views and requests would shard across GPUs; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus} is not supported: this benchmark runs on one GPU (views and "
+              "requests would shard across GPUs, each rendering its share).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "multiview":

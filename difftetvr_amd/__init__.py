@@ -5,10 +5,10 @@ Drop-in for the renderer/optimiser path of the reference Python module ``difftet
 meaning, tensor layouts and error behaviour.  The compute runs in hand-written HIP kernels behind the C ABI of
 ``include/difftetvr_amd.h``; PyTorch supplies device memory and the current stream.
 
-Differences from the reference are listed in DESIGN.md; in short: every RendererType renders with the
-INTERSECTION renderer's maths (Euclidean segment length, 4 linear sub-segments per tet, face-centroid depth
-order), gradients are the exact derivative of that forward (``set_adjoint_mode`` selects the reference's own
-adjoint where supported), and only GPU tensors are supported.
+Differences from the reference: every RendererType renders with the INTERSECTION renderer's maths (Euclidean
+segment length, 4 linear sub-segments per tet, face-centroid depth order), gradients are the exact derivative
+of that forward (``set_adjoint_mode("reference")`` selects the reference's own adjoint), coarse-to-fine
+refinement chooses prism diagonals by vertex index, and only GPU tensors are supported.
 """
 import ctypes
 import enum
