@@ -459,6 +459,8 @@ def main():
                     help="single: capture the step in a HIP graph (planned rendering) and replay it")
     ap.add_argument("--checkpoint-interval", type=int, default=64,
                     help="single: segments between the adjoint's stored states (K)")
+    ap.add_argument("--tile-order", choices=["cost", "row-major"], default="cost",
+                    help="single: launch order of the screen tiles (heaviest first, or row-major)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-row-step", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -520,6 +522,7 @@ def main():
     target = r_gt.render().detach().clone()
     mesh, r = make(w["pos"], True)
     r.set_checkpoint_interval(args.checkpoint_interval)
+    r.set_tile_order(args.tile_order)
     opt = FusedAdam([{"params": [mesh.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
                      {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True,
                     device_step=args.graph)
@@ -607,6 +610,7 @@ def main():
                    "segments": int(r.get_fragment_counts().to(torch.int64).sum()),
                    "gradient_atomics": stats["num_gradient_atomics"],
                    "hip_graph": bool(args.graph), "checkpoint_interval": args.checkpoint_interval,
+                   "tile_order": args.tile_order,
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
         "roofline": {"bound": "hbm", "kernel": f"k_render_{dominant}", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

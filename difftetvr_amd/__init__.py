@@ -325,6 +325,7 @@ class Renderer:
         self._coarse_to_fine_target = 0
         self._adjoint_mode = _lib.DTV_ADJOINT_EXACT
         self._checkpoint_interval = 64
+        self._tile_order = 0
         self._rendered_mesh = None
 
     def __del__(self):
@@ -394,6 +395,14 @@ class Renderer:
     def set_checkpoint_interval(self, k):
         self._checkpoint_interval = int(k)
 
+    def set_tile_order(self, order):
+        """Launch order of the screen tiles: 'cost' (default: heaviest first) or 'row-major'.  Images and
+        gradients do not depend on it (up to the order of the gradients' float atomics)."""
+        o = {"cost": 0, "row-major": 1}.get(str(order).lower())
+        if o is None:
+            raise RuntimeError(f"set_tile_order: unknown order '{order}'.")
+        self._tile_order = o
+
     def set_record_capacity(self, capacity):
         """Planned rendering (``dtv_view_set_record_capacity``): > 0 sizes the (tile, face) record buffers for
         ``capacity`` records and removes the per-render host synchronisation, so that render / adjoint / Adam
@@ -425,6 +434,7 @@ class Renderer:
         p.early_ray_out_alpha = 1.0 - self._ert_thresh
         p.tile_size = 8
         p.checkpoint_interval = self._checkpoint_interval
+        p.tile_order = self._tile_order
         return p
 
     def camera_uniforms(self):

@@ -32,7 +32,8 @@ class DtvCamera(ctypes.Structure):
 
 class DtvRenderParams(ctypes.Structure):
     _fields_ = [("attenuation", c_float), ("clear_color", c_float * 4), ("use_early_ray_termination", c_int32),
-                ("early_ray_out_alpha", c_float), ("tile_size", c_int32), ("checkpoint_interval", c_int32)]
+                ("early_ray_out_alpha", c_float), ("tile_size", c_int32), ("checkpoint_interval", c_int32),
+                ("tile_order", c_int32)]
 
 
 class DtvViewStats(ctypes.Structure):

@@ -157,6 +157,8 @@ struct dtv_view {
         recs, hist,
         lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
+    DevBuf tile_frags, tile_order;  // per tile: forward's fragments; launch orders of the forward and adjoint
+    bool cost_ordered = true;       // the last render launched its tiles heaviest first
     int64_t planned_cap = 0;  // > 0: planned (synchronisation-free) rendering with this record capacity
     // state of the last render
     bool rendered = false;
@@ -183,6 +185,7 @@ struct dtv_view {
         ps.ckpt_color = ckpt_color.as<float4>();
         ps.ckpt_n = ckpt_n.as<uint32_t>();
         ps.counters = nullptr;
+        ps.tile_frags = tile_frags.as<uint32_t>();
         return ps;
     }
     bool atomics_counted = false;  // the last adjoint ran with timing on and counted its atomics
@@ -232,6 +235,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     if (n <= 0 || n > kMaxBatchItems) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": 1..4096 items required");
     const int tile = params->tile_size == 0 ? 8 : params->tile_size;
     if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_size must be 8");
+    if (params->tile_order != 0 && params->tile_order != 1)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_order must be 0 (heaviest first) or 1 (row-major)");
     std::vector<BatchItem> items((size_t)n);
     std::vector<int64_t> item_V((size_t)n);
     int64_t blocks = 0, tiles = 0, pixels = 0;
@@ -357,8 +362,14 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                                          view->hist.as<uint32_t>() + 8 * 256, route, s);
     view->mark(DTV_STAGE_SORT, 1, s);
     DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
+    DTV_HIP(view->tile_frags.ensure(sizeof(uint32_t) * tiles));
+    DTV_HIP(view->tile_order.ensure(2 * sizeof(uint32_t) * tiles));
+    view->cost_ordered = params->tile_order == 0;
+    // forward launch order: tiles by record count, heaviest first
+    const uint32_t* fwd_order = view->cost_ordered ? view->tile_order.as<uint32_t>() : nullptr;
     view->mark(DTV_STAGE_RANGES, 0, s);
     launch_tile_ranges(route, K, n_dev, tiles, view->ranges.as<uint2>(), s);
+    if (fwd_order) launch_tile_order(view->ranges.as<uint2>(), nullptr, tiles, view->tile_order.as<uint32_t>(), s);
     launch_gather_records(dI, n, route, K, n_dev, view->recs.as<uint4>(), s);
     view->mark(DTV_STAGE_RANGES, 1, s);
 
@@ -370,7 +381,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->ckpt_color.ensure(sizeof(float4) * pixels * kMaxCheckpoints));
     DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * pixels * kMaxCheckpoints));
     view->mark(DTV_STAGE_FORWARD, 0, s);
-    launch_render_forward(dI, n, tiles, view->sorted_recs(), view->ranges.as<uint2>(), view->pixel_state(), s);
+    launch_render_forward(dI, n, tiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
+                          view->pixel_state(), s);
     view->mark(DTV_STAGE_FORWARD, 1, s);
     DTV_HIP(hipGetLastError());
     view->items.swap(items);
@@ -424,9 +436,17 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         ps.counters = reinterpret_cast<unsigned long long*>(view->small.as<char>() + 256);
         fill_u32(reinterpret_cast<uint32_t*>(ps.counters), 0u, 2, s);
     }
+    // adjoint launch order: tiles by the forward's composited fragments, heaviest first
+    const uint32_t* adj_order = nullptr;
+    if (view->cost_ordered) {
+        uint32_t* o = view->tile_order.as<uint32_t>() + view->total_tiles;
+        launch_tile_order(nullptr, view->tile_frags.as<uint32_t>(), view->total_tiles, o, s);
+        adj_order = o;
+    }
     view->mark(DTV_STAGE_ADJOINT, 0, s);
     launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
-                          view->sorted_recs(), view->ranges.as<uint2>(), ps, use_abs_grad ? 1 : 0, adjoint_mode, s);
+                          adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps, use_abs_grad ? 1 : 0,
+                          adjoint_mode, s);
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;

@@ -280,6 +280,79 @@ __global__ void k_tile_ranges(const SortRoute* __restrict__ route, int64_t n_cap
     }
 }
 
+// Tile launch order, heaviest first.  A tile's rendering time grows with its records and fragments, and tiles are
+// handed to the CUs in workgroup order: in row-major order the heavy tiles of the image centre start mid-launch
+// and the last of them finish long after the rest of the GPU has gone idle.  Started first, the light tiles fill
+// in behind them.  Costs are ranked coarsely (8 buckets per octave, 256 buckets for 32 bits).
+__device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
+    const uint32_t v = c < 0xFFFFFFFFu ? c + 1u : c;  // >= 1
+    const int e = 31 - __clz((int)v);
+    const uint32_t m = e >= 3 ? (v >> (e - 3)) & 7u : (v << (3 - e)) & 7u;
+    return 8u * (uint32_t)e + m;
+}
+
+// Bucket of tile i (0 for i >= n), the lanes of the wave with the same bucket (eq, live lanes only) and this
+// lane's rank among them.  Every lane of the wave must take part.
+__device__ __forceinline__ uint32_t tile_bucket_match(const uint2* __restrict__ ranges, const uint32_t* __restrict__ cost,
+                                                      uint32_t i, uint32_t n, uint64_t& eq, uint32_t& rank) {
+    const bool live = i < n;
+    uint32_t b = 0;
+    if (live) b = cost_bucket(cost != nullptr ? cost[i] : ranges[i].y - ranges[i].x);
+    eq = ballot(live);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint64_t bk = ballot(((b >> k) & 1u) != 0u);
+        eq &= ((b >> k) & 1u) ? bk : ~bk;
+    }
+    rank = (uint32_t)popc64(eq & lanemask_lt());
+    return b;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ ranges,
+                                                     const uint32_t* __restrict__ cost, uint32_t n,
+                                                     uint32_t* __restrict__ order) {
+    __shared__ uint32_t count[256], start[256];
+    const uint32_t t = threadIdx.x;
+    if (t < 256) count[t] = 0u;
+    __syncthreads();
+    // trip counts are uniform over the workgroup, so every lane of a wave reaches the ballots
+    for (uint32_t i0 = 0; i0 < n; i0 += 1024u) {
+        uint64_t eq;
+        uint32_t rank;
+        const uint32_t b = tile_bucket_match(ranges, cost, i0 + t, n, eq, rank);
+        if (i0 + t < n && rank == 0u) atomicAdd(&count[b], (uint32_t)popc64(eq));
+    }
+    __syncthreads();
+    if (t < 64) {
+        // start[b] = tiles in the buckets above b; lane l takes buckets 255 - 4l down to 252 - 4l
+        uint32_t c[4], sum = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c[k] = count[255 - 4 * t - k];
+            sum += c[k];
+        }
+        uint32_t run = wave_inclusive_scan(sum) - sum;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            start[255 - 4 * t - k] = run;
+            run += c[k];
+        }
+    }
+    __syncthreads();
+    const int lane = (int)(t & 63u);
+    for (uint32_t i0 = 0; i0 < n; i0 += 1024u) {
+        uint64_t eq;
+        uint32_t rank;
+        const uint32_t i = i0 + t;
+        const uint32_t b = tile_bucket_match(ranges, cost, i, n, eq, rank);
+        uint32_t base = 0u;
+        if (i < n && rank == 0u) base = atomicAdd(&start[b], (uint32_t)popc64(eq));
+        const int leader = eq != 0ull ? __ffsll((unsigned long long)eq) - 1 : lane;
+        base = __shfl(base, leader, 64);
+        if (i < n) order[base + rank] = i;
+    }
+}
+
 // Sorted record i = out[2i], out[2i + 1] = {the tet's 4 vertex indices}, {face id, coverage mask low, high, 0}:
 // the face sorted_ids[i] is projected again (the same arithmetic as binning) and its exact coverage of the
 // record's tile computed.  Carrying the vertex indices saves the renderer one dependent load per record.
@@ -337,6 +410,12 @@ void launch_tile_ranges(const SortRoute* route, int64_t n, const uint64_t* n_dev
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_tile_ranges, dim3((unsigned)blocks), dim3(256), 0, s, route, n, n_dev, ranges);
+}
+
+void launch_tile_order(const uint2* ranges, const uint32_t* cost, int64_t num_tiles, uint32_t* order,
+                       hipStream_t s) {
+    if (num_tiles <= 0) return;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, ranges, cost, (uint32_t)num_tiles, order);
 }
 
 void launch_gather_records(const BatchItem* items, int nitems, const SortRoute* route, int64_t n,

@@ -58,6 +58,11 @@ void launch_tile_ranges(const SortRoute* route, int64_t n, const uint64_t* n_dev
 // tile renderer's record stream.
 void launch_gather_records(const BatchItem* items, int nitems, const SortRoute* route, int64_t n,
                            const uint64_t* n_dev, uint4* out, hipStream_t s);
+// Launch order of the tile renderers: order[0, num_tiles) lists every tile once, heaviest first, by the tile's
+// cost: its record count (cost == NULL: ranges[t].y - ranges[t].x) or cost[t].  Costs are ranked in buckets of
+// 1/8 octave; the order within a bucket is unspecified.  One workgroup; needs no cleared memory.
+void launch_tile_order(const uint2* ranges, const uint32_t* cost, int64_t num_tiles, uint32_t* order,
+                       hipStream_t s);
 
 // render.hip
 // Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's
@@ -70,12 +75,14 @@ struct PixelState {
     float4* ckpt_color;     // [max_ckpt][P]
     uint32_t* ckpt_n;       // [max_ckpt][P]
     unsigned long long* counters;  // optional (NULL): [0] += gradient float atomics issued by the adjoint
+    uint32_t* tile_frags;   // [tiles of the batch] fragments composited per tile (written by the forward)
 };
-void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
-                           const uint2* ranges, PixelState ps, hipStream_t s);
+// order (may be NULL = tile t in workgroup t): the tile each workgroup renders, from launch_tile_order.
+void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
+                           const uint4* recs, const uint2* ranges, PixelState ps, hipStream_t s);
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
-                           const uint4* recs, const uint2* ranges, PixelState ps, int use_abs, int mode,
-                           hipStream_t s);
+                           const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
+                           int mode, hipStream_t s);
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 

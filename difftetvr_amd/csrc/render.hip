@@ -94,11 +94,12 @@ __device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
 }
 
 __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
+                                                        const uint32_t* __restrict__ order,
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ RecordLds L;
     __shared__ FragListLds F;
-    const uint32_t gtile = global_tile_of_block();
+    const uint32_t gtile = global_tile_of_block(order);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
     const float* __restrict__ pos = B.pos;
@@ -206,6 +207,9 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
         ps.last_rec[pix] = last;
         ps.num_ckpt[pix] = nck;
     }
+    // the tile's composited fragments: the adjoint's launch order ranks tiles by them
+    const uint32_t tile_total = wave_inclusive_scan(n);
+    if (lane == 63) ps.tile_frags[gtile] = tile_total;
 }
 
 // kRef = false: exact derivative of the forward.  kRef = true: the reference's adjoint as written
@@ -217,13 +221,14 @@ template <bool kRef>
 // one, whose sub-segment arrays need more of both.
 __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const BatchItem* __restrict__ items,
                                                         const AdjointTarget* __restrict__ targets, int nitems,
+                                                        const uint32_t* __restrict__ order,
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all,
                                                         int use_abs) {
     __shared__ RecordLds L;
     __shared__ AdjFragLds A;
     __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
-    const uint32_t gtile = global_tile_of_block();
+    const uint32_t gtile = global_tile_of_block(order);
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
     const ViewUniforms& U = B.U;
@@ -573,7 +578,7 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
                                                           const uint2* __restrict__ ranges, PixelState ps_all,
                                                           int max_per_pixel, int32_t* __restrict__ out_ids_all) {
     __shared__ RecordLds L;
-    const uint32_t gtile = global_tile_of_block();
+    const uint32_t gtile = global_tile_of_block(nullptr);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
     const float* __restrict__ pos = B.pos;
@@ -608,22 +613,23 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
         for (; k < max_per_pixel; k++) out_ids[(int64_t)pix * max_per_pixel + k] = -1;
 }
 
-void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
-                           const uint2* ranges, PixelState ps, hipStream_t s) {
+void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
+                           const uint4* recs, const uint2* ranges, PixelState ps, hipStream_t s) {
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_render_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, recs, ranges, ps);
+    hipLaunchKernelGGL(k_render_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order, recs, ranges,
+                       ps);
 }
 
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
-                           const uint4* recs, const uint2* ranges, PixelState ps, int use_abs, int mode,
-                           hipStream_t s) {
+                           const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
+                           int mode, hipStream_t s) {
     if (ntiles <= 0) return;
     if (mode == 1)
         hipLaunchKernelGGL(k_render_adjoint<true>, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems,
-                           recs, ranges, ps, use_abs);
+                           order, recs, ranges, ps, use_abs);
     else
         hipLaunchKernelGGL(k_render_adjoint<false>, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems,
-                           recs, ranges, ps, use_abs);
+                           order, recs, ranges, ps, use_abs);
 }
 
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,

@@ -270,8 +270,11 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
 
 __device__ __forceinline__ int pixel_index(const ViewUniforms& U, int x, int y) { return y * U.width + x; }
 
-// Batch tile processed by this workgroup.
-__device__ __forceinline__ uint32_t global_tile_of_block() { return blockIdx.x; }
+// Batch tile processed by this workgroup: order[blockIdx.x] (a permutation of the batch's tiles), or
+// blockIdx.x when order is NULL.
+__device__ __forceinline__ uint32_t global_tile_of_block(const uint32_t* __restrict__ order) {
+    return order != nullptr ? order[blockIdx.x] : blockIdx.x;
+}
 
 // Per-pixel state of one batch item: its slice of the batch's concatenated arrays.
 __device__ __forceinline__ PixelState item_pixel_state(const PixelState& all, const BatchItem& B) {
@@ -283,6 +286,7 @@ __device__ __forceinline__ PixelState item_pixel_state(const PixelState& all, co
     ps.ckpt_color = all.ckpt_color + (int64_t)B.U.max_ckpt * B.pixel0;
     ps.ckpt_n = all.ckpt_n + (int64_t)B.U.max_ckpt * B.pixel0;
     ps.counters = all.counters;
+    ps.tile_frags = all.tile_frags;  // indexed by batch tile
     return ps;
 }
 

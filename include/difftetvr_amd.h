@@ -63,6 +63,9 @@ typedef struct dtv_render_params {
     float early_ray_out_alpha;         /* stop once accumulated alpha > this; reference default 1 - 1e-4 */
     int32_t tile_size;                 /* screen tile edge in pixels: 8 (default) */
     int32_t checkpoint_interval;       /* fragments between adjoint checkpoints; 0 = 64 */
+    int32_t tile_order;                /* launch order of screen tiles (no reference counterpart: the reference
+                                          rasterises): 0 = heaviest first (default; forward by record count,
+                                          adjoint by the forward's fragment count), 1 = row-major */
 } dtv_render_params;
 
 typedef struct dtv_view_stats {
