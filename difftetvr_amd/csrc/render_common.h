@@ -175,12 +175,34 @@ __device__ __forceinline__ bool ref_face_contains(const RecordLds& L, int j, int
     return true;
 }
 
+// ref_face_contains at P = ro + t rd, and when P is inside, its barycentric coordinates from the same edge cross
+// products: u = |(p2 - p1) x (P - p1)| / area (the sub-triangle opposite p0), v = |(p0 - p2) x (P - p2)| / area
+// (opposite p1), area = max(|N|, 1e-5) with N = (p1 - p0) x (p2 - p0): the unsigned areas of
+// BarycentricInterpolation.glsl:29-38, whose triangle (P, p0, p2) this spans from p2 instead of p0.
+__device__ __forceinline__ bool ref_face_contains_bary(const RecordLds& L, int j, int g, float3_ ro, float3_ rd,
+                                                       float t, float& u, float& v) {
+    const float3_ p0 = lds_p(L, face_vertex(g, 0), j), p1 = lds_p(L, face_vertex(g, 1), j),
+                  p2 = lds_p(L, face_vertex(g, 2), j);
+    const float3_ N = f3(L.N[3 * g][j], L.N[3 * g + 1][j], L.N[3 * g + 2][j]);
+    const float3_ P = add(ro, scale(rd, t));
+    const float3_ c0 = cross(sub(p1, p0), sub(P, p0));
+    const float3_ c1 = cross(sub(p2, p1), sub(P, p1));
+    const float3_ c2 = cross(sub(p0, p2), sub(P, p2));
+    if (dot(N, c0) < 0.0f || dot(N, c1) < 0.0f || dot(N, c2) < 0.0f) return false;
+    // continuous quantities (no discrete decision depends on them): fast division
+    const float area = fmaxf(length(N), 1e-5f);
+    u = __fdividef(length(c1), area);
+    v = __fdividef(length(c2), area);
+    return true;
+}
+
 // Entry/exit faces and distances with the semantics of intersectRayTet (RayIntersectionTests.glsl:78-106).
 // Fast path: the plane-slab test (last entering / first exiting plane) proposes the two faces, and the
 // reference inside test confirms both; otherwise the reference's four-face procedure decides (edges,
 // vertices and silhouettes, where float rounding can make the reference reject a face).
+// On the fast path, bary = {u, v} at the entry and exit points and have_bary = true (else false).
 __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, float3_ rd, int& f0, int& f1, float& t0,
-                                        float& t1) {
+                                        float& t1, float bary[4], bool& have_bary) {
     const bool outward = (L.meta[j] >> 3) & 1u;
     float tin = -__builtin_inff(), tout = __builtin_inff();
     int fi = -1, fo = -1;
@@ -200,14 +222,17 @@ __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, f
             if (tf[g] < tout) { tout = tf[g]; fo = g; }
         }
     }
-    if (fi >= 0 && fo >= 0 && tout >= tin && tin >= 0.0f && ref_face_contains(L, j, fi, ro, rd, tin) &&
-        ref_face_contains(L, j, fo, ro, rd, tout)) {
+    if (fi >= 0 && fo >= 0 && tout >= tin && tin >= 0.0f &&
+        ref_face_contains_bary(L, j, fi, ro, rd, tin, bary[0], bary[1]) &&
+        ref_face_contains_bary(L, j, fo, ro, rd, tout, bary[2], bary[3])) {
         f0 = fi;
         f1 = fo;
         t0 = tin;
         t1 = tout;
+        have_bary = true;
         return true;
     }
+    have_bary = false;
     t0 = 1e9f;
     t1 = -1e9f;
     f0 = 0;
@@ -231,8 +256,9 @@ __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, f
 __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_ ro, float3_ rd, float att,
                                               FragEval& E) {
     int fi, fo;
-    float tin, tout;
-    E.hit = ray_tet(L, j, ro, rd, fi, fo, tin, tout);
+    float tin, tout, bary[4];
+    bool have_bary;
+    E.hit = ray_tet(L, j, ro, rd, fi, fo, tin, tout, bary, have_bary);
     if (!E.hit) {
         E.F = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
@@ -241,12 +267,16 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
     E.f1 = fo;
     E.t0 = tin;
     E.t1 = tout;
-    const float3_ P0 = add(ro, scale(rd, tin));
-    const float3_ P1 = add(ro, scale(rd, tout));
+    if (!have_bary) {  // edges, vertices, silhouettes, camera inside the tet: the faces' own barycentrics
+        const float3_ P0 = add(ro, scale(rd, tin)), P1 = add(ro, scale(rd, tout));
+        bary_ref(lds_p(L, face_vertex(fi, 0), j), lds_p(L, face_vertex(fi, 1), j), lds_p(L, face_vertex(fi, 2), j),
+                 P0, bary[0], bary[1]);
+        bary_ref(lds_p(L, face_vertex(fo, 0), j), lds_p(L, face_vertex(fo, 1), j), lds_p(L, face_vertex(fo, 2), j),
+                 P1, bary[2], bary[3]);
+    }
     {
         const int a = face_vertex(fi, 0), b = face_vertex(fi, 1), c = face_vertex(fi, 2);
-        float u, v;
-        bary_ref(lds_p(L, a, j), lds_p(L, b, j), lds_p(L, c, j), P0, u, v);
+        const float u = bary[0], v = bary[1];
         const float w = 1.0f - u - v;
         const float4 ca = lds_c(L, a, j), cb = lds_c(L, b, j), cc = lds_c(L, c, j);
         E.u0 = u;
@@ -256,8 +286,7 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
     }
     {
         const int a = face_vertex(fo, 0), b = face_vertex(fo, 1), c = face_vertex(fo, 2);
-        float u, v;
-        bary_ref(lds_p(L, a, j), lds_p(L, b, j), lds_p(L, c, j), P1, u, v);
+        const float u = bary[2], v = bary[3];
         const float w = 1.0f - u - v;
         const float4 ca = lds_c(L, a, j), cb = lds_c(L, b, j), cc = lds_c(L, c, j);
         E.u1 = u;
@@ -300,7 +329,7 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
     const float3_ a = lds_p(L, ia, j), b = lds_p(L, ib, j), c = lds_p(L, ic, j);
     const float4 ca = lds_c(L, ia, j), cb = lds_c(L, ib, j), cc = lds_c(L, ic, j);
     const float3_ e1 = sub(b, a), e2 = sub(c, a);
-    const float3_ n = cross(e1, e2);
+    const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);  // = cross(e1, e2), staged
     const float inn = __fdividef(1.0f, dot(n, n));
     const float3_ gb = scale(cross(e2, n), inn), gc = scale(cross(n, e1), inn);
     // G^T dC and dC . (G d)
@@ -314,9 +343,8 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
     }
     const float3_ gtdc = add(scale(gb, sb), scale(gc, sc));
     const float dcgd = dot(gtdc, rd);
-    const float3_ N = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);
-    const float s = (sdLdt + dcgd) / dot(N, rd);
-    Q = sub(scale(N, s), gtdc);
+    const float s = (sdLdt + dcgd) / dot(n, rd);
+    Q = sub(scale(n, s), gtdc);
 }
 
 __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const RecordLds& L, int j, float3_ rd,
