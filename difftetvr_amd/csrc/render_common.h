@@ -121,6 +121,14 @@ struct FragEval {
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 
+// The hardware's reciprocal, square root and reciprocal square root (about 1 ulp; one instruction each, where
+// x / y and sqrtf expand into correctly rounded sequences of ten or more).  Only for continuous quantities
+// (colours, barycentrics, sub-segment opacity, gradients): no discrete decision may depend on them.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // The 4 sub-segments of a segment of length 4 tSeg between the entry and exit colours (RGB + density):
 // seg[s] = accumulateLinearConst of sub-segment s (ForwardCommon.glsl:42-49, colour endpoints mixed at s/4 and
 // (s+1)/4, density at the midpoint, IntersectRasterization.glsl:228-241), pre[s] = their composite before s,
@@ -138,7 +146,7 @@ __device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSe
         float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!(a < 1e-6f)) {
             const float A = __expf(-a * tSeg);
-            const float ia = __fdividef(1.0f, a);
+            const float ia = fast_rcp(a);
             const float kk = (tSeg + ia) * A - ia;
             cur = make_float4((1.0f - A) * c0x + kk * (c0x - c1x), (1.0f - A) * c0y + kk * (c0y - c1y),
                               (1.0f - A) * c0z + kk * (c0z - c1z), 1.0f - A);
@@ -157,9 +165,8 @@ __device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSe
 __device__ __forceinline__ void bary_ref(float3_ p0, float3_ p1, float3_ p2, float3_ P, float& u, float& v) {
     const float3_ d20 = sub(p2, p0), d21 = sub(p2, p1);
     const float area = fmaxf(length(cross(d20, d21)), 1e-5f);
-    // continuous quantities (no discrete decision depends on them): fast division
-    u = __fdividef(length(cross(d21, sub(P, p1))), area);
-    v = __fdividef(length(cross(sub(P, p0), d20)), area);
+    u = length(cross(d21, sub(P, p1))) / area;
+    v = length(cross(sub(P, p0), d20)) / area;
 }
 
 // Inside test of rayTriangleIntersect (RayIntersectionTests.glsl:45-68) for face g at ray parameter t (the t
@@ -189,10 +196,9 @@ __device__ __forceinline__ bool ref_face_contains_bary(const RecordLds& L, int j
     const float3_ c1 = cross(sub(p2, p1), sub(P, p1));
     const float3_ c2 = cross(sub(p0, p2), sub(P, p2));
     if (dot(N, c0) < 0.0f || dot(N, c1) < 0.0f || dot(N, c2) < 0.0f) return false;
-    // continuous quantities (no discrete decision depends on them): fast division
-    const float area = fmaxf(length(N), 1e-5f);
-    u = __fdividef(length(c1), area);
-    v = __fdividef(length(c2), area);
+    const float inv_area = fminf(fast_rsqrt(dot(N, N)), 1e5f);  // 1 / max(|N|, 1e-5)
+    u = fast_sqrt(dot(c1, c1)) * inv_area;
+    v = fast_sqrt(dot(c2, c2)) * inv_area;
     return true;
 }
 
@@ -330,7 +336,7 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
     const float4 ca = lds_c(L, ia, j), cb = lds_c(L, ib, j), cc = lds_c(L, ic, j);
     const float3_ e1 = sub(b, a), e2 = sub(c, a);
     const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);  // = cross(e1, e2), staged
-    const float inn = __fdividef(1.0f, dot(n, n));
+    const float inn = fast_rcp(dot(n, n));
     const float3_ gb = scale(cross(e2, n), inn), gc = scale(cross(n, e1), inn);
     // G^T dC and dC . (G d)
     const float db[4] = {cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w};
@@ -343,7 +349,7 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
     }
     const float3_ gtdc = add(scale(gb, sb), scale(gc, sc));
     const float dcgd = dot(gtdc, rd);
-    const float s = (sdLdt + dcgd) / dot(n, rd);
+    const float s = fast_div(sdLdt + dcgd, dot(n, rd));
     Q = sub(scale(n, s), gtdc);
 }
 
@@ -373,7 +379,7 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
         float dc0[3] = {0.f, 0.f, 0.f}, dc1[3] = {0.f, 0.f, 0.f}, da = 0.f, dts = 0.f;
         if (!(a < 1e-6f)) {
             const float A = __expf(-a * tSeg);
-            const float ia = __fdividef(1.0f, a);
+            const float ia = fast_rcp(a);
             const float K = (tSeg + ia) * A - ia;
             const float dKda = ia * ia * (1.0f - A) - tSeg * tSeg * A - tSeg * ia * A;
 #pragma unroll
