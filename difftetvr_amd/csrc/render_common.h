@@ -132,24 +132,33 @@ __device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_r
 // The 4 sub-segments of a segment of length 4 tSeg between the entry and exit colours (RGB + density):
 // seg[s] = accumulateLinearConst of sub-segment s (ForwardCommon.glsl:42-49, colour endpoints mixed at s/4 and
 // (s+1)/4, density at the midpoint, IntersectRasterization.glsl:228-241), pre[s] = their composite before s,
-// F = the composite of all four.  The forward and the exact backward both call this, so the backward can
-// recompute seg/pre instead of keeping them in registers from phase A.
-__device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSeg, float att, float4 seg[4],
-                                             float4 pre[4], float4& F) {
+// F = the composite of all four.  The colour at fraction f is formed as c0 + f (c1 - c0).  The forward and the
+// exact backward both call this, so the backward can recompute seg/pre instead of keeping them in registers
+// from phase A; it also takes each sub-segment's density a[s], transmittance A[s] = exp(-a tSeg) and 1 / a
+// (A = 1 and 1 / a = 0 where a < 1e-6: the sub-segment is empty).
+__device__ __forceinline__ void sub_segments_ex(float4 col0, float4 col1, float tSeg, float att, float4 seg[4],
+                                                float4 pre[4], float4& F, float a_s[4], float A_s[4],
+                                                float ia_s[4]) {
+    const float dx = col1.x - col0.x, dy = col1.y - col0.y, dz = col1.z - col0.z, dw = col1.w - col0.w;
     float4 rc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
-        const float c0x = mixf(col0.x, col1.x, fb), c0y = mixf(col0.y, col1.y, fb), c0z = mixf(col0.z, col1.z, fb);
-        const float c1x = mixf(col0.x, col1.x, fe), c1y = mixf(col0.y, col1.y, fe), c1z = mixf(col0.z, col1.z, fe);
-        const float a = mixf(col0.w, col1.w, fm) * att;
+        const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f;
+        const float c0x = col0.x + fb * dx, c0y = col0.y + fb * dy, c0z = col0.z + fb * dz;
+        const float a = (col0.w + fm * dw) * att;
         float4 cur = make_float4(0.f, 0.f, 0.f, 0.f);
+        a_s[s] = a;
+        A_s[s] = 1.0f;
+        ia_s[s] = 0.0f;
         if (!(a < 1e-6f)) {
             const float A = __expf(-a * tSeg);
             const float ia = fast_rcp(a);
-            const float kk = (tSeg + ia) * A - ia;
-            cur = make_float4((1.0f - A) * c0x + kk * (c0x - c1x), (1.0f - A) * c0y + kk * (c0y - c1y),
-                              (1.0f - A) * c0z + kk * (c0z - c1z), 1.0f - A);
+            // accumulateLinearConst: (1 - A) c0 + ((tSeg + 1/a) A - 1/a) (c0 - c1), c0 - c1 = -(c1 - c0) / 4
+            const float kq = -0.25f * ((tSeg + ia) * A - ia);
+            cur = make_float4((1.0f - A) * c0x + kq * dx, (1.0f - A) * c0y + kq * dy, (1.0f - A) * c0z + kq * dz,
+                              1.0f - A);
+            A_s[s] = A;
+            ia_s[s] = ia;
         }
         pre[s] = rc;
         seg[s] = cur;
@@ -159,6 +168,12 @@ __device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSe
         rc.w = rc.w + (1.0f - rc.w) * cur.w;
     }
     F = rc;
+}
+
+__device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSeg, float att, float4 seg[4],
+                                             float4 pre[4], float4& F) {
+    float a_s[4], A_s[4], ia_s[4];
+    sub_segments_ex(col0, col1, tSeg, att, seg, pre, F, a_s, A_s, ia_s);
 }
 
 // Barycentric coordinates with unsigned areas and the 1e-5 area floor (BarycentricInterpolation.glsl:29-38).
@@ -365,7 +380,11 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     float dLdt = 0.f;
     const float tSeg = (E.t1 - E.t0) / 4.0f;
     float4 seg[4], pre[4], Fr;
-    sub_segments(E.col0, E.col1, tSeg, att, seg, pre, Fr);  // as in eval_fragment: not kept live since phase A
+    float a_s[4], A_s[4], ia_s[4];
+    // as in eval_fragment: not kept live since phase A
+    sub_segments_ex(E.col0, E.col1, tSeg, att, seg, pre, Fr, a_s, A_s, ia_s);
+    const float dcx[3] = {E.col1.x - E.col0.x, E.col1.y - E.col0.y, E.col1.z - E.col0.z};
+    const float cb0[3] = {E.col0.x, E.col0.y, E.col0.z};
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
         const float4 sg = seg[s], st = pre[s];
@@ -373,13 +392,13 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
         const float dseg[4] = {omA * dR[0], omA * dR[1], omA * dR[2], omA * dR[3]};
         dR[3] = dR[3] * (1.0f - sg.w) - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
         const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
-        const float c0[3] = {mixf(E.col0.x, E.col1.x, fb), mixf(E.col0.y, E.col1.y, fb), mixf(E.col0.z, E.col1.z, fb)};
-        const float c1[3] = {mixf(E.col0.x, E.col1.x, fe), mixf(E.col0.y, E.col1.y, fe), mixf(E.col0.z, E.col1.z, fe)};
-        const float a = mixf(E.col0.w, E.col1.w, fm) * att;
+        const float c0[3] = {cb0[0] + fb * dcx[0], cb0[1] + fb * dcx[1], cb0[2] + fb * dcx[2]};
+        const float c1[3] = {cb0[0] + fe * dcx[0], cb0[1] + fe * dcx[1], cb0[2] + fe * dcx[2]};
+        const float a = a_s[s];
         float dc0[3] = {0.f, 0.f, 0.f}, dc1[3] = {0.f, 0.f, 0.f}, da = 0.f, dts = 0.f;
         if (!(a < 1e-6f)) {
-            const float A = __expf(-a * tSeg);
-            const float ia = fast_rcp(a);
+            const float A = A_s[s];
+            const float ia = ia_s[s];
             const float K = (tSeg + ia) * A - ia;
             const float dKda = ia * ia * (1.0f - A) - tSeg * tSeg * A - tSeg * ia * A;
 #pragma unroll
