@@ -44,31 +44,9 @@ struct AdjFragLds {
     uint16_t entry[kAdjFrags];  // (pixel lane << 5) | record index, one part of a batch
     float w[32][29];            // gradient rows of half a round (29: conflict-free column access)
     float4 val[kRoundAdj];      // F, then dL/dF; w = kMissMarker when the ray missed the tet
+    uint32_t cnt[kBatch];       // half-round counting sort by record: fragments per record, then
+    uint32_t start[kBatch];     // their first row
 };
-
-// Wave-wide comparisons of 5-bit keys from their bit ballots kb[b] = lanes whose key has bit b (valid lanes
-// only).  key_equal_mask: the valid lanes whose key equals k.  key_rank_start: the number of valid lanes whose
-// key is below k; `count` receives the number equal to k.
-__device__ __forceinline__ uint64_t key_equal_mask(const uint64_t kb[5], uint64_t valid, int k) {
-    uint64_t eq = valid;
-#pragma unroll
-    for (int b = 0; b < 5; b++) eq &= ((k >> b) & 1) ? kb[b] : ~kb[b];
-    return eq;
-}
-__device__ __forceinline__ uint32_t key_rank_start(const uint64_t kb[5], uint64_t valid, int k, uint32_t& count) {
-    uint64_t lt = 0, eq = valid;
-#pragma unroll
-    for (int b = 4; b >= 0; b--) {
-        if ((k >> b) & 1) {
-            lt |= eq & ~kb[b];
-            eq &= kb[b];
-        } else {
-            eq &= ~kb[b];
-        }
-    }
-    count = (uint32_t)popc64(eq);
-    return (uint32_t)popc64(lt);
-}
 
 // Coverage masks -> dense fragment list.  Each lane's fragments are contiguous, ordered by record index
 // (ascending, or descending when `reverse`).  Returns the lane's first entry; `total` = all fragments.
@@ -439,23 +417,23 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                     key = j;
                 }
             }
-            // Per half round (the fragments of lanes 0-31, then 32-63): counting sort by record, row = (fragments
-            // of lower records) + (rank among the same record's fragments in lane order); the owner lanes of
-            // record j (j and j + 32) get its segment [seg0, seg0 + segn) and add its rows.
+            // Per half round (the fragments of lanes 0-31, then 32-63): counting sort by record in LDS, row =
+            // (fragments of lower records) + (the fragment's rank among its record's, from an LDS atomic); the
+            // owner lanes of record j (j and j + 32) get its segment [seg0, seg0 + segn) and add its rows.
 #pragma unroll 1
             for (int h = 0; h < 2; h++) {
                 const int keyh = (lane >> 5) == h ? key : kBatch;
-                uint32_t row = 0, seg0 = 0, segn = 0;
-                {
-                    uint64_t kb[5];
-#pragma unroll
-                    for (int b = 0; b < 5; b++) kb[b] = ballot(keyh < kBatch && ((keyh >> b) & 1));
-                    const uint64_t valid = ballot(keyh < kBatch);
-                    uint32_t own_count;
-                    row = key_rank_start(kb, valid, keyh < kBatch ? keyh : 0, own_count);
-                    row += (uint32_t)popc64(key_equal_mask(kb, valid, keyh < kBatch ? keyh : 0) & lanemask_lt());
-                    seg0 = key_rank_start(kb, valid, lane & 31, segn);
-                }
+                if (lane < kBatch) A.cnt[lane] = 0u;
+                __syncthreads();
+                uint32_t rank = 0;
+                if (keyh < kBatch) rank = atomicAdd(&A.cnt[keyh], 1u);
+                __syncthreads();
+                const uint32_t segn = A.cnt[lane & 31];
+                const uint32_t incl = wave_inclusive_scan(lane < kBatch ? segn : 0u);
+                if (lane < kBatch) A.start[lane] = incl - segn;
+                __syncthreads();
+                const uint32_t seg0 = A.start[lane & 31];
+                const uint32_t row = keyh < kBatch ? A.start[keyh] + rank : 0u;
                 if (keyh < kBatch) {
 #pragma unroll
                     for (int q = 0; q < 28; q++) A.w[row][q] = g[q];

@@ -352,8 +352,8 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
     const float3_ e1 = sub(b, a), e2 = sub(c, a);
     const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);  // = cross(e1, e2), staged
     const float inn = fast_rcp(dot(n, n));
-    const float3_ gb = scale(cross(e2, n), inn), gc = scale(cross(n, e1), inn);
-    // G^T dC and dC . (G d)
+    // G^T dC and dC . (G d), with the in-plane gradients of the barycentrics of b and c
+    // gb = (e2 x n) / |n|^2 and gc = (n x e1) / |n|^2:  gb sb + gc sc = ((sb e2 - sc e1) x n) / |n|^2
     const float db[4] = {cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w};
     const float dcv[4] = {cc.x - ca.x, cc.y - ca.y, cc.z - ca.z, cc.w - ca.w};
     float sb = 0.f, sc = 0.f;
@@ -362,7 +362,7 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
         sb += db[ch] * dC[ch];
         sc += dcv[ch] * dC[ch];
     }
-    const float3_ gtdc = add(scale(gb, sb), scale(gc, sc));
+    const float3_ gtdc = scale(cross(sub(scale(e2, sb), scale(e1, sc)), n), inn);
     const float dcgd = dot(gtdc, rd);
     const float s = fast_div(sdLdt + dcgd, dot(n, rd));
     Q = sub(scale(n, s), gtdc);
