@@ -291,13 +291,9 @@ __device__ __forceinline__ uint32_t cost_bucket(uint32_t c) {
     return 8u * (uint32_t)e + m;
 }
 
-// Bucket of tile i (0 for i >= n), the lanes of the wave with the same bucket (eq, live lanes only) and this
-// lane's rank among them.  Every lane of the wave must take part.
-__device__ __forceinline__ uint32_t tile_bucket_match(const uint2* __restrict__ ranges, const uint32_t* __restrict__ cost,
-                                                      uint32_t i, uint32_t n, uint64_t& eq, uint32_t& rank) {
-    const bool live = i < n;
-    uint32_t b = 0;
-    if (live) b = cost_bucket(cost != nullptr ? cost[i] : ranges[i].y - ranges[i].x);
+// The lanes of the wave whose bucket equals this lane's (eq, live lanes only) and this lane's rank among them.
+// Every lane of the wave must take part.
+__device__ __forceinline__ void bucket_match(bool live, uint32_t b, uint64_t& eq, uint32_t& rank) {
     eq = ballot(live);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -305,7 +301,22 @@ __device__ __forceinline__ uint32_t tile_bucket_match(const uint2* __restrict__ 
         eq &= ((b >> k) & 1u) ? bk : ~bk;
     }
     rank = (uint32_t)popc64(eq & lanemask_lt());
-    return b;
+}
+
+// Tiles per thread per step: their costs are loaded together, so the loads overlap.
+constexpr int kOrderPer = 4;
+
+__device__ __forceinline__ void load_buckets(const uint2* __restrict__ ranges, const uint32_t* __restrict__ cost,
+                                             uint32_t i0, uint32_t n, uint32_t b[kOrderPer]) {
+    uint32_t c[kOrderPer];
+#pragma unroll
+    for (int q = 0; q < kOrderPer; q++) {
+        const uint32_t i = i0 + 1024u * q + threadIdx.x;
+        c[q] = 0u;
+        if (i < n) c[q] = cost != nullptr ? cost[i] : ranges[i].y - ranges[i].x;
+    }
+#pragma unroll
+    for (int q = 0; q < kOrderPer; q++) b[q] = cost_bucket(c[q]);
 }
 
 __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ ranges,
@@ -316,11 +327,17 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ r
     if (t < 256) count[t] = 0u;
     __syncthreads();
     // trip counts are uniform over the workgroup, so every lane of a wave reaches the ballots
-    for (uint32_t i0 = 0; i0 < n; i0 += 1024u) {
-        uint64_t eq;
-        uint32_t rank;
-        const uint32_t b = tile_bucket_match(ranges, cost, i0 + t, n, eq, rank);
-        if (i0 + t < n && rank == 0u) atomicAdd(&count[b], (uint32_t)popc64(eq));
+    for (uint32_t i0 = 0; i0 < n; i0 += 1024u * kOrderPer) {
+        uint32_t b[kOrderPer];
+        load_buckets(ranges, cost, i0, n, b);
+#pragma unroll
+        for (int q = 0; q < kOrderPer; q++) {
+            const bool live = i0 + 1024u * q + t < n;
+            uint64_t eq;
+            uint32_t rank;
+            bucket_match(live, b[q], eq, rank);
+            if (live && rank == 0u) atomicAdd(&count[b[q]], (uint32_t)popc64(eq));
+        }
     }
     __syncthreads();
     if (t < 64) {
@@ -340,16 +357,22 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ r
     }
     __syncthreads();
     const int lane = (int)(t & 63u);
-    for (uint32_t i0 = 0; i0 < n; i0 += 1024u) {
-        uint64_t eq;
-        uint32_t rank;
-        const uint32_t i = i0 + t;
-        const uint32_t b = tile_bucket_match(ranges, cost, i, n, eq, rank);
-        uint32_t base = 0u;
-        if (i < n && rank == 0u) base = atomicAdd(&start[b], (uint32_t)popc64(eq));
-        const int leader = eq != 0ull ? __ffsll((unsigned long long)eq) - 1 : lane;
-        base = __shfl(base, leader, 64);
-        if (i < n) order[base + rank] = i;
+    for (uint32_t i0 = 0; i0 < n; i0 += 1024u * kOrderPer) {
+        uint32_t b[kOrderPer];
+        load_buckets(ranges, cost, i0, n, b);
+#pragma unroll
+        for (int q = 0; q < kOrderPer; q++) {
+            const uint32_t i = i0 + 1024u * q + t;
+            const bool live = i < n;
+            uint64_t eq;
+            uint32_t rank;
+            bucket_match(live, b[q], eq, rank);
+            uint32_t base = 0u;
+            if (live && rank == 0u) base = atomicAdd(&start[b[q]], (uint32_t)popc64(eq));
+            const int leader = eq != 0ull ? __ffsll((unsigned long long)eq) - 1 : lane;
+            base = __shfl(base, leader, 64);
+            if (live) order[base + rank] = i;
+        }
     }
 }
 

@@ -385,41 +385,46 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     sub_segments_ex(E.col0, E.col1, tSeg, att, seg, pre, Fr, a_s, A_s, ia_s);
     const float dcx[3] = {E.col1.x - E.col0.x, E.col1.y - E.col0.y, E.col1.z - E.col0.z};
     const float cb0[3] = {E.col0.x, E.col0.y, E.col0.z};
+    // c0 - c1 of every sub-segment: -(col1 - col0) / 4
+    const float diff[3] = {-0.25f * dcx[0], -0.25f * dcx[1], -0.25f * dcx[2]};
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
         const float4 sg = seg[s], st = pre[s];
-        const float omA = 1.0f - st.w;
-        const float dseg[4] = {omA * dR[0], omA * dR[1], omA * dR[2], omA * dR[3]};
-        dR[3] = dR[3] * (1.0f - sg.w) - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
+        const float omA = 1.0f - st.w;  // dL/d(sub-segment s) = omA dR
         const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
-        const float c0[3] = {cb0[0] + fb * dcx[0], cb0[1] + fb * dcx[1], cb0[2] + fb * dcx[2]};
-        const float c1[3] = {cb0[0] + fe * dcx[0], cb0[1] + fe * dcx[1], cb0[2] + fe * dcx[2]};
+        float sc0 = 0.f, sdf = 0.f;  // sum over RGB of dR c0 and of dR (c0 - c1)
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            sc0 += dR[ch] * (cb0[ch] + fb * dcx[ch]);
+            sdf += dR[ch] * diff[ch];
+        }
+        const float dsw = omA * dR[3];
+        dR[3] = dR[3] * (1.0f - sg.w) - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
         const float a = a_s[s];
-        float dc0[3] = {0.f, 0.f, 0.f}, dc1[3] = {0.f, 0.f, 0.f}, da = 0.f, dts = 0.f;
+        float da = 0.f, dts = 0.f, w0 = 0.f, w1 = 0.f;
         if (!(a < 1e-6f)) {
+            // accumulateLinearConst: out = (1 - A) c0 + K (c0 - c1), alpha 1 - A, A = exp(-a t),
+            // K = (t + 1/a) A - 1/a; dA/dt = -a A, dA/da = -t A, dK/dt = -a t A, dK/da below
             const float A = A_s[s];
             const float ia = ia_s[s];
             const float K = (tSeg + ia) * A - ia;
-            const float dKda = ia * ia * (1.0f - A) - tSeg * tSeg * A - tSeg * ia * A;
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                const float diff = c0[ch] - c1[ch];
-                dts += dseg[ch] * (a * A * c0[ch] - a * tSeg * A * diff);
-                da += dseg[ch] * (tSeg * A * c0[ch] + dKda * diff);
-                dc0[ch] = ((1.0f - A) + K) * dseg[ch];
-                dc1[ch] = -K * dseg[ch];
-            }
-            dts += dseg[3] * a * A;
-            da += dseg[3] * tSeg * A;
+            const float aA = a * A, tA = tSeg * A;
+            const float dKda = ia * ia * (1.0f - A) - tA * (tSeg + ia);
+            dts = aA * (omA * (sc0 - tSeg * sdf) + dsw);
+            da = omA * (tA * sc0 + dKda * sdf) + dsw * tA;
+            // d/dc0 = (1 - A + K), d/dc1 = -K; c0, c1 mix the entry/exit colours at fb, fe
+            const float kc0 = 1.0f - A + K;
+            w0 = omA * ((1.0f - fb) * kc0 - (1.0f - fe) * K);
+            w1 = omA * (fb * kc0 - fe * K);
         }
 #pragma unroll
         for (int ch = 0; ch < 3; ch++) {
-            dcol0[ch] += (1.0f - fb) * dc0[ch] + (1.0f - fe) * dc1[ch];
-            dcol1[ch] += fb * dc0[ch] + fe * dc1[ch];
+            dcol0[ch] += w0 * dR[ch];
+            dcol1[ch] += w1 * dR[ch];
         }
         dcol0[3] += (1.0f - fm) * att * da;
         dcol1[3] += fm * att * da;
-        dLdt += dts / 4.0f;
+        dLdt += 0.25f * dts;
     }
     face_side_q(L, j, E.f0, rd, dcol0, -dLdt, Q0);
     face_side_q(L, j, E.f1, rd, dcol1, dLdt, Q1);
