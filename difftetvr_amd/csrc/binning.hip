@@ -303,8 +303,9 @@ __device__ __forceinline__ void bucket_match(bool live, uint32_t b, uint64_t& eq
     rank = (uint32_t)popc64(eq & lanemask_lt());
 }
 
-// Tiles per thread per step: their costs are loaded together, so the loads overlap.
-constexpr int kOrderPer = 4;
+// Tiles per thread per step: their costs are loaded together, so the loads overlap (one step per pass up to
+// 16K tiles, a 1024x1024 view).
+constexpr int kOrderPer = 16;
 
 __device__ __forceinline__ void load_buckets(const uint2* __restrict__ ranges, const uint32_t* __restrict__ cost,
                                              uint32_t i0, uint32_t n, uint32_t b[kOrderPer]) {
