@@ -1,8 +1,8 @@
 """Tile launch order (dtv_render_params.tile_order): the heaviest-first order must render every tile exactly once.
 Images and fragment counts equal the row-major launch's bit for bit (a tile left out would keep the NaN
 written into the image storage beforehand); the adjoint equals the row-major one within FP32 atomic reordering
-(relative L2 <= 1e-5).  Viewports span 1 tile, a partial last row/column, and more than 1024 tiles (several
-passes of the ordering workgroup)."""
+(relative L2 <= 1e-5).  Viewports span 1 tile, a partial last row/column, more than 1024 tiles, and more than
+16384 tiles (more than one load step per pass of the ordering workgroup)."""
 import numpy as np
 import pytest
 import torch
@@ -24,7 +24,7 @@ def _render_into(d, r, img):
     return img
 
 
-@pytest.mark.parametrize("W,H,use_ert", [(400, 300, True), (61, 45, False), (8, 8, False)])
+@pytest.mark.parametrize("W,H,use_ert", [(400, 300, True), (61, 45, False), (8, 8, False), (1100, 1100, True)])
 def test_cost_order_equals_row_major(gpu, W, H, use_ert):
     import difftetvr_amd as d
     w = grid_case(n=14, seed=7)
