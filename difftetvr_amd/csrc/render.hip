@@ -361,9 +361,9 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                     if (jck >= 0 && ck_n == n - 1) {
                         Cin = ck_c;
                     } else {
-                        // IEEE division: the inversion's rounding error compounds along the chain back to the
-                        // next checkpoint (measured: the hardware reciprocal doubles the adjoint's error)
-                        const float ain = (f.w - C.w) / (f.w - 1.0f);
+                        // the inversion's rounding error compounds along the chain back to the next checkpoint:
+                        // a corrected quotient (the bare hardware reciprocal doubles the adjoint's error)
+                        const float ain = div_nr(f.w - C.w, f.w - 1.0f);
                         Cin = make_float4(C.x - (1.0f - ain) * f.x, C.y - (1.0f - ain) * f.y,
                                           C.z - (1.0f - ain) * f.z, ain);
                     }

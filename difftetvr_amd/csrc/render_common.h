@@ -128,6 +128,13 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ float fast_div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+// x / y from the hardware reciprocal and one Newton correction of the quotient (residual by FMA): within about
+// half an ulp, for a chain of divisions whose errors compound.  Still not for discrete decisions.
+__device__ __forceinline__ float div_nr(float x, float y) {
+    const float r = __builtin_amdgcn_rcpf(y);
+    const float q = x * r;
+    return __builtin_fmaf(__builtin_fmaf(-q, y, x), r, q);
+}
 
 // The 4 sub-segments of a segment of length 4 tSeg between the entry and exit colours (RGB + density):
 // seg[s] = accumulateLinearConst of sub-segment s (ForwardCommon.glsl:42-49, colour endpoints mixed at s/4 and
