@@ -22,6 +22,9 @@ if ROOT not in sys.path:
 
 METRIC = "fwd+adjoint+Adam iters/s, 1.57M tets @1024²; serve p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+# VALU issue: one wave64 VALU instruction per SIMD every 4 cycles (packed FP32 off), 256 CUs x 4 SIMDs at the
+# 2.4 GHz peak engine clock
+VALU_PEAK_INSTR_S = 256 * 4 * 2.4e9 / 4
 
 
 def algorithmic_bytes(stage, T, V, P):
@@ -42,6 +45,22 @@ def load_traffic(kernel, workload="grid64_1024"):
         return None
     with open(path) as f:
         return json.load(f).get("bytes_per_launch")
+
+
+def valu_roofline(kernel, avg_ms, workload="grid64_1024"):
+    """VALU issue rate of the kernel: its wave64 VALU instructions per launch (SQ_INSTS_VALU from a rocprofv3
+    --pmc run, kept in profiles/) over the launch duration measured here, against VALU_PEAK_INSTR_S."""
+    path = os.path.join(ROOT, "profiles", f"valu_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        instr = json.load(f).get("valu_instructions_per_launch")
+    if not instr:
+        return None
+    achieved = instr / (avg_ms * 1e-3)
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": VALU_PEAK_INSTR_S,
+            "unit": "wave64 VALU instr/s", "frac": achieved / VALU_PEAK_INSTR_S, "instructions": instr,
+            "avg_ms": avg_ms}
 
 
 def cpu_baseline(w, view, fovy, W, H, row_step):
@@ -616,6 +635,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(f"k_render_{dominant}"),
                      "algorithmic_bytes": abytes, "avg_ms": stage_ms[dominant]},
+        # the render kernels are bound by VALU issue, not HBM: the fraction that measures them
+        "valu_roofline": valu_roofline(f"k_render_{dominant}", stage_ms[dominant]),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, view, fovy, W, H, args.cpu_row_step)
