@@ -211,6 +211,7 @@ def bench_multiview(args):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(f"k_render_{dominant}", f"multiview{nv}_grid{args.grid}_{W}"),
                      "algorithmic_bytes": abytes, "avg_ms": acc[dominant]},
+        "valu_roofline": valu_roofline(f"k_render_{dominant}", acc[dominant], f"multiview{nv}_grid{args.grid}_{W}"),
     }
     print(json.dumps(out), flush=True)
 
