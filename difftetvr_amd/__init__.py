@@ -620,6 +620,7 @@ class BatchRenderer(Renderer):
                 pos.grad = torch.zeros_like(pos)
             if col.grad is None:
                 col.grad = torch.zeros_like(col)
+            arr[i].mesh = mesh._desc()
             arr[i].dimage = d.data_ptr()
             arr[i].grad_positions = pos.grad.data_ptr()
             arr[i].grad_colors = col.grad.data_ptr()

@@ -47,7 +47,7 @@ class DtvRenderItem(ctypes.Structure):
 
 
 class DtvAdjointItem(ctypes.Structure):
-    _fields_ = [("dimage", c_void_p), ("grad_positions", c_void_p), ("grad_colors", c_void_p)]
+    _fields_ = [("mesh", DtvMesh), ("dimage", c_void_p), ("grad_positions", c_void_p), ("grad_colors", c_void_p)]
 
 
 class DtvServerRequest(ctypes.Structure):

@@ -66,7 +66,7 @@ struct DevBuf {
     }
 };
 
-constexpr int kMaxCheckpoints = 24;
+constexpr int kMaxCheckpoints = 32;
 
 // Camera uniforms: perspective RH, depth in [0,1], y flipped; built in double, rounded once to float.
 // Column-major 4x4 (m[c*4 + r]).
@@ -148,6 +148,8 @@ struct dtv_view {
     std::vector<int64_t> item_V;
     BatchItem* pinnedItems = nullptr;
     int pinned_items_cap = 0;
+    hipEvent_t items_copied = nullptr;     // the last upload out of pinnedItems
+    bool items_pending = false;
     AdjointTarget* pinnedTargets = nullptr;
     int pinned_targets_cap = 0;
     hipEvent_t targets_copied = nullptr;   // the last upload out of pinnedTargets
@@ -274,6 +276,15 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     }
     DTV_HIP(hipSetDevice(view->ctx->device));
     view->rendered = false;
+    // The items reach the device through pinned staging.  Under stream capture the upload becomes a graph node
+    // that reads the staging at every replay; otherwise the previous render's upload must have left the staging
+    // before it is overwritten (in planned mode nothing else waits for the device between renders).
+    if (!view->items_copied) DTV_HIP(hipEventCreateWithFlags(&view->items_copied, hipEventDisableTiming));
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    DTV_HIP(hipStreamIsCapturing(s, &capture));
+    const bool capturing = capture != hipStreamCaptureStatusNone;
+    if (view->items_pending && !capturing) DTV_HIP(hipEventSynchronize(view->items_copied));
+    view->items_pending = false;
     if (view->pinned_items_cap < n) {
         if (view->pinnedItems) DTV_HIP(hipHostFree(view->pinnedItems));
         view->pinnedItems = nullptr;
@@ -285,6 +296,10 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->dItems.ensure(sizeof(BatchItem) * (size_t)n));
     DTV_HIP(hipMemcpyAsync(view->dItems.p, view->pinnedItems, sizeof(BatchItem) * (size_t)n, hipMemcpyHostToDevice,
                            s));
+    if (!capturing) {
+        DTV_HIP(hipEventRecord(view->items_copied, s));
+        view->items_pending = true;
+    }
     const BatchItem* dI = view->dItems.as<BatchItem>();
 
     // 1. count (tile, face) records per tet, 2. scan
@@ -452,6 +467,14 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     return DTV_OK;
 }
 
+// The adjoint reads the cells, positions and colours the render recorded and scatters into gradients sized by
+// the vertex count: the caller's mesh must be that one, unchanged in its buffers and sizes.
+bool same_mesh(const dtv_view* view, int i, const dtv_mesh& m) {
+    const BatchItem& B = view->items[(size_t)i];
+    return m.cells == B.cells && m.positions == B.pos && m.colors == B.col && m.num_cells == B.T &&
+           m.num_vertices == view->item_V[(size_t)i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -504,12 +527,13 @@ int dtv_view_destroy(dtv_view* v) {
     DevBuf* bufs[] = {&v->dItems, &v->dTargets, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
                       &v->vals, &v->keys2,
                       &v->vals2, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
-                      &v->num_ckpt, &v->ckpt_color, &v->ckpt_n};
+                      &v->num_ckpt, &v->ckpt_color, &v->ckpt_n, &v->tile_frags, &v->tile_order};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < DTV_NUM_STAGES; i++)
         for (int k = 0; k < 2; k++)
             if (v->ev[i][k]) (void)hipEventDestroy(v->ev[i][k]);
     if (v->targets_copied) (void)hipEventDestroy(v->targets_copied);
+    if (v->items_copied) (void)hipEventDestroy(v->items_copied);
     if (v->pinnedItems) (void)hipHostFree(v->pinnedItems);
     if (v->pinnedTargets) (void)hipHostFree(v->pinnedTargets);
     if (v->pinnedStats) (void)hipHostFree(v->pinnedStats);
@@ -546,8 +570,7 @@ int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage
     if (!view || !mesh || !dimage || !grad_positions || !grad_colors)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: NULL argument");
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_render_adjoint: view has not been rendered");
-    if (view->nitems() != 1 || mesh->cells != view->items[0].cells || mesh->num_cells != view->items[0].T ||
-        mesh->num_vertices != view->item_V[0])
+    if (view->nitems() != 1 || !same_mesh(view, 0, *mesh))
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: mesh differs from the rendered one");
     AdjointTarget t;
     t.dimage = dimage;
@@ -563,8 +586,13 @@ int dtv_render_adjoint_batch(dtv_view* view, const dtv_adjoint_item* items, int3
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_render_adjoint_batch: view has not been rendered");
     if (num_items <= 0 || num_items > kMaxBatchItems)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint_batch: item count differs from the render");
+    if (num_items != view->nitems())
+        return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint_batch: item count differs from the render");
     std::vector<AdjointTarget> tg((size_t)num_items);
     for (int i = 0; i < num_items; i++) {
+        if (items[i].dimage && !same_mesh(view, i, items[i].mesh))
+            return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint_batch: item " + std::to_string(i) +
+                                                      ": mesh differs from the rendered one");
         tg[(size_t)i].dimage = items[i].dimage;
         tg[(size_t)i].grad_pos = items[i].grad_positions;
         tg[(size_t)i].grad_col = items[i].grad_colors;

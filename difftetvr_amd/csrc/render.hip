@@ -16,8 +16,8 @@
 //   early ray termination              PPLL/Forward.glsl:168-173
 // The adjoint re-traverses each pixel back to front from the stored final colour, inverting the 'over'
 // operator per tet (the inversion of AdjointIntersectRasterization.glsl:213-228) and resetting to stored
-// checkpoints (every K fragments, before any tet of opacity > 7/8, and whenever transmittance fell by 8x) to
-// bound inversion drift.  Gradients of the 4 tet vertices are reduced per record in LDS before one global
+// checkpoints (every K fragments, and before any fragment that takes the transmittance below 1/8 of the last
+// checkpoint's) to bound inversion drift.  Gradients of the 4 tet vertices are reduced per record in LDS before one global
 // atomic per value.
 #include "render_common.h"
 
@@ -27,6 +27,7 @@ constexpr int kRound = 128;                 // fragments shaded per round (forwa
 constexpr int kRoundAdj = 64;               // fragments per round in the adjoint: one per lane
 constexpr int kMaxFrags = kBatch * 64;      // fragments a batch can produce
 constexpr float kMissMarker = -1.0f;        // alpha of a fragment whose ray missed the tet
+constexpr int kCkptReserve = 10;            // checkpoint slots periodic checkpoints leave to the transmittance rule
 
 // The forward's fragment list and round values.
 struct FragListLds {
@@ -149,12 +150,21 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
                 }
                 const float4 f = F.val[e - r0];
                 if (f.w != kMissMarker) {
-                    if ((1.0f - f.w) < 0.125f && nlast != n && (int)nck < U.max_ckpt) {
-                        ps.ckpt_color[(int64_t)nck * P + pix] = C;
-                        ps.ckpt_n[(int64_t)nck * P + pix] = n;
-                        nck++;
-                        nlast = n;
-                        Tlast = 1.0f - C.w;
+                    // Checkpoint rules (the adjoint recovers each fragment's input state by inverting 'over'
+                    // from the next checkpoint; an inversion through a drop in transmittance loses precision):
+                    // a checkpoint goes before any fragment that takes the transmittance below 1/8 of the last
+                    // checkpoint's (opaque tets included, so the state before them is never inverted), and none
+                    // is stored once alpha is exactly 1 (later fragments leave the composite unchanged).  The
+                    // state before the first fragment (zero) is known without a slot.
+                    const float Tafter = (1.0f - C.w) * (1.0f - f.w);
+                    if (C.w < 1.0f && Tafter < Tlast * 0.125f) {
+                        if (nlast != n && (int)nck < U.max_ckpt) {
+                            ps.ckpt_color[(int64_t)nck * P + pix] = C;
+                            ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                            nck++;
+                            nlast = n;
+                        }
+                        Tlast = Tafter;
                     }
                     C.x = C.x + (1.0f - C.w) * f.x;
                     C.y = C.y + (1.0f - C.w) * f.y;
@@ -163,8 +173,9 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
                 }
                 n++;
                 last = base + (F.entry[e] & 31) + 1;
-                if (((int)(n - nlast) >= U.ckpt_interval || (1.0f - C.w) < Tlast * 0.125f) &&
-                    (int)nck < U.max_ckpt) {
+                // periodic checkpoints bound the inversion chains; the last kCkptReserve slots are kept for the
+                // transmittance rule above, which needs at most 9 before alpha reaches 1 in float
+                if ((int)(n - nlast) >= U.ckpt_interval && C.w < 1.0f && (int)nck < U.max_ckpt - kCkptReserve) {
                     ps.ckpt_color[(int64_t)nck * P + pix] = C;
                     ps.ckpt_n[(int64_t)nck * P + pix] = n;
                     nck++;
@@ -360,10 +371,15 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                     float4 Cin;
                     if (jck >= 0 && ck_n == n - 1) {
                         Cin = ck_c;
+                    } else if (n == 1) {
+                        Cin = make_float4(0.f, 0.f, 0.f, 0.f);  // the first fragment starts from zero
                     } else {
                         // the inversion's rounding error compounds along the chain back to the next checkpoint:
-                        // a corrected quotient (the bare hardware reciprocal doubles the adjoint's error)
-                        const float ain = div_nr(f.w - C.w, f.w - 1.0f);
+                        // a corrected quotient (the bare hardware reciprocal doubles the adjoint's error).  With
+                        // no checkpoint before it, a fragment of alpha 1 can only follow a saturated composite
+                        // (the forward's checkpoint rule), whose state the fragment did not change.
+                        const float den = f.w - 1.0f;
+                        const float ain = den == 0.0f ? C.w : div_nr(f.w - C.w, den);
                         Cin = make_float4(C.x - (1.0f - ain) * f.x, C.y - (1.0f - ain) * f.y,
                                           C.z - (1.0f - ain) * f.z, ain);
                     }

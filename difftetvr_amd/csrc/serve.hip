@@ -162,6 +162,7 @@ void worker_main(dtv_server* S) {
                 if (!q.target_image) continue;
                 const int64_t n = 4LL * q.camera.width * q.camera.height;
                 rc = dtv_mse_loss_grad(q.out_image, q.target_image, n, scratch + off, nullptr, stream);
+                adj[i].mesh = items[i].mesh;
                 adj[i].dimage = scratch + off;
                 adj[i].grad_positions = q.grad_positions;
                 adj[i].grad_colors = q.grad_colors;

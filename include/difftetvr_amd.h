@@ -87,8 +87,10 @@ typedef struct dtv_render_item {
 } dtv_render_item;
 
 /* Adjoint input and gradient targets of one batch item: dimage [height, width, 4]; gradients accumulate (+=) into
- * grad_positions [V,3] and grad_colors [V,4] (items that share a mesh may share them). */
+ * grad_positions [V,3] and grad_colors [V,4] (items that share a mesh may share them).  mesh must equal the
+ * item's mesh as rendered (same buffers and sizes; checked). */
 typedef struct dtv_adjoint_item {
+    dtv_mesh mesh;
     const float* dimage;
     float* grad_positions;
     float* grad_colors;
@@ -141,13 +143,15 @@ int dtv_render_batch(dtv_view* view, const dtv_render_item* items, int32_t num_i
                      const dtv_render_params* params, void* stream);
 
 /* Adjoint of the last dtv_render_batch of this view, items[i] belonging to its render item i (same count).
- * One launch over all tiles of the batch; an item with dimage == NULL is not differentiated. */
+ * One launch over all tiles of the batch; an item with dimage == NULL is not differentiated.  An item whose mesh
+ * differs from the one rendered (buffers or sizes) fails with DTV_ERR_INVALID_ARGUMENT before any work. */
 int dtv_render_adjoint_batch(dtv_view* view, const dtv_adjoint_item* items, int32_t num_items, int32_t use_abs_grad,
                              int32_t adjoint_mode, void* stream);
 
 /* Adjoint of the last dtv_render of this view: accumulates (+=) into grad_positions [V,3] and
  * grad_colors [V,4].  Replaces Renderer.render_adjoint (Module.cpp:922-959) ->
- * TetMeshRendererIntersection::renderAdjoint (:709-715).  The mesh must be the one rendered. */
+ * TetMeshRendererIntersection::renderAdjoint (:709-715).  The mesh must be the one rendered (same cells,
+ * positions and colours buffers and sizes; else DTV_ERR_INVALID_ARGUMENT). */
 int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage, int32_t use_abs_grad,
                        int32_t adjoint_mode, float* grad_positions, float* grad_colors, void* stream);
 

@@ -27,8 +27,9 @@
  *                        lists do not change)
  *   reference adjoint    on rays through one tet: FP64 central differences of the forward with sub-segment 0
  *                        frozen (colours, density) and of the two frozen-geometry forwards whose derivatives it
- *                        adds (positions) (tests/test_reference_adjoint_pinning.py); the chaining of state
- *                        between fragments (sub-segment 0 never inverted) is parity unpinned
+ *                        adds (positions); on rays through two to four tets, the chaining of state between
+ *                        fragments (sub-segment 0 never inverted) by FP64 central differences of the closed form
+ *                        it gives each tet's gradient (tests/test_reference_adjoint_pinning.py)
  *   Adam                 golden vectors written by torch.optim.Adam (tests/golden/adam_golden.npz)
  */
 #include <math.h>

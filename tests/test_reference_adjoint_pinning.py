@@ -12,8 +12,9 @@ of a well-defined function and FP64 central differences can check it:
 
 The single-fragment forward below restates the oracle's frag_eval in numpy (checked against oracle.forward in
 FP64 first).  What single fragments leave open is the state chaining between fragments (the reference inverts
-sub-segments 3..1 only, so the state it passes to the fragment in front still holds that fragment's sub-segment
-0): that is not the derivative of any function and stays restated from the source."""
+sub-segments 3..1 only, so the state it passes to the fragment in front still holds the sub-segment 0 of every
+fragment behind it): the second test pins it on rays through two to four tets, through the closed form that
+chaining gives each tet's gradient (see the comment above it)."""
 import numpy as np
 
 import oracle
@@ -165,3 +166,192 @@ def test_reference_adjoint_single_fragment_rays():
     # and the parts are really what differ from the exact adjoint
     ex_pos, ex_col = oracle.adjoint_exact(fr, pos32, col32, uf, att, dimg)
     assert _rel(gcol, ex_col) > 1e-2
+
+
+# ---- state chaining between fragments, on rays through two to four tets ------------------------------------
+#
+# Along one ray, with tets k = 0..n-1 front to back and their sub-segments s_kj (j = 0..3, premultiplied RGBA),
+# the reference starts from the final composite and, per tet from the back, inverts sub-segments 3, 2, 1 only.
+# Inverting 'over' is exact algebra, so the state it hands to tet k is the true composite of tets 0..k followed by
+# sub-segment 0 of every tet behind k; and the image adjoint it carries there has passed back through sub-segments
+# 1..3 of the tets behind only.  For tet k's sub-segments 1..3 this makes the reference's gradient the derivative
+# of a closed-form function of tet k's parameters:
+#     dL . ( Y_k  over  s_k1 over s_k2 over s_k3  over  Q_k ),
+#     Y_k frozen, of transmittance  T_k (1 - a(s_k0)) prod_{i > k} (1 - a(s_i0))   (its RGB does not enter),
+#     Q_k frozen = s_{k+1,1} over s_{k+1,2} over s_{k+1,3} over s_{k+2,1} over ... (sub-segments 0 left out),
+# with T_k the true transmittance in front of tet k.  Summed over tets and rays, FP64 central differences of it
+# pin the chaining; the same sum with the damping prod (1 - a(s_i0)) left out, or with the s_i0 kept in Q_k
+# (a reference that inverted every sub-segment), must not match.
+
+def _over(a, b):
+    return a + (1.0 - a[3]) * b
+
+
+def _subsegs(tseg, col0, col1, att):
+    out = []
+    for s in range(4):
+        fb, fm, fe = s * 0.25, (s + 0.5) * 0.25, (s + 1.0) * 0.25
+        c0 = col0[:3] * (1 - fb) + col1[:3] * fb
+        c1 = col0[:3] * (1 - fe) + col1[:3] * fe
+        al = col0[3] * (1 - fm) + col1[3] * fm
+        out.append(oracle.accumulate_linear_const(tseg, c0, c1, al * att))
+    return out
+
+
+class TetOnRay:
+    """One tet crossed by a ray: faces, hit points and barycentrics at the original geometry, and its
+    sub-segments as functions of the mesh's colours (geometry frozen) or positions (the two frozen-geometry
+    derivatives the reference adds, as in Ray above)."""
+
+    def __init__(self, ro, rd, vid, P, C, att):
+        self.ro, self.rd, self.vid, self.att = ro, rd, list(vid), att
+        Pt = P[self.vid]
+        self.hit, self.f0, self.f1, self.t0, self.t1 = oracle.ray_tet(ro, rd, Pt)
+        if not self.hit:
+            return
+        self.faces = [[self.vid[i] for i in FT[self.f0]], [self.vid[i] for i in FT[self.f1]]]
+        self.H = [ro + self.t0 * rd, ro + self.t1 * rd]
+        self.P0 = P.copy()
+        self.C0 = C.copy()
+        self.lam = [self._lam(P, self.faces[k], self.H[k]) for k in range(2)]
+        self.seg = self.segs_colours(C)
+
+    @staticmethod
+    def _lam(P, face, Hk):
+        u, v = _bary(P[face[0]], P[face[1]], P[face[2]], Hk)
+        return np.array([u, v, 1.0 - u - v])
+
+    def segs_colours(self, C):
+        cols = [self.lam[k] @ C[self.faces[k]] for k in range(2)]
+        return _subsegs((self.t1 - self.t0) / 4.0, cols[0], cols[1], self.att)
+
+    def segs_positions_i(self, P):
+        pts = [self.lam[k] @ P[self.faces[k]] for k in range(2)]
+        cols = [self.lam[k] @ self.C0[self.faces[k]] for k in range(2)]
+        return _subsegs(np.linalg.norm(pts[0] - pts[1]) / 4.0, cols[0], cols[1], self.att)
+
+    def segs_positions_ii(self, P):
+        lam = [self._lam(P, self.faces[k], self.H[k]) for k in range(2)]
+        pts = [lam[k] @ self.P0[self.faces[k]] for k in range(2)]
+        cols = [lam[k] @ self.C0[self.faces[k]] for k in range(2)]
+        return _subsegs(np.linalg.norm(pts[0] - pts[1]) / 4.0, cols[0], cols[1], self.att)
+
+    def stable(self, P):
+        """Hit faces unchanged a tiny turn of the ray away, hit points away from the face edges (where the
+        unsigned-area barycentrics have their kink)."""
+        for d in (np.array([1e-4, 0, 0]), np.array([0, 1e-4, 0]), np.array([0, 0, 1e-4])):
+            if oracle.ray_tet(self.ro, self.rd + d, P[self.vid])[1:3] != (self.f0, self.f1):
+                return False
+        return min(self.lam[0].min(), self.lam[1].min()) >= 0.02
+
+
+class ChainRay:
+    """The frozen parts of the closed form for every tet of one ray."""
+
+    def __init__(self, tets, dl, damp=True, q_with_s0=False):
+        self.tets, self.dl = tets, dl
+        n = len(tets)
+        self.Y, self.Q = [], []
+        T = 1.0
+        for k, tk in enumerate(tets):
+            d = np.prod([1.0 - tets[i].seg[0][3] for i in range(k + 1, n)]) if damp else 1.0
+            self.Y.append(np.array([0.0, 0.0, 0.0, 1.0 - T * (1.0 - tk.seg[0][3]) * d]))
+            q = np.zeros(4)
+            for i in range(k + 1, n):
+                for j in range(0 if q_with_s0 else 1, 4):
+                    q = _over(q, tets[i].seg[j])
+            self.Q.append(q)
+            T *= np.prod([1.0 - tk.seg[j][3] for j in range(4)])
+
+    def terms(self, segs_of):
+        """(vertex ids, function of the parameters) of each tet's term: the FD of a vertex needs only the terms
+        of the tets that hold it."""
+        out = []
+        for k, tk in enumerate(self.tets):
+            def term(x, k=k, tk=tk):
+                s = segs_of(tk, x)
+                z = _over(_over(_over(self.Y[k], s[1]), s[2]), s[3])
+                return float(self.dl @ _over(z, self.Q[k]))
+            out.append((set(tk.vid), term))
+        return out
+
+
+def _fd_terms(terms, x0, h):
+    """Central differences of the sum of the terms, each differenced only in the vertices it holds."""
+    g = np.zeros_like(x0)
+    for v in range(x0.shape[0]):
+        mine = [fn for vids, fn in terms if v in vids]
+        for c in range(x0.shape[1]):
+            xp, xm = x0.copy(), x0.copy()
+            xp[v, c] += h
+            xm[v, c] -= h
+            g[v, c] = sum(fn(xp) - fn(xm) for fn in mine) / (2 * h)
+    return g
+
+
+def test_reference_adjoint_chaining_multi_tet_rays():
+    w = meshgen.grid_workload(2, seed=5, jitter=0.2)
+    pos32 = (w["pos"] * np.float32(0.6)).astype(np.float32)
+    cells = w["cells"]
+    rng = np.random.default_rng(8)
+    col32 = np.concatenate([rng.uniform(0.1, 0.9, size=(pos32.shape[0], 3)),
+                            rng.uniform(0.5, 3.0, size=(pos32.shape[0], 1))], axis=1).astype(np.float32)
+    W = H = 40
+    eye = (0.35, 0.45, 0.75)
+    att = 1.0
+    uf, _ = oracle.camera_uniforms(view_from(eye), FOVY, ZNEAR, ZFAR, W, H)
+    fr = oracle.Fragments(cells, pos32, uf, ZNEAR, W, H)
+    img, _, cnt = oracle.forward(fr, pos32, col32, uf, att)
+    off, ids = fr.lists()
+    P = pos32.astype(np.float64)
+    C = col32.astype(np.float64)
+    ro = uf[64:67].astype(np.float64)
+    dimg = np.zeros((H, W, 4), np.float32)
+    rays = []
+    nfrag = {2: 0, 3: 0, 4: 0}
+    for y in range(H):
+        for x in range(W):
+            n = int(cnt[y, x])
+            if n < 2 or n > 4 or (x + y) % 3:
+                continue
+            p = y * W + x
+            rd = oracle.ray_dir(uf, W, H, x, y).astype(np.float64)
+            tets = [TetOnRay(ro, rd, cells[i >> 2], P, C, att) for i in ids[off[p]:off[p + 1]]]
+            if not all(t.hit and t.stable(P) for t in tets):
+                continue
+            dimg[y, x] = rng.normal(size=4)
+            rays.append((tets, dimg[y, x].astype(np.float64)))
+            nfrag[n] += 1
+    print(f"chained rays: {len(rays)} ({nfrag} by fragment count)", flush=True)
+    assert len(rays) >= 40 and nfrag[3] + nfrag[4] >= 20
+    # the restated sub-segments composite to the oracle's FP64 image on these rays
+    img64, _, _ = oracle.forward(fr, P, C, uf.astype(np.float64), att, dtype=np.float64)
+    for y in range(H):
+        for x in range(W):
+            if dimg[y, x].any():
+                p = y * W + x
+                rd = oracle.ray_dir(uf, W, H, x, y).astype(np.float64)
+                acc = np.zeros(4)
+                for i in ids[off[p]:off[p + 1]]:
+                    for s in TetOnRay(ro, rd, cells[i >> 2], P, C, att).seg:
+                        acc = _over(acc, s)
+                assert np.abs(acc - img64[y, x]).max() < 1e-6
+    gpos, gcol = oracle.adjoint_reference(fr, pos32, col32, uf, att, img, dimg)
+
+    def fd_colours(chains):
+        return _fd_terms([t for ch in chains for t in ch.terms(lambda tk, x: tk.segs_colours(x))], C, 1e-6)
+
+    chains = [ChainRay(t, d) for t, d in rays]
+    fd_col = fd_colours(chains)
+    # positions: the sum of the two frozen-geometry parts' derivatives
+    fd_pos = _fd_terms([t for ch in chains for t in ch.terms(lambda tk, x: tk.segs_positions_i(x))] +
+                       [t for ch in chains for t in ch.terms(lambda tk, x: tk.segs_positions_ii(x))], P, 1e-7)
+    ec, ep = _rel(gcol, fd_col), _rel(gpos, fd_pos)
+    print(f"reference adjoint vs the chained closed form: colours rel {ec:.2e}, positions rel {ep:.2e}", flush=True)
+    assert ec <= 1e-5
+    assert ep <= 1e-4
+    # what the closed form rules out: no damping by the sub-segments 0 behind, or every sub-segment inverted
+    for kw in (dict(damp=False), dict(damp=True, q_with_s0=True)):
+        alt = _rel(gcol, fd_colours([ChainRay(t, d, **kw) for t, d in rays]))
+        print(f"  variant {kw}: colours rel {alt:.2e}", flush=True)
+        assert alt > 100 * 1e-5
