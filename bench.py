@@ -95,7 +95,7 @@ def cpu_baseline(w, view, fovy, W, H, row_step):
                       f"{t3 - t0:.2f} s measured"}
 
 
-def bench_multiview(args):
+def bench_multiview(args, emit=True):
     """Batched multi-view step: the mesh seen from `views` cameras on a Fibonacci sphere of radius 2 (512 x 512
     each) rendered, differentiated (loss = sum of the per-view MSEs) and optimised in one batched step, against
     the plain single-view step (fwd + adjoint + Adam of one view) run for every view."""
@@ -213,7 +213,9 @@ def bench_multiview(args):
                      "algorithmic_bytes": abytes, "avg_ms": acc[dominant]},
         "valu_roofline": valu_roofline(f"k_render_{dominant}", acc[dominant], f"multiview{nv}_grid{args.grid}_{W}"),
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def _delaunay_job(args):
@@ -329,12 +331,24 @@ def bench_uneven(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args):
+SERVE_GRID_NS = [int(n) for n in __import__("numpy").linspace(16, 80, 16).round()]
+SERVE_DELAUNAY_NS = [int(n) for n in __import__("numpy").geomspace(10_000, 500_000, 16).round()]
+
+
+def start_serve_meshes(procs=8):
+    """Starts the 16 Delaunay tetrahedralisations of the serving mix in child processes (before this process
+    touches the GPU) and returns (pool, async result)."""
+    import multiprocessing
+    pool = multiprocessing.get_context("spawn").Pool(min(procs, os.cpu_count() or 1))
+    return pool, pool.map_async(_delaunay_job, [(n, 100 + i) for i, n in enumerate(SERVE_DELAUNAY_NS)])
+
+
+def bench_serve(args, qps_list=None, pending=None, emit=True):
     """Serving: a Poisson stream of render requests at a fixed rate over 32 resident meshes (16 cube grids of
     16^3..80^3 cells x 6 tets, 16 Delaunay meshes of 10k..500k points in the unit ball), each request a seeded
     random camera on a sphere of radius 1.5..3 with a viewport drawn from {256^2, 512^2, 1024^2}; 90% forward
-    renders, 10% forward + MSE loss + adjoint.  Reports p50/p99 latency (submit to completion) and frames/s."""
-    import multiprocessing
+    renders, 10% forward + MSE loss + adjoint.  Reports p50/p99 latency (submit to completion) and frames/s, one
+    JSON object per offered rate in qps_list (default: args.qps)."""
     import threading
 
     import numpy as np
@@ -344,14 +358,15 @@ def bench_serve(args):
     from difftetvr_amd import meshgen
     from difftetvr_amd.serve import Server
 
-    rng = np.random.default_rng(0)
-    grid_ns = np.linspace(16, 80, 16).round().astype(int)
-    del_ns = np.geomspace(10_000, 500_000, 16).round().astype(int)
     t0 = time.perf_counter()
-    with multiprocessing.get_context("spawn").Pool(min(8, os.cpu_count() or 1)) as pool:
-        dels = pool.map(_delaunay_job, [(int(n), 100 + i) for i, n in enumerate(del_ns)])
+    if pending is None:
+        pending = start_serve_meshes()
+    pool, res = pending
+    dels = res.get()
+    pool.close()
+    pool.join()
     meshes, tets = [], []
-    for i, n in enumerate(grid_ns):
+    for i, n in enumerate(SERVE_GRID_NS):
         w = meshgen.grid_workload(int(n), seed=i)
         m = d.TetMesh()
         m.set_use_gradients(True)
@@ -367,97 +382,104 @@ def bench_serve(args):
         meshes.append(m)
         tets.append(int(cells.shape[0]))
     t_setup = time.perf_counter() - t0
-    # early ray termination on, the reference renderer's default (TetMeshVolumeRenderer.hpp:243-244)
-    srv = Server(num_workers=args.workers, max_batch=args.max_batch, memory_budget=16 << 30, attenuation=1.0,
-                 fovy=math.pi / 4.0, use_early_ray_termination=True)
-    ids = [srv.add_mesh(m) for m in meshes]
     sizes = [256, 512, 1024]
-    nreq = int(args.qps * args.duration)
-    plan = []
-    t = 0.0
-    for _ in range(nreq):
-        t += rng.exponential(1.0 / args.qps)
-        mi = int(rng.integers(len(meshes)))
-        e = rng.normal(size=3)
-        e = e / np.linalg.norm(e) * rng.uniform(1.5, 3.0)
-        res = sizes[int(rng.integers(3))]
-        plan.append((t, mi, meshgen.look_at_view_matrix(tuple(e)), res, bool(rng.random() < 0.1)))
     # output and target buffers allocated up front (the server writes them from its own streams)
-    outs = {s: [torch.empty((s, s, 4), device="cuda") for _ in range(32)] for s in sizes}
-    targets = {s: torch.zeros((s, s, 4), device="cuda") for s in sizes}
+    outs = {sz: [torch.empty((sz, sz, 4), device="cuda") for _ in range(32)] for sz in sizes}
+    targets = {sz: torch.zeros((sz, sz, 4), device="cuda") for sz in sizes}
     torch.cuda.synchronize()
-    # warm-up: every mesh once at every size
-    for mi in range(len(meshes)):
-        for s in sizes:
-            srv.wait(srv.submit(ids[mi], plan[0][2], s, s, out=outs[s][0]))
-    tickets = []
-    lock = threading.Lock()
-    results = []
+    results_all = []
+    for qps in (qps_list or [args.qps]):
+        rng = np.random.default_rng(0)
+        nreq = int(qps * args.duration)
+        plan = []
+        t = 0.0
+        for _ in range(nreq):
+            t += rng.exponential(1.0 / qps)
+            mi = int(rng.integers(len(meshes)))
+            e = rng.normal(size=3)
+            e = e / np.linalg.norm(e) * rng.uniform(1.5, 3.0)
+            sz = sizes[int(rng.integers(3))]
+            plan.append((t, mi, meshgen.look_at_view_matrix(tuple(e)), sz, bool(rng.random() < 0.1)))
+        # early ray termination on, the reference renderer's default (TetMeshVolumeRenderer.hpp:243-244)
+        srv = Server(num_workers=args.workers, max_batch=args.max_batch, memory_budget=16 << 30, attenuation=1.0,
+                     fovy=math.pi / 4.0, use_early_ray_termination=True)
+        ids = [srv.add_mesh(m) for m in meshes]
+        # warm-up: every mesh once at every size
+        for mi in range(len(meshes)):
+            for sz in sizes:
+                srv.wait(srv.submit(ids[mi], plan[0][2], sz, sz, out=outs[sz][0]))
+        tickets = []
+        lock = threading.Lock()
+        results = []
 
-    def collector():
-        k = 0
-        while k < nreq:
+        def collector():
+            k = 0
+            while k < nreq:
+                with lock:
+                    have = len(tickets)
+                if k >= have:
+                    time.sleep(0.0002)
+                    continue
+                tk, sz, adj = tickets[k]
+                _, lat, q = srv.wait(tk)
+                results.append((lat, q, sz, adj))
+                k += 1
+
+        th = threading.Thread(target=collector)
+        th.start()
+        start = time.perf_counter()
+        for i, (ta, mi, vm, sz, adj) in enumerate(plan):
+            now = time.perf_counter() - start
+            if ta > now:
+                time.sleep(ta - now)
+            tk = srv.submit(ids[mi], vm, sz, sz, target=targets[sz] if adj else None, out=outs[sz][i % 32])
             with lock:
-                have = len(tickets)
-            if k >= have:
-                time.sleep(0.0002)
-                continue
-            tk, res, adj = tickets[k]
-            _, lat, q = srv.wait(tk)
-            results.append((lat, q, res, adj))
-            k += 1
+                tickets.append((tk, sz, adj))
+        th.join()
+        elapsed = time.perf_counter() - start
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r[0] for r in results])
+        lat1k = np.array([r[0] for r in results if r[2] == 1024])
 
-    th = threading.Thread(target=collector)
-    th.start()
-    start = time.perf_counter()
-    for i, (ta, mi, vm, res, adj) in enumerate(plan):
-        now = time.perf_counter() - start
-        if ta > now:
-            time.sleep(ta - now)
-        tk = srv.submit(ids[mi], vm, res, res, target=targets[res] if adj else None, out=outs[res][i % 32])
-        with lock:
-            tickets.append((tk, res, adj))
-    th.join()
-    elapsed = time.perf_counter() - start
-    st = srv.stats()
-    srv.close()
-    lat = np.array([r[0] for r in results])
-    lat1k = np.array([r[0] for r in results if r[2] == 1024])
+        def pct(sel):
+            v = np.array([r[0] for r in results if sel(r)])
+            return {"n": int(v.size), "p50_ms": float(np.percentile(v, 50)) if v.size else None,
+                    "p99_ms": float(np.percentile(v, 99)) if v.size else None}
 
-    def pct(sel):
-        v = np.array([r[0] for r in results if sel(r)])
-        return {"n": int(v.size), "p50_ms": float(np.percentile(v, 50)) if v.size else None,
-                "p99_ms": float(np.percentile(v, 99)) if v.size else None}
-
-    classes = {f"render_{s}": pct(lambda r, s=s: r[2] == s and not r[3]) for s in sizes}
-    classes.update({f"train_{s}": pct(lambda r, s=s: r[2] == s and r[3]) for s in sizes})
-    out = {
-        "metric": "serve p50/p99 ms (Poisson stream, 32 resident meshes)",
-        "value": float(np.percentile(lat, 99)),
-        "unit": "ms (p99 latency)",
-        "n_gpus": 1,
-        "higher_is_better": False,
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)),
-        "p99_ms_1024": float(np.percentile(lat1k, 99)) if lat1k.size else None,
-        "p50_ms_1024": float(np.percentile(lat1k, 50)) if lat1k.size else None,
-        "p99_ms_1024_render": classes["render_1024"]["p99_ms"],
-        "by_class": classes,
-        "mean_queue_ms": float(np.mean([r[1] for r in results])),
-        "frames_per_s": len(results) / elapsed,
-        "offered_qps": args.qps,
-        "config": {"workload": "Poisson render requests over 16 grid meshes (16^3..80^3 x6) and 16 Delaunay meshes "
-                               f"(10k..500k points), viewports {{256,512,1024}}^2, 10% with loss + adjoint, "
-                               f"{args.workers} workers, up to {args.max_batch} requests per batch, "
-                               "16 GiB in-flight budget, early ray termination on",
-                   "requests": nreq, "meshes": len(meshes), "tets_total": int(sum(tets)),
-                   "mesh_setup_s": round(t_setup, 1), "batches": st["batches"],
-                   "max_batch_seen": st["max_batch_seen"]},
-    }
-    print(json.dumps(out), flush=True)
+        classes = {f"render_{sz}": pct(lambda r, sz=sz: r[2] == sz and not r[3]) for sz in sizes}
+        classes.update({f"train_{sz}": pct(lambda r, sz=sz: r[2] == sz and r[3]) for sz in sizes})
+        out = {
+            "metric": "serve p50/p99 ms (Poisson stream, 32 resident meshes)",
+            "value": float(np.percentile(lat, 99)),
+            "unit": "ms (p99 latency)",
+            "n_gpus": 1,
+            "higher_is_better": False,
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)),
+            "p99_ms_1024": float(np.percentile(lat1k, 99)) if lat1k.size else None,
+            "p50_ms_1024": float(np.percentile(lat1k, 50)) if lat1k.size else None,
+            "p99_ms_1024_render": classes["render_1024"]["p99_ms"],
+            "p50_ms_1024_render": classes["render_1024"]["p50_ms"],
+            "by_class": classes,
+            "mean_queue_ms": float(np.mean([r[1] for r in results])),
+            "frames_per_s": len(results) / elapsed,
+            "offered_qps": qps,
+            "config": {"workload": "Poisson render requests over 16 grid meshes (16^3..80^3 x6) and 16 Delaunay "
+                                   "meshes (10k..500k points), viewports {256,512,1024}^2, 10% with loss + adjoint, "
+                                   f"{args.workers} workers, up to {args.max_batch} requests per batch, "
+                                   "16 GiB in-flight budget, early ray termination on",
+                       "requests": nreq, "duration_s": args.duration, "meshes": len(meshes),
+                       "tets_total": int(sum(tets)), "mesh_setup_s": round(t_setup, 1), "batches": st["batches"],
+                       "max_batch_seen": st["max_batch_seen"]},
+        }
+        if emit:
+            print(json.dumps(out), flush=True)
+        results_all.append(out)
+    return results_all
 
 
 def main():
@@ -482,6 +504,11 @@ def main():
     ap.add_argument("--tile-order", choices=["cost", "row-major"], default="cost",
                     help="single: launch order of the screen tiles (heaviest first, or row-major)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="single: skip the multi-view and serving legs the default line also reports")
+    ap.add_argument("--extra-steps", type=int, default=10, help="single: timed batched steps of the multi-view leg")
+    ap.add_argument("--serve-qps", type=str, default="150,250",
+                    help="single: offered rates of the serving legs (comma-separated requests/s)")
     ap.add_argument("--cpu-row-step", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (scale reference for HBM PMC counters)")
@@ -510,6 +537,10 @@ def main():
     if args.workload == "uneven":
         bench_uneven(args)
         return
+
+    # the serving mix's Delaunay meshes are built in child processes while the GPU legs run (started before
+    # this process touches the GPU)
+    serve_pending = None if args.no_extras else start_serve_meshes()
 
     import numpy as np
     import torch
@@ -639,6 +670,29 @@ def main():
         # the render kernels are bound by VALU issue, not HBM: the fraction that measures them
         "valu_roofline": valu_roofline(f"k_render_{dominant}", stage_ms[dominant]),
     }
+    if not args.no_extras:
+        # the rest of the headline metric, measured in the same run: the batched multi-view step (16 views at
+        # 512^2 on the same mesh) against the plain single-view path, and serving latency at two offered rates
+        del r, r_gt, mesh, opt
+        torch.cuda.empty_cache()
+        mv_args = argparse.Namespace(**vars(args))
+        mv_args.steps, mv_args.warmup, mv_args.res = args.extra_steps, 2, 512
+        mv = bench_multiview(mv_args, emit=False)
+        out["multiview"] = {"views_per_s": mv["value"], "ms_per_step": mv["ms_per_step"],
+                            "single_view_views_per_s": mv["config"]["single_view_views_per_s"],
+                            "speedup_vs_single_view": mv["config"]["speedup_vs_single_view"],
+                            "views": args.views, "res": 512, "steps": args.extra_steps,
+                            "stage_ms": mv["config"]["stage_ms"],
+                            "valu_frac": (mv["valu_roofline"] or {}).get("frac")}
+        torch.cuda.empty_cache()
+        sv = bench_serve(args, qps_list=[float(q) for q in args.serve_qps.split(",")], pending=serve_pending,
+                         emit=False)
+        out["serve"] = [{"offered_qps": o["offered_qps"], "frames_per_s": o["frames_per_s"], "p50_ms": o["p50_ms"],
+                         "p99_ms": o["p99_ms"], "p50_ms_1024_render": o["p50_ms_1024_render"],
+                         "p99_ms_1024_render": o["p99_ms_1024_render"], "p99_ms_1024": o["p99_ms_1024"],
+                         "requests": o["config"]["requests"], "max_batch_seen": o["config"]["max_batch_seen"]}
+                        for o in sv]
+        out["serve_config"] = sv[0]["config"]["workload"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, view, fovy, W, H, args.cpu_row_step)
     print(json.dumps(out), flush=True)

@@ -5,9 +5,12 @@ Drop-in for the renderer/optimiser path of the reference Python module ``difftet
 meaning, tensor layouts and error behaviour.  The compute runs in hand-written HIP kernels behind the C ABI of
 ``include/difftetvr_amd.h``; PyTorch supplies device memory and the current stream.
 
-Differences from the reference: every RendererType renders with the INTERSECTION renderer's maths (Euclidean
-segment length, 4 linear sub-segments per tet, face-centroid depth order), gradients are the exact derivative
-of that forward (``set_adjoint_mode("reference")`` selects the reference's own adjoint), coarse-to-fine
+Renderer types: INTERSECTION renders with that renderer's maths (Euclidean segment length through each tet, 4
+linear sub-segments, face-centroid depth order); PPLL (the reference's default) with the same tile renderer and
+PPLL's segment length, the difference of linear eye depths (PPLL/Forward.glsl:53,188), in the same face-centroid
+order (PPLL sorts each pixel's faces by depth; the two orders agree wherever the centroid order is
+visibility-correct along the ray); PROJECTION is not available and raises.  Gradients are the exact derivative of
+the forward (``set_adjoint_mode("reference")`` selects the INTERSECTION renderer's own adjoint), coarse-to-fine
 refinement chooses prism diagonals by vertex index, and only GPU tensors are supported.
 """
 import ctypes
@@ -306,6 +309,9 @@ class Renderer:
     """Tet-mesh volume renderer (TetMeshVolumeRenderer, src/Renderer/TetMeshVolumeRenderer.hpp:84-280)."""
 
     def __init__(self, renderer_type=RendererType.PPLL):
+        if RendererType(renderer_type) == RendererType.PROJECTION:
+            raise RuntimeError("RendererType.PROJECTION (projected tetrahedra) is not available in difftetvr_amd; "
+                               "use RendererType.PPLL or RendererType.INTERSECTION.")
         ctx = _ctx()
         lib = _lib.load()
         self._renderer_type = RendererType(renderer_type)
@@ -386,7 +392,7 @@ class Renderer:
 
     def set_adjoint_mode(self, mode):
         """'exact' (default): derivative of the forward.  'reference': AdjointIntersectRasterization.glsl as
-        written, including its skipped first sub-segment."""
+        written, including its skipped first sub-segment (INTERSECTION renderers only)."""
         m = {"exact": _lib.DTV_ADJOINT_EXACT, "reference": _lib.DTV_ADJOINT_REFERENCE}.get(str(mode).lower())
         if m is None:
             raise RuntimeError(f"set_adjoint_mode: unknown mode '{mode}'.")
@@ -435,6 +441,9 @@ class Renderer:
         p.tile_size = 8
         p.checkpoint_interval = self._checkpoint_interval
         p.tile_order = self._tile_order
+        # PPLL composites between consecutive faces by their linear depths (PPLL/Forward.glsl:53,188)
+        p.segment_length = (_lib.DTV_SEGMENT_LINEAR_DEPTH if self._renderer_type == RendererType.PPLL
+                            else _lib.DTV_SEGMENT_EUCLIDEAN)
         return p
 
     def camera_uniforms(self):

@@ -226,6 +226,11 @@ bool make_uniforms(const dtv_camera& camera, const dtv_render_params& params, Vi
     for (int i = 0; i < 4; i++) U.clear[i] = params.clear_color[i];
     U.ckpt_interval = params.checkpoint_interval > 0 ? params.checkpoint_interval : 64;
     U.max_ckpt = kMaxCheckpoints;
+    // the view axis: minus row 2 of the view matrix (column-major), i.e. the direction of increasing linear depth
+    U.depth_axis[0] = -camera.view[2];
+    U.depth_axis[1] = -camera.view[6];
+    U.depth_axis[2] = -camera.view[10];
+    U.linear_depth = params.segment_length == DTV_SEGMENT_LINEAR_DEPTH ? 1 : 0;
     return true;
 }
 
@@ -239,6 +244,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_size must be 8");
     if (params->tile_order != 0 && params->tile_order != 1)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_order must be 0 (heaviest first) or 1 (row-major)");
+    if (params->segment_length != DTV_SEGMENT_EUCLIDEAN && params->segment_length != DTV_SEGMENT_LINEAR_DEPTH)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
     std::vector<BatchItem> items((size_t)n);
     std::vector<int64_t> item_V((size_t)n);
     int64_t blocks = 0, tiles = 0, pixels = 0;
@@ -416,6 +423,10 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     if (adjoint_mode != DTV_ADJOINT_EXACT && adjoint_mode != DTV_ADJOINT_REFERENCE)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": unsupported adjoint_mode");
     if (n != view->nitems()) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": item count differs from the render");
+    if (adjoint_mode == DTV_ADJOINT_REFERENCE && view->items[0].U.linear_depth)
+        return fail(DTV_ERR_INVALID_ARGUMENT,
+                    f + ": the reference adjoint mode restates the INTERSECTION renderer's adjoint and needs "
+                        "Euclidean segment lengths");
     for (int i = 0; i < n; i++) {
         if (!tg[i].dimage) continue;  // item not differentiated
         if (!tg[i].grad_pos || !tg[i].grad_col) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");

@@ -45,6 +45,8 @@ struct ViewUniforms {
     float clear[4];
     int32_t ckpt_interval;
     int32_t max_ckpt;
+    float depth_axis[3];     // unit view axis in world space, -(row 2 of the view matrix)
+    int32_t linear_depth;    // 1: segment length = difference of linear eye depths (PPLL), 0: Euclidean
 };
 
 // One render of a batch: a mesh seen through one camera.  A batch is a device array of these, ordered so that

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
                 const int pl = ent >> 5, j = ent & 31;
                 const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
                 FragEval E;
-                eval_fragment(L, j, ro, rdp, att, E);
+                eval_fragment(L, j, ro, rdp, att, seg_scale(U, rdp), E);
                 F.val[i] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
             }
             __syncthreads();
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                 const int pl = ent >> 5, j = ent & 31;
                 const float3_ rdp = lane_ray(my_rd, pl);
                 if (e < total) {
-                    eval_fragment(L, j, ro, rdp, att, E);
+                    eval_fragment(L, j, ro, rdp, att, seg_scale(U, rdp), E);
                     A.val[lane] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
                     if constexpr (kRef) {
 #pragma unroll
@@ -410,7 +410,8 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                     } else {
                         float lam0[4], lam1[4], dcol0[4], dcol1[4];
                         float3_ Q0, Q1;
-                        frag_backward_exact(E, L, j, rdp, att, A.val[lane], lam0, lam1, dcol0, dcol1, Q0, Q1);
+                        frag_backward_exact(E, L, j, rdp, att, seg_scale(U, rdp), A.val[lane], lam0, lam1, dcol0,
+                                            dcol1, Q0, Q1);
                         if (use_abs) {
 #pragma unroll
                             for (int ch = 0; ch < 4; ch++) {

@@ -281,8 +281,15 @@ __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, f
     return t1 >= t0;
 }
 
+// Segment length per unit length along the ray: 1 for the INTERSECTION renderer's Euclidean segments
+// (IntersectRasterization.glsl:211); for the PPLL renderer's differences of linear eye depths (PPLL/Forward.glsl
+// :53,188), the ray's component along the view axis.
+__device__ __forceinline__ float seg_scale(const ViewUniforms& U, float3_ rd) {
+    return U.linear_depth ? ((U.depth_axis[0] * rd.x + U.depth_axis[1] * rd.y) + U.depth_axis[2] * rd.z) : 1.0f;
+}
+
 __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_ ro, float3_ rd, float att,
-                                              FragEval& E) {
+                                              float tscale, FragEval& E) {
     int fi, fo;
     float tin, tout, bary[4];
     bool have_bary;
@@ -322,7 +329,7 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
         E.col1 = make_float4((ca.x * u + cb.x * v) + cc.x * w, (ca.y * u + cb.y * v) + cc.y * w,
                              (ca.z * u + cb.z * v) + cc.z * w, (ca.w * u + cb.w * v) + cc.w * w);
     }
-    sub_segments(E.col0, E.col1, (tout - tin) / 4.0f, att, E.seg, E.pre, E.F);
+    sub_segments(E.col0, E.col1, ((tout - tin) / 4.0f) * tscale, att, E.seg, E.pre, E.F);
 }
 
 __device__ __forceinline__ int pixel_index(const ViewUniforms& U, int x, int y) { return y * U.width + x; }
@@ -376,8 +383,9 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
 }
 
 __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const RecordLds& L, int j, float3_ rd,
-                                                    float att, float4 dF, float lam0[4], float lam1[4],
-                                                    float dcol0[4], float dcol1[4], float3_& Q0, float3_& Q1) {
+                                                    float att, float tscale, float4 dF, float lam0[4],
+                                                    float lam1[4], float dcol0[4], float dcol1[4], float3_& Q0,
+                                                    float3_& Q1) {
     float dR[4] = {dF.x, dF.y, dF.z, dF.w};
 #pragma unroll
     for (int ch = 0; ch < 4; ch++) {
@@ -385,7 +393,7 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
         dcol1[ch] = 0.f;
     }
     float dLdt = 0.f;
-    const float tSeg = (E.t1 - E.t0) / 4.0f;
+    const float tSeg = ((E.t1 - E.t0) / 4.0f) * tscale;
     float4 seg[4], pre[4], Fr;
     float a_s[4], A_s[4], ia_s[4];
     // as in eval_fragment: not kept live since phase A
@@ -433,6 +441,7 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
         dcol1[3] += fm * att * da;
         dLdt += 0.25f * dts;
     }
+    dLdt *= tscale;  // d tSeg / d t1 = tscale / 4
     face_side_q(L, j, E.f0, rd, dcol0, -dLdt, Q0);
     face_side_q(L, j, E.f1, rd, dcol1, dLdt, Q1);
     // barycentric weights over the 4 local vertices (0 at the face's opposite vertex)

@@ -66,7 +66,16 @@ typedef struct dtv_render_params {
     int32_t tile_order;                /* launch order of screen tiles (no reference counterpart: the reference
                                           rasterises): 0 = heaviest first (default; forward by record count,
                                           adjoint by the forward's fragment count), 1 = row-major */
+    int32_t segment_length;            /* DTV_SEGMENT_EUCLIDEAN (INTERSECTION renderer) or
+                                          DTV_SEGMENT_LINEAR_DEPTH (PPLL renderer) */
 } dtv_render_params;
+
+/* Length of a ray's segment through a tet.  EUCLIDEAN: the distance between the entry and exit points, as the
+ * INTERSECTION renderer (Data/Shaders/Intersect/IntersectRasterization.glsl:211).  LINEAR_DEPTH: the difference
+ * of the two faces' linear eye depths, as the PPLL renderer (Data/Shaders/PPLL/Forward.glsl:53,188), i.e. the
+ * Euclidean length times the cosine between the ray and the view axis. */
+#define DTV_SEGMENT_EUCLIDEAN 0
+#define DTV_SEGMENT_LINEAR_DEPTH 1
 
 typedef struct dtv_view_stats {
     int64_t num_triangles;    /* generated front faces (GenerateTrianglesIntersect.glsl:124) */

@@ -49,13 +49,13 @@ def _load():
     lib.orc_frags_export_tris.argtypes = [c.c_void_p, _u32p, _i32p]
     lib.orc_frags_free.argtypes = [c.c_void_p]
     lib.orc_forward_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
-                                  _f32p, _f32p, _i32p]
+                                  c.c_int, _f32p, _f32p, _i32p]
     lib.orc_forward_d.argtypes = [c.c_void_p, _i32p, _f64p, _f64p, _f64p, c.c_double, _f64p, c.c_int, c.c_double,
-                                  _f64p, _f64p, _i32p]
+                                  c.c_int, _f64p, _f64p, _i32p]
     lib.orc_adjoint_exact_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
-                                        _f32p, c.c_int, _f32p, _f32p]
+                                        c.c_int, _f32p, c.c_int, _f32p, _f32p]
     lib.orc_adjoint_exact_d.argtypes = [c.c_void_p, _i32p, _f64p, _f64p, _f64p, c.c_double, _f64p, c.c_int,
-                                        c.c_double, _f64p, c.c_int, _f64p, _f64p]
+                                        c.c_double, c.c_int, _f64p, c.c_int, _f64p, _f64p]
     lib.orc_adjoint_reference_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, _f32p, c.c_int,
                                             _f32p, _f32p]
     lib.orc_accumulate_linear_const_d.argtypes = [c.c_double, _f64p, _f64p, c.c_double, _f64p]
@@ -127,8 +127,10 @@ class Fragments:
 
 
 def forward(frags, positions, colors, cam, attenuation, clear=(0, 0, 0, 0), use_ert=False, ert_alpha=1.0 - 1e-4,
-            dtype=np.float32):
-    """Renders; returns (image[H,W,4], ray colour C[H,W,4] before the clear blend, fragment count[H,W])."""
+            dtype=np.float32, linear_depth=False):
+    """Renders; returns (image[H,W,4], ray colour C[H,W,4] before the clear blend, fragment count[H,W]).
+    linear_depth: segment lengths as differences of linear eye depths (the PPLL renderer's) instead of the
+    Euclidean lengths of the INTERSECTION renderer."""
     lib = _load()
     H, W = frags.height, frags.width
     img = np.zeros((H, W, 4), dtype)
@@ -137,12 +139,12 @@ def forward(frags, positions, colors, cam, attenuation, clear=(0, 0, 0, 0), use_
     fn = lib.orc_forward_f if dtype == np.float32 else lib.orc_forward_d
     fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1),
        np.ascontiguousarray(colors, dtype).reshape(-1), np.ascontiguousarray(cam, dtype), attenuation,
-       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, img, C, cnt)
+       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)), img, C, cnt)
     return img, C, cnt
 
 
 def adjoint_exact(frags, positions, colors, cam, attenuation, dimg, clear=(0, 0, 0, 0), use_ert=False,
-                  ert_alpha=1.0 - 1e-4, use_abs_grad=False, dtype=np.float32):
+                  ert_alpha=1.0 - 1e-4, use_abs_grad=False, dtype=np.float32, linear_depth=False):
     """Exact gradient of sum(dimg * image) w.r.t. vertex positions [V,3] and colours [V,4]."""
     lib = _load()
     V = np.asarray(positions).reshape(-1, 3).shape[0]
@@ -151,8 +153,8 @@ def adjoint_exact(frags, positions, colors, cam, attenuation, dimg, clear=(0, 0,
     fn = lib.orc_adjoint_exact_f if dtype == np.float32 else lib.orc_adjoint_exact_d
     fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1),
        np.ascontiguousarray(colors, dtype).reshape(-1), np.ascontiguousarray(cam, dtype), attenuation,
-       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, np.ascontiguousarray(dimg, dtype).reshape(-1),
-       int(use_abs_grad), gpos.reshape(-1), gcol.reshape(-1))
+       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)),
+       np.ascontiguousarray(dimg, dtype).reshape(-1), int(use_abs_grad), gpos.reshape(-1), gcol.reshape(-1))
     return gpos, gcol
 
 

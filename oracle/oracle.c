@@ -320,23 +320,26 @@ void orc_frags_free(orc_frags* fr) {
 #undef REXP_
 #undef RABS_
 
+/* linear_depth: 0 = INTERSECTION segment length (Euclidean), 1 = PPLL's (difference of linear eye depths). */
 void orc_forward_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* camu,
-                   float att, const float* clear, int use_ert, float ert_alpha, float* img, float* C, int32_t* cnt) {
-    forward_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, img, C, cnt);
+                   float att, const float* clear, int use_ert, float ert_alpha, int linear_depth, float* img, float* C,
+                   int32_t* cnt) {
+    forward_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, img, C, cnt);
 }
 void orc_forward_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col, const double* camu,
-                   double att, const double* clear, int use_ert, double ert_alpha, double* img, double* C, int32_t* cnt) {
-    forward_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, img, C, cnt);
+                   double att, const double* clear, int use_ert, double ert_alpha, int linear_depth, double* img,
+                   double* C, int32_t* cnt) {
+    forward_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, img, C, cnt);
 }
 void orc_adjoint_exact_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* camu,
-                         float att, const float* clear, int use_ert, float ert_alpha, const float* dimg, int use_abs,
-                         float* gpos, float* gcol) {
-    adjoint_exact_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, dimg, use_abs, gpos, gcol);
+                         float att, const float* clear, int use_ert, float ert_alpha, int linear_depth, const float* dimg,
+                         int use_abs, float* gpos, float* gcol) {
+    adjoint_exact_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, dimg, use_abs, gpos, gcol);
 }
 void orc_adjoint_exact_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col,
                          const double* camu, double att, const double* clear, int use_ert, double ert_alpha,
-                         const double* dimg, int use_abs, double* gpos, double* gcol) {
-    adjoint_exact_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, dimg, use_abs, gpos, gcol);
+                         int linear_depth, const double* dimg, int use_abs, double* gpos, double* gcol) {
+    adjoint_exact_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, dimg, use_abs, gpos, gcol);
 }
 void orc_adjoint_reference_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col,
                              const float* camu, float att, const float* img, const float* dimg, int use_abs, float* gpos,

@@ -130,6 +130,7 @@ typedef struct {
     SFX(v4) state[5];     /* front-to-back composite inside the tet, state[0] = 0 */
     R a[4];               /* density * attenuation per sub-segment */
     SFX(v3) sc0[4], sc1[4];
+    R tscale;             /* segment length per unit ray length: 1, or the linear-depth factor (PPLL) */
 } SFX(frag);
 
 static void SFX(load_tet)(const int32_t* cells, const R* pos, const R* col, int64_t tet, SFX(v3) P[4], SFX(v4) C[4], int32_t vid[4]) {
@@ -141,9 +142,21 @@ static void SFX(load_tet)(const int32_t* cells, const R* pos, const R* col, int6
     }
 }
 
-/* One fragment of the INTERSECTION forward: IntersectRasterization.glsl:146-241. */
-static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX(v4) C[4], R att, SFX(frag) * F) {
+/* Segment length per unit length along the ray.  INTERSECTION: the Euclidean distance between the entry and
+ * exit points (IntersectRasterization.glsl:211), factor 1.  PPLL: the difference of the two faces' linear eye
+ * depths (PPLL/Forward.glsl:53,188), i.e. the distance along the view axis: -(row 2 of the view matrix) . d. */
+static R SFX(seg_scale)(const SFX(cam) * c, SFX(v3) rd, int linear_depth) {
+    if (!linear_depth) return (R)1.0;
+    const R* v = c->view;
+    return (((-v[2]) * rd.x + (-v[6]) * rd.y) + (-v[10]) * rd.z);
+}
+
+/* One fragment of the INTERSECTION forward: IntersectRasterization.glsl:146-241 (segment length scaled by
+ * tscale, seg_scale above). */
+static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX(v4) C[4], R att, R tscale,
+                           SFX(frag) * F) {
     memset(F, 0, sizeof(*F));
+    F->tscale = tscale;
     F->hit = SFX(ray_tet)(ro, rd, P, &F->f0, &F->f1, &F->t0, &F->t1);
     if (!F->hit) return;
     F->P0 = SFX(add3)(ro, SFX(scl3)(rd, F->t0));
@@ -162,7 +175,7 @@ static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX
     F->col1 = SFX(mk4)((C[b[0]].x * u + C[b[1]].x * v) + C[b[2]].x * w, (C[b[0]].y * u + C[b[1]].y * v) + C[b[2]].y * w,
                        (C[b[0]].z * u + C[b[1]].z * v) + C[b[2]].z * w, (C[b[0]].w * u + C[b[1]].w * v) + C[b[2]].w * w);
     R t = F->t1 - F->t0;
-    R tSeg = t / (R)4.0;
+    R tSeg = (t / (R)4.0) * tscale;
     SFX(v4) rc = SFX(mk4)(0, 0, 0, 0);
     F->state[0] = rc;
     for (int s = 0; s < 4; s++) {
@@ -186,7 +199,8 @@ static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX
  * premultiplied blend (TetMeshRendererIntersection.cpp:258-276); the clear colour is blended last.
  * Early ray termination follows PPLL/Forward.glsl:168-173 (checked before each fragment). */
 static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* camu, R att,
-                         const R clear[4], int use_ert, R ert_alpha, R* out_img, R* out_C, int32_t* out_cnt) {
+                         const R clear[4], int use_ert, R ert_alpha, int linear_depth, R* out_img, R* out_C,
+                         int32_t* out_cnt) {
     SFX(cam) cam;
     SFX(cam_load)(&cam, camu, fr->W, fr->H);
     SFX(v3) ro = SFX(mk3)(cam.camPos[0], cam.camPos[1], cam.camPos[2]);
@@ -195,6 +209,7 @@ static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos
     for (int64_t px = 0; px < P; px++) {
         int x = (int)(px % fr->W), y = (int)(px / fr->W);
         SFX(v3) rd = SFX(ray_dir)(&cam, x, y);
+        const R tscale = SFX(seg_scale)(&cam, rd, linear_depth);
         SFX(v4) rc = SFX(mk4)(0, 0, 0, 0);
         int32_t cnt = 0;
         for (int64_t k = fr->off[px]; k < fr->off[px + 1]; k++) {
@@ -203,7 +218,7 @@ static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos
             SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
             SFX(load_tet)(cells, pos, col, id >> 2, Pt, Ct, vid);
             SFX(frag) F;
-            SFX(frag_eval)(ro, rd, Pt, Ct, att, &F);
+            SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, &F);
             cnt++;
             if (!F.hit) continue;
             SFX(v4) f = F.state[4];
@@ -278,7 +293,7 @@ static void SFX(frag_backward_exact)(const SFX(frag) * F, SFX(v3) rd, const SFX(
     R dcol0[4] = {0, 0, 0, 0}, dcol1[4] = {0, 0, 0, 0};
     R dLdt = 0;
     R t = F->t1 - F->t0;
-    R tSeg = t / (R)4.0;
+    R tSeg = (t / (R)4.0) * F->tscale;
     for (int s = 3; s >= 0; s--) {
         SFX(v4) sg = F->seg[s];
         SFX(v4) st = F->state[s];
@@ -314,6 +329,7 @@ static void SFX(frag_backward_exact)(const SFX(frag) * F, SFX(v3) rd, const SFX(
         dcol1[3] += fm * att * da;
         dLdt += dts / (R)4.0;
     }
+    dLdt *= F->tscale;  /* d tSeg / d t1 = tscale / 4 */
     /* colour and position gradients per face corner */
     for (int side = 0; side < 2; side++) {
         int f = side == 0 ? F->f0 : F->f1;
@@ -354,7 +370,8 @@ static void SFX(scatter_tet_grads)(const int32_t vid[4], R gp[4][3], R gc[4][4],
 /* Exact adjoint over all pixels.  gpos/gcol are accumulated into (like the reference's .grad buffers,
  * TetMesh.cpp:243-249).  dimg is dL/d(blended image). */
 static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* camu, R att,
-                               const R clear[4], int use_ert, R ert_alpha, const R* dimg, int use_abs, R* gpos, R* gcol) {
+                               const R clear[4], int use_ert, R ert_alpha, int linear_depth, const R* dimg, int use_abs,
+                               R* gpos, R* gcol) {
     SFX(cam) cam;
     SFX(cam_load)(&cam, camu, fr->W, fr->H);
     SFX(v3) ro = SFX(mk3)(cam.camPos[0], cam.camPos[1], cam.camPos[2]);
@@ -370,6 +387,7 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
             if (n + 1 > cap) { cap = 2 * (n + 1); states = (SFX(v4)*)realloc(states, cap * sizeof(SFX(v4))); }
             int x = (int)(px % fr->W), y = (int)(px / fr->W);
             SFX(v3) rd = SFX(ray_dir)(&cam, x, y);
+            const R tscale = SFX(seg_scale)(&cam, rd, linear_depth);
             /* forward, storing the composite state before every fragment */
             SFX(v4) rc = SFX(mk4)(0, 0, 0, 0);
             int64_t m = 0;
@@ -379,7 +397,7 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
                 SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
                 SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
                 SFX(frag) F;
-                SFX(frag_eval)(ro, rd, Pt, Ct, att, &F);
+                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, &F);
                 m = k + 1;
                 if (!F.hit) continue;
                 SFX(v4) f = F.state[4];
@@ -395,7 +413,7 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
                 SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
                 SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
                 SFX(frag) F;
-                SFX(frag_eval)(ro, rd, Pt, Ct, att, &F);
+                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, &F);
                 if (!F.hit) continue;
                 SFX(v4) ci = states[k];
                 SFX(v4) f = F.state[4];
@@ -496,7 +514,7 @@ static void SFX(adjoint_reference)(const orc_frags* fr, const int32_t* cells, co
             SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
             SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
             SFX(frag) F;
-            SFX(frag_eval)(ro, rd, Pt, Ct, att, &F);
+            SFX(frag_eval)(ro, rd, Pt, Ct, att, (R)1.0, &F);
             if (!F.hit) continue;
             const int* a0 = ORC_FT[F.f0];
             const int* a1 = ORC_FT[F.f1];

@@ -254,3 +254,46 @@ def test_adjoint_dense_field_no_early_termination(gpu, att, interval):
     assert float(np.abs(img.cpu().numpy() - oimg).max()) <= IMG_TOL
     assert rel_l2(gc, oc) <= GRAD_TOL
     assert rel_l2(gp, op) <= GRAD_TOL
+
+
+def test_ppll_renderer_linear_depth_parity(gpu):
+    """Renderer() defaults to RendererType.PPLL (Module.cpp:861), whose segment lengths are differences of linear
+    eye depths (PPLL/Forward.glsl:53,188): image and exact adjoint against the oracle's linear-depth mode; off
+    axis it differs from the INTERSECTION renderer's Euclidean lengths; the reference adjoint mode (the
+    INTERSECTION renderer's) is refused for it."""
+    import difftetvr_amd as d
+    w = grid_case(12, seed=8)
+    eye = (1.1, 0.6, 1.3)
+    W, H = 128, 112
+    mesh = d.TetMesh()
+    mesh.set_use_gradients(True)
+    mesh.set_tet_mesh_data(w["cells"], w["pos"], w["col"])
+    r = d.Renderer()
+    assert r.get_renderer_type() == d.RendererType.PPLL
+    r.set_tet_mesh(mesh)
+    r.set_attenuation(2.0)
+    r.set_viewport_size(W, H)
+    r.set_camera_fovy(FOVY)
+    r.set_view_matrix(view_from(eye))
+    r.set_use_early_ray_termination(False)
+    img = r.render()
+    dimg = np.random.default_rng(12).normal(size=(H, W, 4)).astype(np.float32)
+    r.render_adjoint(torch.from_numpy(dimg).cuda(), False)
+    torch.cuda.synchronize()
+    fr, uf, _ = _oracle_frags(w["cells"], w["pos"], eye, W, H)
+    oimg, _, ocnt = oracle.forward(fr, w["pos"], w["col"], uf, 2.0, linear_depth=True)
+    op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 2.0, dimg, linear_depth=True)
+    gp = mesh.get_vertex_positions().grad.cpu().numpy()
+    gc = mesh.get_vertex_colors().grad.cpu().numpy()
+    err = float(np.abs(img.cpu().numpy() - oimg).max())
+    iimg, _, _ = oracle.forward(fr, w["pos"], w["col"], uf, 2.0)
+    print(f"PPLL: max|image - oracle| {err:.2e} (vs the INTERSECTION image {np.abs(iimg - oimg).max():.2e}); "
+          f"adjoint rel L2 pos {rel_l2(gp, op):.2e} col {rel_l2(gc, oc):.2e}", flush=True)
+    assert np.array_equal(r.get_fragment_counts().cpu().numpy(), ocnt)
+    assert err <= IMG_TOL
+    assert float(np.abs(iimg - oimg).max()) > 1e-3
+    assert rel_l2(gc, oc) <= GRAD_TOL
+    assert rel_l2(gp, op) <= GRAD_TOL
+    r.set_adjoint_mode("reference")
+    with pytest.raises(RuntimeError, match="Euclidean"):
+        r.render_adjoint(torch.from_numpy(dimg).cuda(), False)

@@ -137,3 +137,10 @@ def test_module_fails_loudly_without_gpu():
         d.set_device_type("cpu")
     with pytest.raises(d.DtvError, match="GPU"):
         d.TetMesh()
+
+
+def test_projection_renderer_type_is_refused():
+    """RendererType.PROJECTION is not built: asking for it raises instead of rendering something else."""
+    import difftetvr_amd as d
+    with pytest.raises(RuntimeError, match="PROJECTION"):
+        d.Renderer(d.RendererType.PROJECTION)

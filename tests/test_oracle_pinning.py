@@ -98,20 +98,24 @@ def _cube_chord(eye, d):
     return max(0.0, tmax - tmin), tmin, tmax
 
 
+@pytest.mark.parametrize("linear_depth", [False, True])
 @pytest.mark.parametrize("eye", [(0.05, 0.03, 2.0), (0.8, 0.5, 1.6), (-1.1, 0.3, -1.2)])
-def test_constant_field_closed_form_and_watertight(eye):
+def test_constant_field_closed_form_and_watertight(eye, linear_depth):
     """Constant colour c and density s: pixel = c (1 - exp(-s * chord)) exactly (accumulateLinearConst with
     c0 = c1 composes through 'over' to a single exponential).  Missing or doubled fragments would change the
-    chord, so this pins coverage, ray-tet intersection and compositing together.  Generic cameras only: a ray
-    lying exactly in a face plane (e.g. an on-axis eye and the image diagonal) is degenerate for the
-    reference's ray-tet test."""
+    chord, so this pins coverage, ray-tet intersection and compositing together.  With the PPLL renderer's
+    linear-depth segment lengths the chord is measured along the view axis: chord * (d . axis), the axis being
+    the direction from the eye to the origin the camera looks at.  Generic cameras only: a ray lying exactly in
+    a face plane (e.g. an on-axis eye and the image diagonal) is degenerate for the reference's ray-tet test."""
     w = meshgen.grid_workload(5, seed=3)
     col = np.tile(np.float32([0.3, 0.6, 0.9, 2.5]), (w["pos"].shape[0], 1))
     W, H = 50, 46
     uf, ud = oracle.camera_uniforms(view_from(eye), FOVY, ZNEAR, ZFAR, W, H)
     fr = oracle.Fragments(w["cells"], w["pos"], uf, ZNEAR, W, H)
-    img, _, cnt = oracle.forward(fr, w["pos"].astype(np.float64), col.astype(np.float64), ud, 1.0, dtype=np.float64)
+    img, _, cnt = oracle.forward(fr, w["pos"].astype(np.float64), col.astype(np.float64), ud, 1.0, dtype=np.float64,
+                                 linear_depth=linear_depth)
     e = np.asarray(eye, np.float64)
+    axis = -e / np.linalg.norm(e)
     checked = 0
     errs = []
     for y in range(H):
@@ -127,6 +131,8 @@ def test_constant_field_closed_form_and_watertight(eye):
                         near_edge |= (c2 > 0) != (chord > 0)
             if near_edge:
                 continue
+            if linear_depth:
+                chord *= float(d @ axis) / float(np.linalg.norm(d))
             a = 1 - math.exp(-2.5 * chord)
             errs.append(max(abs(img[y, x, 3] - a), float(np.abs(img[y, x, :3] - np.array([0.3, 0.6, 0.9]) * a).max())))
             checked += 1
@@ -199,11 +205,12 @@ def _fd_case():
     return w, col
 
 
-def test_exact_adjoint_matches_fp64_central_differences():
+@pytest.mark.parametrize("linear_depth", [False, True])
+def test_exact_adjoint_matches_fp64_central_differences(linear_depth):
     """FP64 central differences of sum(w * image) on seeded vertices, restricted to pixels whose fragment lists
     are identical at p - h, p and p + h (visibility changes are not differentiable).  h = 1e-6: at 1e-4 a step
     often crosses a switch of a ray's entry or exit face inside the same fragment list (a kink), and the
-    difference quotient then averages two one-sided slopes."""
+    difference quotient then averages two one-sided slopes.  Both segment lengths: Euclidean and linear depth."""
     w, col = _fd_case()
     pos = w["pos"].astype(np.float64)
     colors = col.astype(np.float64)
@@ -237,19 +244,19 @@ def test_exact_adjoint_matches_fp64_central_differences():
                 mask &= lists_equal_mask(frs)
                 vals.append((frs, pp))
             mw = wimg * mask.reshape(H, W, 1)
-            fplus = oracle.forward(vals[0][0], vals[0][1], colors, ud, 1.0, dtype=np.float64)[0]
-            fminus = oracle.forward(vals[1][0], vals[1][1], colors, ud, 1.0, dtype=np.float64)[0]
+            fplus = oracle.forward(vals[0][0], vals[0][1], colors, ud, 1.0, dtype=np.float64, linear_depth=linear_depth)[0]
+            fminus = oracle.forward(vals[1][0], vals[1][1], colors, ud, 1.0, dtype=np.float64, linear_depth=linear_depth)[0]
             fd = float(np.sum(mw * (fplus - fminus)) / (2 * h))
-            gp, _ = oracle.adjoint_exact(fr0, pos, colors, ud, 1.0, mw, dtype=np.float64)
+            gp, _ = oracle.adjoint_exact(fr0, pos, colors, ud, 1.0, mw, dtype=np.float64, linear_depth=linear_depth)
             errs_p.append((fd, gp[v, comp]))
         for comp in range(4):
             cc = colors.copy()
             cc[v, comp] += h
-            fp = oracle.forward(fr0, pos, cc, ud, 1.0, dtype=np.float64)[0]
+            fp = oracle.forward(fr0, pos, cc, ud, 1.0, dtype=np.float64, linear_depth=linear_depth)[0]
             cc[v, comp] -= 2 * h
-            fm = oracle.forward(fr0, pos, cc, ud, 1.0, dtype=np.float64)[0]
+            fm = oracle.forward(fr0, pos, cc, ud, 1.0, dtype=np.float64, linear_depth=linear_depth)[0]
             fd = float(np.sum(wimg * (fp - fm)) / (2 * h))
-            _, gc = oracle.adjoint_exact(fr0, pos, colors, ud, 1.0, wimg, dtype=np.float64)
+            _, gc = oracle.adjoint_exact(fr0, pos, colors, ud, 1.0, wimg, dtype=np.float64, linear_depth=linear_depth)
             errs_c.append((fd, gc[v, comp]))
     for name, errs in (("pos", errs_p), ("col", errs_c)):
         fd = np.array([e[0] for e in errs])
