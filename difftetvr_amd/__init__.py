@@ -270,6 +270,23 @@ class TetMesh:
         o = torch.sum(torch.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], dim=1) * (p[:, 3] - p[:, 0]), dim=1)
         return bool((o >= 0).any())
 
+    def unlink_tets(self):
+        """Gives every tet its own four vertices, undoing the shared-index representation (TetMesh.cpp:1950-1989).
+        The mesh tensors are new afterwards."""
+        self._require()
+        cells, pos, col = meshgen.unlink_tets(self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(),
+                                              self._colors.detach().cpu().numpy())
+        self.set_tet_mesh_data(cells, pos, col)
+
+    def remove_transparent_tets(self, alpha_threshold):
+        """Removes the tets none of whose vertices has alpha >= alpha_threshold, and the vertices no remaining tet
+        uses (TetMesh.cpp:1991-2079).  The mesh tensors are new afterwards."""
+        self._require()
+        cells, pos, col, _ = meshgen.remove_transparent_tets(
+            self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(), self._colors.detach().cpu().numpy(),
+            float(alpha_threshold))
+        self.set_tet_mesh_data(cells, pos, col)
+
     def set_force_use_ovm_representation(self):
         """Accepted for compatibility (Module.cpp:794): refinement works on the plain cell array."""
 

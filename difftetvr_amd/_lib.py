@@ -63,7 +63,8 @@ class DtvServerStats(ctypes.Structure):
 
 class DtvAdamGroup(ctypes.Structure):
     _fields_ = [("param", c_void_p), ("grad", c_void_p), ("exp_avg", c_void_p), ("exp_avg_sq", c_void_p),
-                ("numel", c_int64), ("lr", c_float), ("beta1", c_float), ("beta2", c_float), ("eps", c_float),
+                ("numel", c_int64), ("lr", ctypes.c_double), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
+                ("eps", ctypes.c_double),
                 ("step", c_int64), ("clamp_nonneg", c_int32), ("zero_grad", c_int32)]
 
 

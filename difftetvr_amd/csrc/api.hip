@@ -158,7 +158,7 @@ struct dtv_view {
     DevBuf dItems, dTargets, counts, offsets, scan_status, small, block_stats, keys, vals, keys2, vals2,
         recs, hist,
         lookback, ranges;
-    DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_color, ckpt_n;
+    DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_t, ckpt_n;
     DevBuf tile_frags, tile_order;  // per tile: forward's fragments; launch orders of the forward and adjoint
     bool cost_ordered = true;       // the last render launched its tiles heaviest first
     int64_t planned_cap = 0;  // > 0: planned (synchronisation-free) rendering with this record capacity
@@ -184,7 +184,7 @@ struct dtv_view {
         ps.frag_count = frag_count.as<uint32_t>();
         ps.last_rec = last_rec.as<uint32_t>();
         ps.num_ckpt = num_ckpt.as<uint32_t>();
-        ps.ckpt_color = ckpt_color.as<float4>();
+        ps.ckpt_t = ckpt_t.as<float>();
         ps.ckpt_n = ckpt_n.as<uint32_t>();
         ps.counters = nullptr;
         ps.tile_frags = tile_frags.as<uint32_t>();
@@ -400,7 +400,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->frag_count.ensure(sizeof(uint32_t) * pixels));
     DTV_HIP(view->last_rec.ensure(sizeof(uint32_t) * pixels));
     DTV_HIP(view->num_ckpt.ensure(sizeof(uint32_t) * pixels));
-    DTV_HIP(view->ckpt_color.ensure(sizeof(float4) * pixels * kMaxCheckpoints));
+    DTV_HIP(view->ckpt_t.ensure(sizeof(float) * pixels * kMaxCheckpoints));
     DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * pixels * kMaxCheckpoints));
     view->mark(DTV_STAGE_FORWARD, 0, s);
     launch_render_forward(dI, n, tiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
@@ -538,7 +538,7 @@ int dtv_view_destroy(dtv_view* v) {
     DevBuf* bufs[] = {&v->dItems, &v->dTargets, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
                       &v->vals, &v->keys2,
                       &v->vals2, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
-                      &v->num_ckpt, &v->ckpt_color, &v->ckpt_n, &v->tile_frags, &v->tile_order};
+                      &v->num_ckpt, &v->ckpt_t, &v->ckpt_n, &v->tile_frags, &v->tile_order};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < DTV_NUM_STAGES; i++)
         for (int k = 0; k < 2; k++)
@@ -727,21 +727,24 @@ int adam(const dtv_adam_group* groups, int32_t num_groups, int64_t* step_dev, hi
             return fail(DTV_ERR_INVALID_ARGUMENT, f + ": buffers must be 16-byte aligned");
         const int64_t step = step_dev ? 1 : G.step;  // with a device step count the host value is unused
         if (step < 1) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": step must be >= 1");
-        const double bc1 = 1.0 - std::pow((double)G.beta1, (double)step);
-        const double bc2 = 1.0 - std::pow((double)G.beta2, (double)step);
+        // torch.optim.Adam: bias corrections in double from the double hyper-parameters, the update's constants
+        // (1 - beta1, beta2, 1 - beta2, lr / bc1, sqrt(bc2), eps) rounded once to float
+        const double bc1 = 1.0 - std::pow(G.beta1, (double)step);
+        const double bc2 = 1.0 - std::pow(G.beta2, (double)step);
         g[i].lr = G.lr;
         g[i].beta1 = G.beta1;
+        g[i].beta2d = G.beta2;
         g[i].param = G.param;
         g[i].grad = G.grad;
         g[i].m = G.exp_avg;
         g[i].v = G.exp_avg_sq;
         g[i].n = G.numel;
-        g[i].step_size = (float)((double)G.lr / bc1);
+        g[i].step_size = (float)(G.lr / bc1);
         g[i].bc2_sqrt = (float)std::sqrt(bc2);
-        g[i].w1 = (float)(1.0 - (double)G.beta1);
-        g[i].beta2 = G.beta2;
-        g[i].w2 = (float)(1.0 - (double)G.beta2);
-        g[i].eps = G.eps;
+        g[i].w1 = (float)(1.0 - G.beta1);
+        g[i].beta2 = (float)G.beta2;
+        g[i].w2 = (float)(1.0 - G.beta2);
+        g[i].eps = (float)G.eps;
         g[i].clamp_nonneg = G.clamp_nonneg;
         g[i].zero_grad = G.zero_grad;
     }

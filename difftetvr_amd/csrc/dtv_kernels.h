@@ -68,11 +68,11 @@ void launch_tile_order(const uint2* ranges, const uint32_t* cost, int64_t num_ti
 // Per-pixel state of a rendered batch, concatenated over its items (P = all pixels of the batch); item i's
 // checkpoint planes start at max_ckpt * pixel0_i.
 struct PixelState {
-    float4* ray_color;      // [P] composite before the clear-colour blend
+    float4* ray_color;      // [P] composite before the clear-colour blend: premultiplied RGB, transmittance
     uint32_t* frag_count;   // [P] fragments composited
     uint32_t* last_rec;     // [P] index (in the sorted record list) one past the last composited record
     uint32_t* num_ckpt;     // [P]
-    float4* ckpt_color;     // [max_ckpt][P]
+    float* ckpt_t;          // [max_ckpt][P] transmittance after ckpt_n fragments
     uint32_t* ckpt_n;       // [max_ckpt][P]
     unsigned long long* counters;  // optional (NULL): [0] += gradient float atomics issued by the adjoint
     uint32_t* tile_frags;   // [tiles of the batch] fragments composited per tile (written by the forward)
@@ -101,7 +101,7 @@ struct AdamGroupDev {
     float eps;
     int32_t clamp_nonneg;
     int32_t zero_grad;
-    float lr, beta1;    // for bias corrections computed on the device
+    double lr, beta1, beta2d;  // for bias corrections computed on the device
 };
 // step_dev (may be NULL): steps taken so far in device memory; then the bias corrections use step_dev + 1
 // (step_size / bc2_sqrt of the groups are ignored) and a second launch increments it.

@@ -35,8 +35,8 @@ __global__ __launch_bounds__(256) void k_adam(AdamLaunch A) {
     if (A.step_dev != nullptr) {
         // bias corrections of step t = taken + 1, in double as on the host
         const double t = (double)(*A.step_dev + 1);
-        G.step_size = (float)((double)G.lr / (1.0 - pow((double)G.beta1, t)));
-        G.bc2_sqrt = (float)sqrt(1.0 - pow((double)G.beta2, t));
+        G.step_size = (float)(G.lr / (1.0 - pow(G.beta1, t)));
+        G.bc2_sqrt = (float)sqrt(1.0 - pow(G.beta2d, t));
     }
     const int64_t i4 = ((int64_t)blockIdx.x - A.block_start[gi]) * 256 + threadIdx.x;
     const int64_t n4 = G.n / 4;

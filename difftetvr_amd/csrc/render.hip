@@ -26,8 +26,8 @@ namespace dtv {
 constexpr int kRound = 128;                 // fragments shaded per round (forward)
 constexpr int kRoundAdj = 64;               // fragments per round in the adjoint: one per lane
 constexpr int kMaxFrags = kBatch * 64;      // fragments a batch can produce
-constexpr float kMissMarker = -1.0f;        // alpha of a fragment whose ray missed the tet
-constexpr int kCkptReserve = 10;            // checkpoint slots periodic checkpoints leave to the transmittance rule
+constexpr float kMissMarker = -1.0f;        // transmittance slot of a fragment whose ray missed the tet
+constexpr float kTinyT = 0x1p-100f;         // transmittance below which a fragment's input is not recovered
 
 // The forward's fragment list and round values.
 struct FragListLds {
@@ -103,9 +103,8 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     const int P = U.width * U.height;
     const int pix = valid ? pixel_index(U, x, y) : 0;
 
-    float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 C = make_float4(0.f, 0.f, 0.f, 1.0f);  // premultiplied RGB, transmittance
     uint32_t n = 0, last = rg.x, nck = 0, nlast = 0;
-    float Tlast = 1.0f;
     bool done = !valid;
     // Two-stage prefetch: while a batch is shaded, the next batch's vertex data and the record words of the
     // batch after it are in flight.
@@ -144,43 +143,40 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
             __syncthreads();
             const uint32_t e0 = max(off, r0), e1 = min(mine_end, r0 + kRound);
             for (uint32_t e = e0; e < e1 && !done; e++) {
-                if (U.use_ert && C.w > U.ert_alpha) {
+                if (U.use_ert && (1.0f - C.w) > U.ert_alpha) {
                     done = true;
                     break;
                 }
                 const float4 f = F.val[e - r0];
                 if (f.w != kMissMarker) {
-                    // Checkpoint rules (the adjoint recovers each fragment's input state by inverting 'over'
-                    // from the next checkpoint; an inversion through a drop in transmittance loses precision):
-                    // a checkpoint goes before any fragment that takes the transmittance below 1/8 of the last
-                    // checkpoint's (opaque tets included, so the state before them is never inverted), and none
-                    // is stored once alpha is exactly 1 (later fragments leave the composite unchanged).  The
-                    // state before the first fragment (zero) is known without a slot.
-                    const float Tafter = (1.0f - C.w) * (1.0f - f.w);
-                    if (C.w < 1.0f && Tafter < Tlast * 0.125f) {
-                        if (nlast != n && (int)nck < U.max_ckpt) {
-                            ps.ckpt_color[(int64_t)nck * P + pix] = C;
-                            ps.ckpt_n[(int64_t)nck * P + pix] = n;
-                            nck++;
-                            nlast = n;
-                        }
-                        Tlast = Tafter;
+                    // C.w is the transmittance T, the product of the fragments' transmittances (relative
+                    // precision at any depth), so the adjoint recovers each fragment's input as T_out / T_f.
+                    // Only an underflow (T_out below kTinyT) cannot be inverted: a checkpoint goes before the
+                    // fragment that causes it.  The state before the first fragment (T = 1) needs no slot.
+                    const float Tout = C.w * f.w;
+                    if (C.w >= kTinyT && Tout < kTinyT && nlast != n && (int)nck < U.max_ckpt) {
+                        ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
+                        ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                        nck++;
+                        nlast = n;
                     }
-                    C.x = C.x + (1.0f - C.w) * f.x;
-                    C.y = C.y + (1.0f - C.w) * f.y;
-                    C.z = C.z + (1.0f - C.w) * f.z;
-                    C.w = C.w + (1.0f - C.w) * f.w;
+                    {
+#pragma clang fp contract(fast)
+                        C.x = C.x + C.w * f.x;
+                        C.y = C.y + C.w * f.y;
+                        C.z = C.z + C.w * f.z;
+                    }
+                    C.w = Tout;
                 }
                 n++;
                 last = base + (F.entry[e] & 31) + 1;
-                // periodic checkpoints bound the inversion chains; the last kCkptReserve slots are kept for the
-                // transmittance rule above, which needs at most 9 before alpha reaches 1 in float
-                if ((int)(n - nlast) >= U.ckpt_interval && C.w < 1.0f && (int)nck < U.max_ckpt - kCkptReserve) {
-                    ps.ckpt_color[(int64_t)nck * P + pix] = C;
+                // periodic checkpoints bound the rounding the adjoint's chain of divisions accumulates; the last
+                // slot is kept for the underflow rule above
+                if ((int)(n - nlast) >= U.ckpt_interval && C.w >= kTinyT && (int)nck < U.max_ckpt - 1) {
+                    ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
                     ps.ckpt_n[(int64_t)nck * P + pix] = n;
                     nck++;
                     nlast = n;
-                    Tlast = 1.0f - C.w;
                 }
             }
             __syncthreads();
@@ -188,9 +184,9 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
         if (ballot(!done) == 0ull) break;
     }
     if (valid) {
-        const float T = 1.0f - C.w;
+        const float T = C.w;
         out_image[pix] = make_float4(C.x + T * U.clear[0], C.y + T * U.clear[1], C.z + T * U.clear[2],
-                                     C.w + T * U.clear[3]);
+                                     (1.0f - T) + T * U.clear[3]);
         ps.ray_color[pix] = C;
         ps.frag_count[pix] = n;
         ps.last_rec[pix] = last;
@@ -242,14 +238,16 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
     const int pix = valid ? pixel_index(U, x, y) : 0;
 
     uint32_t natom = 0;  // gradient atomics this lane issued (counted when ps.counters is set)
-    float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+    // exact mode: C.w = the transmittance behind the fragment being undone (only it is needed); reference mode:
+    // the blended premultiplied RGBA the reference inverts
+    float4 C = make_float4(0.f, 0.f, 0.f, 1.0f);
     float dC[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t n = 0, last = 0;
     int jck = -1;                // next checkpoint slot (back to front)
-    // Checkpoint jck's fragment count and colour, and the count of the one before it, are held in registers:
-    // the next checkpoint is always loaded well before it is needed.
+    // Checkpoint jck's fragment count and transmittance, and the count of the one before it, are held in
+    // registers: the next checkpoint is always loaded well before it is needed.
     uint32_t ck_n = 0xFFFFFFFFu, ck_n2 = 0xFFFFFFFFu;
-    float4 ck_c = make_float4(0.f, 0.f, 0.f, 0.f);
+    float ck_t = 1.0f;
     if (valid) {
         C = ps.ray_color[pix];
         n = ps.frag_count[pix];
@@ -257,7 +255,7 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
         jck = (int)ps.num_ckpt[pix] - 1;
         if (jck >= 0) {
             ck_n = ps.ckpt_n[(int64_t)jck * P + pix];
-            ck_c = ps.ckpt_color[(int64_t)jck * P + pix];
+            ck_t = ps.ckpt_t[(int64_t)jck * P + pix];
             if (jck >= 1) ck_n2 = ps.ckpt_n[(int64_t)(jck - 1) * P + pix];
         }
         const float4 di = dimage[pix];
@@ -266,8 +264,9 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
         dC[2] = di.z;
         if constexpr (kRef) {
             // the reference starts from the blended output image and the raw image adjoint
-            const float T = 1.0f - C.w;
-            C = make_float4(C.x + T * U.clear[0], C.y + T * U.clear[1], C.z + T * U.clear[2], C.w + T * U.clear[3]);
+            const float T = C.w;
+            C = make_float4(C.x + T * U.clear[0], C.y + T * U.clear[1], C.z + T * U.clear[2],
+                            (1.0f - T) + T * U.clear[3]);
             dC[3] = di.w;
             jck = -1;
         } else {
@@ -358,35 +357,35 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                     continue;
                 }
                 if (jck >= 0 && ck_n == n) {
-                    C = ck_c;
+                    C.w = ck_t;
                     jck--;
                     ck_n = ck_n2;
                     if (jck >= 0) {
-                        ck_c = ps.ckpt_color[(int64_t)jck * P + pix];
+                        ck_t = ps.ckpt_t[(int64_t)jck * P + pix];
                         ck_n2 = jck >= 1 ? ps.ckpt_n[(int64_t)(jck - 1) * P + pix] : 0xFFFFFFFFu;
                     }
                 }
                 const float4 f = A.val[e - r0];
                 if (f.w != kMissMarker) {
-                    float4 Cin;
+                    // the fragment's input transmittance: a checkpoint, 1 before the first fragment, or the
+                    // output divided by the fragment's transmittance (a corrected quotient: the rounding
+                    // compounds along the chain back to the next checkpoint).  Below kTinyT the forward's rule
+                    // leaves no checkpoint only where the input was below kTinyT too: that fragment's
+                    // contribution (at most kTinyT |dL/dI|) is dropped.
+                    float Tin;
                     if (jck >= 0 && ck_n == n - 1) {
-                        Cin = ck_c;
+                        Tin = ck_t;
                     } else if (n == 1) {
-                        Cin = make_float4(0.f, 0.f, 0.f, 0.f);  // the first fragment starts from zero
+                        Tin = 1.0f;
                     } else {
-                        // the inversion's rounding error compounds along the chain back to the next checkpoint:
-                        // a corrected quotient (the bare hardware reciprocal doubles the adjoint's error).  With
-                        // no checkpoint before it, a fragment of alpha 1 can only follow a saturated composite
-                        // (the forward's checkpoint rule), whose state the fragment did not change.
-                        const float den = f.w - 1.0f;
-                        const float ain = den == 0.0f ? C.w : div_nr(f.w - C.w, den);
-                        Cin = make_float4(C.x - (1.0f - ain) * f.x, C.y - (1.0f - ain) * f.y,
-                                          C.z - (1.0f - ain) * f.z, ain);
+                        Tin = C.w < kTinyT ? 0.0f : div_nr(C.w, f.w);
                     }
-                    const float omA = 1.0f - Cin.w;
-                    A.val[e - r0] = make_float4(omA * dC[0], omA * dC[1], omA * dC[2], omA * dC[3]);
-                    dC[3] = dC[3] * (1.0f - f.w) - (dC[0] * f.x + dC[1] * f.y + dC[2] * f.z);
-                    C = Cin;
+                    A.val[e - r0] = make_float4(Tin * dC[0], Tin * dC[1], Tin * dC[2], Tin * dC[3]);
+                    {
+#pragma clang fp contract(fast)
+                        dC[3] = dC[3] * f.w - (dC[0] * f.x + dC[1] * f.y + dC[2] * f.z);
+                    }
+                    C.w = Tin;
                 }
                 n--;
             }

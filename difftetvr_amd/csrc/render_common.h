@@ -115,11 +115,27 @@ struct FragEval {
     float u0, v0, u1, v1;
     float4 col0, col1;
     float4 seg[4];
-    float4 pre[4];   // composite inside the tet before sub-segment s
-    float4 F;        // composite of the 4 sub-segments
+    float4 pre[4];   // composite inside the tet before sub-segment s (RGB, transmittance)
+    float4 F;        // composite of the 4 sub-segments (premultiplied RGB, transmittance of the fragment)
 };
 
 __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+
+// Vector helpers for the continuous maths (colours, sub-segment integrals, compositing, gradients), where the
+// products and sums may fuse into FMAs.  The discrete maths (projection, coverage, ray-tet decisions) keeps the
+// separately rounded operations of dtv_device.h, which the binning and the oracle evaluate alike.
+__device__ __forceinline__ float cdot(float3_ a, float3_ b) {
+#pragma clang fp contract(fast)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ float3_ ccross(float3_ a, float3_ b) {
+#pragma clang fp contract(fast)
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float3_ caxpy(float s, float3_ a, float3_ b) {  // s a + b
+#pragma clang fp contract(fast)
+    return {s * a.x + b.x, s * a.y + b.y, s * a.z + b.z};
+}
 
 // The hardware's reciprocal, square root and reciprocal square root (about 1 ulp; one instruction each, where
 // x / y and sqrtf expand into correctly rounded sequences of ten or more).  Only for continuous quantities
@@ -139,15 +155,18 @@ __device__ __forceinline__ float div_nr(float x, float y) {
 // The 4 sub-segments of a segment of length 4 tSeg between the entry and exit colours (RGB + density):
 // seg[s] = accumulateLinearConst of sub-segment s (ForwardCommon.glsl:42-49, colour endpoints mixed at s/4 and
 // (s+1)/4, density at the midpoint, IntersectRasterization.glsl:228-241), pre[s] = their composite before s,
-// F = the composite of all four.  The colour at fraction f is formed as c0 + f (c1 - c0).  The forward and the
-// exact backward both call this, so the backward can recompute seg/pre instead of keeping them in registers
-// from phase A; it also takes each sub-segment's density a[s], transmittance A[s] = exp(-a tSeg) and 1 / a
-// (A = 1 and 1 / a = 0 where a < 1e-6: the sub-segment is empty).
+// F = the composite of all four.  The colour at fraction f is formed as c0 + f (c1 - c0).  Composites carry
+// premultiplied RGB and, in .w, the transmittance (1 - alpha) as a product of the sub-segments' exp(-a tSeg):
+// a product keeps its relative precision at any depth, where 1 - alpha formed from an alpha near 1 does not.
+// The forward and the exact backward both call this, so the backward can recompute seg/pre instead of keeping
+// them in registers from phase A; it also takes each sub-segment's density a[s], transmittance
+// A[s] = exp(-a tSeg) and 1 / a (A = 1 and 1 / a = 0 where a < 1e-6: the sub-segment is empty).
 __device__ __forceinline__ void sub_segments_ex(float4 col0, float4 col1, float tSeg, float att, float4 seg[4],
                                                 float4 pre[4], float4& F, float a_s[4], float A_s[4],
                                                 float ia_s[4]) {
+#pragma clang fp contract(fast)
     const float dx = col1.x - col0.x, dy = col1.y - col0.y, dz = col1.z - col0.z, dw = col1.w - col0.w;
-    float4 rc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 rc = make_float4(0.f, 0.f, 0.f, 1.0f);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f;
@@ -169,10 +188,10 @@ __device__ __forceinline__ void sub_segments_ex(float4 col0, float4 col1, float 
         }
         pre[s] = rc;
         seg[s] = cur;
-        rc.x = rc.x + (1.0f - rc.w) * cur.x;
-        rc.y = rc.y + (1.0f - rc.w) * cur.y;
-        rc.z = rc.z + (1.0f - rc.w) * cur.z;
-        rc.w = rc.w + (1.0f - rc.w) * cur.w;
+        rc.x = rc.x + rc.w * cur.x;
+        rc.y = rc.y + rc.w * cur.y;
+        rc.z = rc.z + rc.w * cur.z;
+        rc.w = rc.w * A_s[s];
     }
     F = rc;
 }
@@ -224,6 +243,8 @@ __device__ __forceinline__ bool ref_face_contains_bary(const RecordLds& L, int j
     return true;
 }
 
+constexpr float kEdgeMargin = 1e-4f;
+
 // Entry/exit faces and distances with the semantics of intersectRayTet (RayIntersectionTests.glsl:78-106).
 // Fast path: the plane-slab test (last entering / first exiting plane) proposes the two faces, and the
 // reference inside test confirms both; otherwise the reference's four-face procedure decides (edges,
@@ -250,9 +271,15 @@ __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, f
             if (tf[g] < tout) { tout = tf[g]; fo = g; }
         }
     }
+    // The reference procedure accepts every face whose inside test passes and takes the nearest and farthest:
+    // where the ray passes within rounding of an edge, a neighbouring face of the slab test's pick passes too and
+    // may be the one chosen.  The fast path is taken only when both hits lie at least kEdgeMargin (in barycentric
+    // coordinates) inside their faces, where no other face can pass; otherwise the reference procedure decides.
     if (fi >= 0 && fo >= 0 && tout >= tin && tin >= 0.0f &&
         ref_face_contains_bary(L, j, fi, ro, rd, tin, bary[0], bary[1]) &&
-        ref_face_contains_bary(L, j, fo, ro, rd, tout, bary[2], bary[3])) {
+        ref_face_contains_bary(L, j, fo, ro, rd, tout, bary[2], bary[3]) &&
+        fminf(fminf(fminf(bary[0], bary[1]), 1.0f - bary[0] - bary[1]),
+              fminf(fminf(bary[2], bary[3]), 1.0f - bary[2] - bary[3])) >= kEdgeMargin) {
         f0 = fi;
         f1 = fo;
         t0 = tin;
@@ -295,7 +322,7 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
     bool have_bary;
     E.hit = ray_tet(L, j, ro, rd, fi, fo, tin, tout, bary, have_bary);
     if (!E.hit) {
-        E.F = make_float4(0.f, 0.f, 0.f, 0.f);
+        E.F = make_float4(0.f, 0.f, 0.f, 1.0f);
         return;
     }
     E.f0 = fi;
@@ -310,6 +337,7 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
                  P1, bary[2], bary[3]);
     }
     {
+#pragma clang fp contract(fast)
         const int a = face_vertex(fi, 0), b = face_vertex(fi, 1), c = face_vertex(fi, 2);
         const float u = bary[0], v = bary[1];
         const float w = 1.0f - u - v;
@@ -320,6 +348,7 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
                              (ca.z * u + cb.z * v) + cc.z * w, (ca.w * u + cb.w * v) + cc.w * w);
     }
     {
+#pragma clang fp contract(fast)
         const int a = face_vertex(fo, 0), b = face_vertex(fo, 1), c = face_vertex(fo, 2);
         const float u = bary[2], v = bary[3];
         const float w = 1.0f - u - v;
@@ -347,7 +376,7 @@ __device__ __forceinline__ PixelState item_pixel_state(const PixelState& all, co
     ps.frag_count = all.frag_count + B.pixel0;
     ps.last_rec = all.last_rec + B.pixel0;
     ps.num_ckpt = all.num_ckpt + B.pixel0;
-    ps.ckpt_color = all.ckpt_color + (int64_t)B.U.max_ckpt * B.pixel0;
+    ps.ckpt_t = all.ckpt_t + (int64_t)B.U.max_ckpt * B.pixel0;
     ps.ckpt_n = all.ckpt_n + (int64_t)B.U.max_ckpt * B.pixel0;
     ps.counters = all.counters;
     ps.tile_frags = all.tile_frags;  // indexed by batch tile
@@ -360,12 +389,13 @@ __device__ __forceinline__ PixelState item_pixel_state(const PixelState& all, co
 // in-plane colour gradient of the face and s = -1 (entry face) / +1 (exit face).
 __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, float3_ rd, const float dC[4],
                                             float sdLdt, float3_& Q) {
+#pragma clang fp contract(fast)
     const int ia = face_vertex(f, 0), ib = face_vertex(f, 1), ic = face_vertex(f, 2);
     const float3_ a = lds_p(L, ia, j), b = lds_p(L, ib, j), c = lds_p(L, ic, j);
     const float4 ca = lds_c(L, ia, j), cb = lds_c(L, ib, j), cc = lds_c(L, ic, j);
     const float3_ e1 = sub(b, a), e2 = sub(c, a);
     const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);  // = cross(e1, e2), staged
-    const float inn = fast_rcp(dot(n, n));
+    const float inn = fast_rcp(cdot(n, n));
     // G^T dC and dC . (G d), with the in-plane gradients of the barycentrics of b and c
     // gb = (e2 x n) / |n|^2 and gc = (n x e1) / |n|^2:  gb sb + gc sc = ((sb e2 - sc e1) x n) / |n|^2
     const float db[4] = {cb.x - ca.x, cb.y - ca.y, cb.z - ca.z, cb.w - ca.w};
@@ -376,16 +406,17 @@ __device__ __forceinline__ void face_side_q(const RecordLds& L, int j, int f, fl
         sb += db[ch] * dC[ch];
         sc += dcv[ch] * dC[ch];
     }
-    const float3_ gtdc = scale(cross(sub(scale(e2, sb), scale(e1, sc)), n), inn);
-    const float dcgd = dot(gtdc, rd);
-    const float s = fast_div(sdLdt + dcgd, dot(n, rd));
-    Q = sub(scale(n, s), gtdc);
+    const float3_ gtdc = scale(ccross(caxpy(-sc, e1, scale(e2, sb)), n), inn);
+    const float dcgd = cdot(gtdc, rd);
+    const float s = fast_div(sdLdt + dcgd, cdot(n, rd));
+    Q = caxpy(s, n, scale(gtdc, -1.0f));
 }
 
 __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const RecordLds& L, int j, float3_ rd,
                                                     float att, float tscale, float4 dF, float lam0[4],
                                                     float lam1[4], float dcol0[4], float dcol1[4], float3_& Q0,
                                                     float3_& Q1) {
+#pragma clang fp contract(fast)
     float dR[4] = {dF.x, dF.y, dF.z, dF.w};
 #pragma unroll
     for (int ch = 0; ch < 4; ch++) {
@@ -405,7 +436,7 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
         const float4 sg = seg[s], st = pre[s];
-        const float omA = 1.0f - st.w;  // dL/d(sub-segment s) = omA dR
+        const float omA = st.w;  // transmittance before s: dL/d(sub-segment s) = omA dR
         const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
         float sc0 = 0.f, sdf = 0.f;  // sum over RGB of dR c0 and of dR (c0 - c1)
 #pragma unroll
@@ -414,7 +445,7 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
             sdf += dR[ch] * diff[ch];
         }
         const float dsw = omA * dR[3];
-        dR[3] = dR[3] * (1.0f - sg.w) - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
+        dR[3] = dR[3] * A_s[s] - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
         const float a = a_s[s];
         float da = 0.f, dts = 0.f, w0 = 0.f, w1 = 0.f;
         if (!(a < 1e-6f)) {

@@ -188,3 +188,31 @@ def grid_workload(n, seed=0, jitter=0.2):
     col = blob_colors(pos_gt, seed=seed + 1)
     pos = jitter_interior(pos_gt, cells, 1.0 / n, jitter, seed=seed + 2)
     return {"pos_gt": pos_gt, "pos": pos, "cells": cells, "col": col}
+
+
+def unlink_tets(cells, positions, colors):
+    """Every tet gets its own four vertices (TetMesh::unlinkTets, TetMesh.cpp:1950-1989): vertex 4t + i is the
+    i-th corner of tet t, with that corner's position and colour.  Returns (cells [T,4], positions [4T,3],
+    colors [4T,4])."""
+    cells = np.asarray(cells, np.int64).reshape(-1, 4)
+    flat = cells.reshape(-1)
+    return (np.arange(flat.size, dtype=np.int32).reshape(-1, 4), np.asarray(positions)[flat],
+            np.asarray(colors)[flat])
+
+
+def remove_transparent_tets(cells, positions, colors, alpha_threshold, cell_colors=None):
+    """Keeps the tets with a vertex (or, with cell_colors, the cell) whose alpha is >= alpha_threshold and the
+    vertices they use, in their original order (TetMesh::removeTransparentTets, TetMesh.cpp:1991-2079).  Returns
+    (cells, positions, colors, cell_colors) of the kept part (colors / cell_colors None where not given)."""
+    cells = np.asarray(cells, np.int64).reshape(-1, 4)
+    positions = np.asarray(positions)
+    if cell_colors is not None:
+        keep = np.asarray(cell_colors)[:, 3] >= alpha_threshold
+    else:
+        keep = (np.asarray(colors)[cells, 3] >= alpha_threshold).any(axis=1)
+    used = np.zeros(positions.shape[0], bool)
+    used[cells[keep].reshape(-1)] = True
+    new_index = np.cumsum(used) - 1
+    return (new_index[cells[keep]].astype(np.int32), positions[used],
+            None if colors is None else np.asarray(colors)[used],
+            None if cell_colors is None else np.asarray(cell_colors)[keep])

@@ -113,8 +113,8 @@ typedef struct dtv_adam_group {
     float* exp_avg;
     float* exp_avg_sq;
     int64_t numel;
-    float lr;
-    float beta1, beta2, eps;
+    double lr;               /* hyper-parameters in double, as torch.optim.Adam keeps them: the bias corrections */
+    double beta1, beta2, eps;/* are formed in double, the update's constants rounded once to float */
     int64_t step;            /* 1-based step count after this update */
     int32_t clamp_nonneg;    /* 1: param = max(param, 0) after the update */
     int32_t zero_grad;       /* 1: grad = 0 after it is consumed */

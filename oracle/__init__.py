@@ -51,11 +51,11 @@ def _load():
     lib.orc_forward_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
                                   c.c_int, _f32p, _f32p, _i32p]
     lib.orc_forward_d.argtypes = [c.c_void_p, _i32p, _f64p, _f64p, _f64p, c.c_double, _f64p, c.c_int, c.c_double,
-                                  c.c_int, _f64p, _f64p, _i32p]
+                                  c.c_int, c.c_int, c.c_void_p, _f64p, _f64p, _i32p]
     lib.orc_adjoint_exact_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
                                         c.c_int, _f32p, c.c_int, _f32p, _f32p]
     lib.orc_adjoint_exact_d.argtypes = [c.c_void_p, _i32p, _f64p, _f64p, _f64p, c.c_double, _f64p, c.c_int,
-                                        c.c_double, c.c_int, _f64p, c.c_int, _f64p, _f64p]
+                                        c.c_double, c.c_int, c.c_int, _f64p, c.c_int, _f64p, _f64p]
     lib.orc_adjoint_reference_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, _f32p, c.c_int,
                                             _f32p, _f32p]
     lib.orc_accumulate_linear_const_d.argtypes = [c.c_double, _f64p, _f64p, c.c_double, _f64p]
@@ -127,24 +127,29 @@ class Fragments:
 
 
 def forward(frags, positions, colors, cam, attenuation, clear=(0, 0, 0, 0), use_ert=False, ert_alpha=1.0 - 1e-4,
-            dtype=np.float32, linear_depth=False):
+            dtype=np.float32, linear_depth=False, f32_decisions=False, decision_positions=None):
     """Renders; returns (image[H,W,4], ray colour C[H,W,4] before the clear blend, fragment count[H,W]).
     linear_depth: segment lengths as differences of linear eye depths (the PPLL renderer's) instead of the
-    Euclidean lengths of the INTERSECTION renderer."""
+    Euclidean lengths of the INTERSECTION renderer.  f32_decisions (float64 only; cam = the float64 uniforms):
+    the ray-tet decisions of the FP32 evaluation, the continuous maths in FP64; decision_positions (float32
+    [V,3]) takes those decisions at other positions (decisions frozen, e.g. for differences along one branch)."""
     lib = _load()
     H, W = frags.height, frags.width
     img = np.zeros((H, W, 4), dtype)
     C = np.zeros((H, W, 4), dtype)
     cnt = np.zeros((H, W), np.int32)
+    dp = None if decision_positions is None else np.ascontiguousarray(decision_positions, np.float32).reshape(-1)
+    extra = () if dtype == np.float32 else (int(bool(f32_decisions)),
+                                            None if dp is None else dp.ctypes.data_as(ctypes.c_void_p))
     fn = lib.orc_forward_f if dtype == np.float32 else lib.orc_forward_d
     fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1),
        np.ascontiguousarray(colors, dtype).reshape(-1), np.ascontiguousarray(cam, dtype), attenuation,
-       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)), img, C, cnt)
+       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)), *extra, img, C, cnt)
     return img, C, cnt
 
 
 def adjoint_exact(frags, positions, colors, cam, attenuation, dimg, clear=(0, 0, 0, 0), use_ert=False,
-                  ert_alpha=1.0 - 1e-4, use_abs_grad=False, dtype=np.float32, linear_depth=False):
+                  ert_alpha=1.0 - 1e-4, use_abs_grad=False, dtype=np.float32, linear_depth=False, f32_decisions=False):
     """Exact gradient of sum(dimg * image) w.r.t. vertex positions [V,3] and colours [V,4]."""
     lib = _load()
     V = np.asarray(positions).reshape(-1, 3).shape[0]
@@ -154,6 +159,7 @@ def adjoint_exact(frags, positions, colors, cam, attenuation, dimg, clear=(0, 0,
     fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1),
        np.ascontiguousarray(colors, dtype).reshape(-1), np.ascontiguousarray(cam, dtype), attenuation,
        np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)),
+       *(() if dtype == np.float32 else (int(bool(f32_decisions)),)),
        np.ascontiguousarray(dimg, dtype).reshape(-1), int(use_abs_grad), gpos.reshape(-1), gcol.reshape(-1))
     return gpos, gcol
 

@@ -301,7 +301,9 @@ void orc_frags_free(orc_frags* fr) {
 #define RSQRT_ sqrtf
 #define REXP_ expf
 #define RABS_ fabsf
+#define IS_DOUBLE_ 0
 #include "oracle_cont.h"
+#undef IS_DOUBLE_
 #undef R
 #undef SFX
 #undef RSQRT_
@@ -313,33 +315,43 @@ void orc_frags_free(orc_frags* fr) {
 #define RSQRT_ sqrt
 #define REXP_ exp
 #define RABS_ fabs
+#define IS_DOUBLE_ 1
 #include "oracle_cont.h"
+#undef IS_DOUBLE_
 #undef R
 #undef SFX
 #undef RSQRT_
 #undef REXP_
 #undef RABS_
 
-/* linear_depth: 0 = INTERSECTION segment length (Euclidean), 1 = PPLL's (difference of linear eye depths). */
+/* linear_depth: 0 = INTERSECTION segment length (Euclidean), 1 = PPLL's (difference of linear eye depths).
+ * f32_decisions (double versions): the ray-tet decisions (hit, entry and exit faces, camera inside) are taken
+ * from the FP32 evaluation (FP32 camera and ray, positions rounded to float), the continuous maths in FP64: the
+ * FP64 counterpart of the FP32 function, free of the edge-grazing rays where FP32 and FP64 decide differently. */
 void orc_forward_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* camu,
                    float att, const float* clear, int use_ert, float ert_alpha, int linear_depth, float* img, float* C,
                    int32_t* cnt) {
-    forward_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, img, C, cnt);
+    forward_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, 0, NULL, img, C, cnt);
 }
+/* dec_pos (may be NULL, with f32_decisions): the float positions the decisions are taken at. */
 void orc_forward_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col, const double* camu,
-                   double att, const double* clear, int use_ert, double ert_alpha, int linear_depth, double* img,
-                   double* C, int32_t* cnt) {
-    forward_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, img, C, cnt);
+                   double att, const double* clear, int use_ert, double ert_alpha, int linear_depth, int f32_decisions,
+                   const float* dec_pos, double* img, double* C, int32_t* cnt) {
+    forward_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, f32_decisions, dec_pos, img, C,
+              cnt);
 }
 void orc_adjoint_exact_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* camu,
                          float att, const float* clear, int use_ert, float ert_alpha, int linear_depth, const float* dimg,
                          int use_abs, float* gpos, float* gcol) {
-    adjoint_exact_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, dimg, use_abs, gpos, gcol);
+    adjoint_exact_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, 0, dimg, use_abs, gpos,
+                    gcol);
 }
 void orc_adjoint_exact_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col,
                          const double* camu, double att, const double* clear, int use_ert, double ert_alpha,
-                         int linear_depth, const double* dimg, int use_abs, double* gpos, double* gcol) {
-    adjoint_exact_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, dimg, use_abs, gpos, gcol);
+                         int linear_depth, int f32_decisions, const double* dimg, int use_abs, double* gpos,
+                         double* gcol) {
+    adjoint_exact_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, f32_decisions, dimg,
+                    use_abs, gpos, gcol);
 }
 void orc_adjoint_reference_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col,
                              const float* camu, float att, const float* img, const float* dimg, int use_abs, float* gpos,

@@ -153,11 +153,29 @@ static R SFX(seg_scale)(const SFX(cam) * c, SFX(v3) rd, int linear_depth) {
 
 /* One fragment of the INTERSECTION forward: IntersectRasterization.glsl:146-241 (segment length scaled by
  * tscale, seg_scale above). */
+/* Ray parameter of the hit with face (p0, p1, p2): rayTriangleIntersect's t (RayIntersectionTests.glsl:30-76). */
+static R SFX(face_t)(SFX(v3) ro, SFX(v3) rd, SFX(v3) p0, SFX(v3) p1, SFX(v3) p2) {
+    SFX(v3) N = SFX(cross3)(SFX(sub3)(p1, p0), SFX(sub3)(p2, p0));
+    R d = -SFX(dot3)(N, p0);
+    return -(SFX(dot3)(N, ro) + d) / SFX(dot3)(N, rd);
+}
+
+/* dec (may be NULL): {hit, entry face, exit face, entry clamped to 0} decided elsewhere (the FP32 decisions of
+ * an FP64 evaluation); the distances are then this precision's t of those faces. */
 static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX(v4) C[4], R att, R tscale,
-                           SFX(frag) * F) {
+                           const int* dec, SFX(frag) * F) {
     memset(F, 0, sizeof(*F));
     F->tscale = tscale;
-    F->hit = SFX(ray_tet)(ro, rd, P, &F->f0, &F->f1, &F->t0, &F->t1);
+    if (dec) {
+        F->hit = dec[0];
+        if (!F->hit) return;
+        F->f0 = dec[1];
+        F->f1 = dec[2];
+        F->t0 = dec[3] ? (R)0 : SFX(face_t)(ro, rd, P[ORC_FT[F->f0][0]], P[ORC_FT[F->f0][1]], P[ORC_FT[F->f0][2]]);
+        F->t1 = SFX(face_t)(ro, rd, P[ORC_FT[F->f1][0]], P[ORC_FT[F->f1][1]], P[ORC_FT[F->f1][2]]);
+    } else {
+        F->hit = SFX(ray_tet)(ro, rd, P, &F->f0, &F->f1, &F->t0, &F->t1);
+    }
     if (!F->hit) return;
     F->P0 = SFX(add3)(ro, SFX(scl3)(rd, F->t0));
     F->P1 = SFX(add3)(ro, SFX(scl3)(rd, F->t1));
@@ -195,14 +213,43 @@ static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX
     }
 }
 
+#if IS_DOUBLE_
+/* The FP32 evaluation's ray-tet decisions for tet Pt seen along the FP32 ray (camf, rdf); with dec_pos (float
+ * positions [V*3], may be NULL) the tet's vertices are taken from there instead (decisions frozen at other
+ * positions, as for differences of the function along one branch of its discrete choices). */
+static const int* decide_f32(const cam_f* camf, v3_f rdf, const v3_d Pt[4], const float* dec_pos,
+                             const int32_t vid[4], int dec[4]) {
+    v3_f Pf[4];
+    for (int i = 0; i < 4; i++) {
+        if (dec_pos)
+            Pf[i] = mk3_f(dec_pos[3 * (int64_t)vid[i]], dec_pos[3 * (int64_t)vid[i] + 1], dec_pos[3 * (int64_t)vid[i] + 2]);
+        else
+            Pf[i] = mk3_f((float)Pt[i].x, (float)Pt[i].y, (float)Pt[i].z);
+    }
+    v3_f rof = mk3_f(camf->camPos[0], camf->camPos[1], camf->camPos[2]);
+    float t0, t1;
+    dec[0] = ray_tet_f(rof, rdf, Pf, &dec[1], &dec[2], &t0, &t1);
+    dec[3] = t0 == 0.0f;
+    return dec;
+}
+#endif
+
 /* Forward over the per-pixel fragment lists.  Front-to-back 'over' equals the reference's back-to-front
  * premultiplied blend (TetMeshRendererIntersection.cpp:258-276); the clear colour is blended last.
  * Early ray termination follows PPLL/Forward.glsl:168-173 (checked before each fragment). */
 static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* camu, R att,
-                         const R clear[4], int use_ert, R ert_alpha, int linear_depth, R* out_img, R* out_C,
-                         int32_t* out_cnt) {
+                         const R clear[4], int use_ert, R ert_alpha, int linear_depth, int f32_decisions,
+                         const float* dec_pos, R* out_img, R* out_C, int32_t* out_cnt) {
     SFX(cam) cam;
     SFX(cam_load)(&cam, camu, fr->W, fr->H);
+#if IS_DOUBLE_
+    float uff[68];
+    for (int i = 0; i < 68; i++) uff[i] = (float)camu[i];
+    cam_f camf;
+    cam_load_f(&camf, uff, fr->W, fr->H);
+#else
+    (void)f32_decisions;
+#endif
     SFX(v3) ro = SFX(mk3)(cam.camPos[0], cam.camPos[1], cam.camPos[2]);
     const int64_t P = (int64_t)fr->W * fr->H;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -210,6 +257,9 @@ static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos
         int x = (int)(px % fr->W), y = (int)(px / fr->W);
         SFX(v3) rd = SFX(ray_dir)(&cam, x, y);
         const R tscale = SFX(seg_scale)(&cam, rd, linear_depth);
+#if IS_DOUBLE_
+        const v3_f rdf = ray_dir_f(&camf, x, y);
+#endif
         SFX(v4) rc = SFX(mk4)(0, 0, 0, 0);
         int32_t cnt = 0;
         for (int64_t k = fr->off[px]; k < fr->off[px + 1]; k++) {
@@ -218,7 +268,15 @@ static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos
             SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
             SFX(load_tet)(cells, pos, col, id >> 2, Pt, Ct, vid);
             SFX(frag) F;
-            SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, &F);
+            int dec[4];
+            const int* decp = NULL;
+#if IS_DOUBLE_
+            if (f32_decisions) decp = decide_f32(&camf, rdf, Pt, dec_pos, vid, dec);
+#else
+            (void)dec;
+            (void)dec_pos;
+#endif
+            SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, decp, &F);
             cnt++;
             if (!F.hit) continue;
             SFX(v4) f = F.state[4];
@@ -370,10 +428,18 @@ static void SFX(scatter_tet_grads)(const int32_t vid[4], R gp[4][3], R gc[4][4],
 /* Exact adjoint over all pixels.  gpos/gcol are accumulated into (like the reference's .grad buffers,
  * TetMesh.cpp:243-249).  dimg is dL/d(blended image). */
 static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* camu, R att,
-                               const R clear[4], int use_ert, R ert_alpha, int linear_depth, const R* dimg, int use_abs,
-                               R* gpos, R* gcol) {
+                               const R clear[4], int use_ert, R ert_alpha, int linear_depth, int f32_decisions,
+                               const R* dimg, int use_abs, R* gpos, R* gcol) {
     SFX(cam) cam;
     SFX(cam_load)(&cam, camu, fr->W, fr->H);
+#if IS_DOUBLE_
+    float uff[68];
+    for (int i = 0; i < 68; i++) uff[i] = (float)camu[i];
+    cam_f camf;
+    cam_load_f(&camf, uff, fr->W, fr->H);
+#else
+    (void)f32_decisions;
+#endif
     SFX(v3) ro = SFX(mk3)(cam.camPos[0], cam.camPos[1], cam.camPos[2]);
     const int64_t P = (int64_t)fr->W * fr->H;
 #pragma omp parallel
@@ -388,6 +454,13 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
             int x = (int)(px % fr->W), y = (int)(px / fr->W);
             SFX(v3) rd = SFX(ray_dir)(&cam, x, y);
             const R tscale = SFX(seg_scale)(&cam, rd, linear_depth);
+            int dec[4];
+            const int* decp = NULL;
+#if IS_DOUBLE_
+            const v3_f rdf = ray_dir_f(&camf, x, y);
+#else
+            (void)dec;
+#endif
             /* forward, storing the composite state before every fragment */
             SFX(v4) rc = SFX(mk4)(0, 0, 0, 0);
             int64_t m = 0;
@@ -397,7 +470,10 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
                 SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
                 SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
                 SFX(frag) F;
-                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, &F);
+#if IS_DOUBLE_
+                if (f32_decisions) decp = decide_f32(&camf, rdf, Pt, NULL, vid, dec);
+#endif
+                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, decp, &F);
                 m = k + 1;
                 if (!F.hit) continue;
                 SFX(v4) f = F.state[4];
@@ -413,7 +489,10 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
                 SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
                 SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
                 SFX(frag) F;
-                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, &F);
+#if IS_DOUBLE_
+                if (f32_decisions) decp = decide_f32(&camf, rdf, Pt, NULL, vid, dec);
+#endif
+                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, decp, &F);
                 if (!F.hit) continue;
                 SFX(v4) ci = states[k];
                 SFX(v4) f = F.state[4];
@@ -514,7 +593,7 @@ static void SFX(adjoint_reference)(const orc_frags* fr, const int32_t* cells, co
             SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
             SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
             SFX(frag) F;
-            SFX(frag_eval)(ro, rd, Pt, Ct, att, (R)1.0, &F);
+            SFX(frag_eval)(ro, rd, Pt, Ct, att, (R)1.0, NULL, &F);
             if (!F.hit) continue;
             const int* a0 = ORC_FT[F.f0];
             const int* a1 = ORC_FT[F.f1];

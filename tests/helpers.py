@@ -37,3 +37,36 @@ def make_renderer(d, cells, pos, col, width, height, eye=(0.0, 0.0, 2.0), attenu
     r.set_view_matrix(view_from(eye))
     r.set_use_early_ray_termination(use_ert)
     return mesh, r
+
+
+def adjoint_references(oracle, frags, pos, col, cam_f, cam_d, att, dimg, **kw):
+    """The oracle's exact adjoint in FP32, and in FP64 with the FP32 evaluation's ray-tet decisions (the FP64
+    counterpart of the same function: no edge-grazing ray decides differently)."""
+    op, oc = oracle.adjoint_exact(frags, pos, col, cam_f, att, dimg, **kw)
+    mp, mc = oracle.adjoint_exact(frags, np.asarray(pos, np.float64), np.asarray(col, np.float64), cam_d, att,
+                                  np.asarray(dimg, np.float64), dtype=np.float64, f32_decisions=True, **kw)
+    return op, oc, mp, mc
+
+
+def check_gradients(tag, gp, gc, refs, tol=1e-4):
+    """Colours within tol relative L2 of the FP32 oracle's exact adjoint (the same FP32 function: same fragments,
+    same ray-tet decisions).  Positions: a vertex's position gradient sums large per-fragment terms that cancel
+    across the faces neighbouring tets share, so every FP32 evaluation sits at some distance from the FP64
+    gradient with the same decisions, the FP32 floor (the oracle's own is measured here: 1e-5 to 5e-2 over the
+    test workloads).  Two bars: the GPU within 1.25 times the floor of FP64 (tol when the floor is below it); and
+    the GPU within tol of the FP32 oracle, or within a quarter of the floor where that is larger.  Where the floor
+    is large (deep rays through fine Delaunay meshes) the oracle's own error dominates that distance: it forms
+    each fragment's input transmittance as 1 - alpha, as the reference does, which keeps only absolute
+    precision, so vertices whose exact gradient is ~1e-14 get ~1e-5 from it and ~1e-14 from the GPU, whose
+    transmittance is a product.  Returns the measured distances."""
+    op, oc, mp, mc = refs
+    d = {"col_vs_fp32": rel_l2(gc, oc), "col_vs_fp64": rel_l2(gc, mc), "pos_vs_fp32": rel_l2(gp, op),
+         "pos_vs_fp64": rel_l2(gp, mp), "fp32_pos_vs_fp64": rel_l2(op, mp), "fp32_col_vs_fp64": rel_l2(oc, mc)}
+    print(f"{tag}: rel L2 vs FP32 oracle: colours {d['col_vs_fp32']:.2e}, positions {d['pos_vs_fp32']:.2e}; "
+          f"vs FP64 (same decisions): colours {d['col_vs_fp64']:.2e}, positions {d['pos_vs_fp64']:.2e} "
+          f"(FP32 oracle: {d['fp32_pos_vs_fp64']:.2e})", flush=True)
+    assert np.isfinite(gp).all() and np.isfinite(gc).all()
+    assert d["col_vs_fp32"] <= tol
+    assert d["pos_vs_fp32"] <= max(tol, 0.25 * d["fp32_pos_vs_fp64"])
+    assert d["pos_vs_fp64"] <= max(tol, 1.25 * d["fp32_pos_vs_fp64"])
+    return d

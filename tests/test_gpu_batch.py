@@ -7,7 +7,7 @@ import pytest
 import torch
 
 import oracle
-from tests.helpers import FOVY, ZNEAR, ZFAR, grid_case, rel_l2, view_from
+from tests.helpers import FOVY, ZFAR, ZNEAR, adjoint_references, check_gradients, grid_case, rel_l2, view_from
 
 pytestmark = pytest.mark.gpu
 
@@ -126,16 +126,9 @@ def test_batch_two_meshes_adjoint_targets(gpu):
         uf, ud = oracle.camera_uniforms(view_from(e), FOVY, ZNEAR, ZFAR, W, H)
         fr = oracle.Fragments(w["cells"], w["pos"], uf, ZNEAR, W, H)
         # meshes with gradients render with early ray termination (the renderer's default)
-        op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 1.0, ones, use_ert=True)
-        dp, _ = oracle.adjoint_exact(fr, w["pos"].astype(np.float64), w["col"].astype(np.float64), ud, 1.0,
-                                     ones.astype(np.float64), use_ert=True, dtype=np.float64)
-        gp = m.get_vertex_positions().grad.cpu().numpy()
-        print(f"batch item adjoint vs FP32 oracle: positions {rel_l2(gp, op):.2e}, colours "
-              f"{rel_l2(m.get_vertex_colors().grad.cpu().numpy(), oc):.2e}", flush=True)
-        assert rel_l2(m.get_vertex_colors().grad.cpu().numpy(), oc) <= 1e-4
-        assert rel_l2(gp, op) <= 1e-4
-        # and the FP64 exact gradient, matched as closely as the FP32 oracle matches it
-        assert rel_l2(gp, dp) <= max(1e-4, 1.5 * rel_l2(op, dp))
+        check_gradients("batch item", m.get_vertex_positions().grad.cpu().numpy(),
+                        m.get_vertex_colors().grad.cpu().numpy(),
+                        adjoint_references(oracle, fr, w["pos"], w["col"], uf, ud, 1.0, ones, use_ert=True))
 
 
 def test_batch_validation(gpu):

@@ -18,7 +18,7 @@ import torch
 
 import oracle
 from difftetvr_amd import meshgen
-from tests.helpers import FOVY, ZFAR, ZNEAR, make_renderer, view_from
+from tests.helpers import FOVY, ZFAR, ZNEAR, adjoint_references, check_gradients, make_renderer, view_from
 
 pytestmark = pytest.mark.gpu
 
@@ -164,8 +164,7 @@ def test_fullsize_blob_field_rows(fullsize_blob):
 
 def test_fullsize_adjoint_rows(fullsize_blob):
     """The exact adjoint at full size with the image adjoint nonzero on the sampled rows only: the gradients then
-    sum exactly those rows' fragments, which the oracle's adjoint of the same rows gives.  Colours and positions
-    within 1e-4 relative L2 of the FP32 oracle."""
+    sum exactly those rows' fragments, which the oracle's adjoint of the same rows gives (check_gradients)."""
     w, mesh, r, img, uf, fr = fullsize_blob
     rng = np.random.default_rng(17)
     dimg = np.zeros((H, W, 4), np.float32)
@@ -177,10 +176,6 @@ def test_fullsize_adjoint_rows(fullsize_blob):
     torch.cuda.synchronize()
     gp = mesh.get_vertex_positions().grad.cpu().numpy()
     gc = mesh.get_vertex_colors().grad.cpu().numpy()
-    op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 1.0, dimg)
-    ep = float(np.linalg.norm(gp - op) / np.linalg.norm(op))
-    ec = float(np.linalg.norm(gc - oc) / np.linalg.norm(oc))
-    print(f"full size adjoint over {len(ROWS)} rows: rel L2 vs FP32 oracle: positions {ep:.2e}, colours {ec:.2e}",
-          flush=True)
-    assert ec <= 1e-4
-    assert ep <= 1e-4
+    _, ud = oracle.camera_uniforms(view_from(EYE), FOVY, ZNEAR, ZFAR, W, H)
+    check_gradients(f"full size adjoint over {len(ROWS)} rows", gp, gc,
+                    adjoint_references(oracle, fr, w["pos"], w["col"], uf, ud, 1.0, dimg))

@@ -10,7 +10,8 @@ import pytest
 import torch
 
 import oracle
-from tests.helpers import FOVY, ZFAR, ZNEAR, grid_case, make_renderer, rel_l2, view_from
+from tests.helpers import (FOVY, ZFAR, ZNEAR, adjoint_references, check_gradients, grid_case, make_renderer,
+                           rel_l2, view_from)
 
 pytestmark = pytest.mark.gpu
 
@@ -141,19 +142,8 @@ def test_grid16_adjoint_parity(gpu):
     gp = mesh.get_vertex_positions().grad.cpu().numpy()
     gc = mesh.get_vertex_colors().grad.cpu().numpy()
     fr, uf, ud = _oracle_frags(w["cells"], w["pos"], eye, W, H)
-    op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 1.0, dimg)
-    dp, dc = oracle.adjoint_exact(fr, w["pos"].astype(np.float64), w["col"].astype(np.float64), ud, 1.0,
-                                  dimg.astype(np.float64), dtype=np.float64)
-    ep, ec = rel_l2(gp, op), rel_l2(gc, oc)
-    ep64, ep_ref = rel_l2(gp, dp), rel_l2(op, dp)
-    print(f"adjoint rel L2 vs FP32 oracle: pos {ep:.3e} col {ec:.3e}; pos vs FP64: gpu {ep64:.3e}, "
-          f"FP32 oracle {ep_ref:.3e}")
-    assert ec <= GRAD_TOL
-    assert ep <= GRAD_TOL
-    # Position gradients sum large per-fragment terms that cancel across shared faces, so any FP32 evaluation
-    # (the reference's included) carries noise well above 1e-4 against the FP64 gradient; a second bar: the GPU
-    # matches the FP64 exact gradient as closely as the FP32 oracle does.
-    assert ep64 <= max(GRAD_TOL, 1.5 * ep_ref)
+    check_gradients("grid16 192^2 attenuation 1", gp, gc,
+                    adjoint_references(oracle, fr, w["pos"], w["col"], uf, ud, 1.0, dimg))
 
 
 def test_adjoint_accumulates_and_abs(gpu):
@@ -205,15 +195,15 @@ def test_reference_adjoint_mode(gpu, use_abs):
     W, H = 96, 80
     mesh, r = make_renderer(d, w["cells"], w["pos"], w["col"], W, H, eye=eye, attenuation=1.0)
     r.set_adjoint_mode("reference")
-    r.render()
+    img = r.render()
     dimg = np.random.default_rng(9).normal(size=(H, W, 4)).astype(np.float32)
     r.render_adjoint(torch.from_numpy(dimg).cuda(), use_abs)
     torch.cuda.synchronize()
     gp = mesh.get_vertex_positions().grad.cpu().numpy()
     gc = mesh.get_vertex_colors().grad.cpu().numpy()
     fr, uf, _ = _oracle_frags(w["cells"], w["pos"], eye, W, H)
-    oimg, _, _ = oracle.forward(fr, w["pos"], w["col"], uf, 1.0)
-    op, oc = oracle.adjoint_reference(fr, w["pos"], w["col"], uf, 1.0, oimg, dimg, use_abs_grad=use_abs)
+    # the reference adjoint starts from the rendered image (it inverts the blended output): the GPU's own
+    op, oc = oracle.adjoint_reference(fr, w["pos"], w["col"], uf, 1.0, img.cpu().numpy(), dimg, use_abs_grad=use_abs)
     _, ec = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 1.0, dimg, use_abs_grad=use_abs)
     print(f"reference adjoint rel L2: pos {rel_l2(gp, op):.3e} col {rel_l2(gc, oc):.3e}; "
           f"reference vs exact colour {rel_l2(oc, ec):.3e}", flush=True)
@@ -243,17 +233,14 @@ def test_adjoint_dense_field_no_early_termination(gpu, att, interval):
     torch.cuda.synchronize()
     gp = mesh.get_vertex_positions().grad.cpu().numpy()
     gc = mesh.get_vertex_colors().grad.cpu().numpy()
-    fr, uf, _ = _oracle_frags(w["cells"], w["pos"], eye, W, H)
+    fr, uf, ud = _oracle_frags(w["cells"], w["pos"], eye, W, H)
     oimg, _, ocnt = oracle.forward(fr, w["pos"], col, uf, att)
-    op, oc = oracle.adjoint_exact(fr, w["pos"], col, uf, att, dimg)
     a = img.cpu().numpy()[..., 3]
-    print(f"att={att} K={interval}: saturated pixels {(a == 1.0).sum()}, max fragments {ocnt.max()}, "
-          f"rel L2 vs FP32 oracle: pos {rel_l2(gp, op):.2e} col {rel_l2(gc, oc):.2e}", flush=True)
+    print(f"att={att} K={interval}: saturated pixels {(a == 1.0).sum()}, max fragments {ocnt.max()}", flush=True)
     assert (a == 1.0).sum() > 100
-    assert np.isfinite(gp).all() and np.isfinite(gc).all()
     assert float(np.abs(img.cpu().numpy() - oimg).max()) <= IMG_TOL
-    assert rel_l2(gc, oc) <= GRAD_TOL
-    assert rel_l2(gp, op) <= GRAD_TOL
+    check_gradients(f"dense field att={att} K={interval}", gp, gc,
+                    adjoint_references(oracle, fr, w["pos"], col, uf, ud, att, dimg))
 
 
 def test_ppll_renderer_linear_depth_parity(gpu):
@@ -280,20 +267,19 @@ def test_ppll_renderer_linear_depth_parity(gpu):
     dimg = np.random.default_rng(12).normal(size=(H, W, 4)).astype(np.float32)
     r.render_adjoint(torch.from_numpy(dimg).cuda(), False)
     torch.cuda.synchronize()
-    fr, uf, _ = _oracle_frags(w["cells"], w["pos"], eye, W, H)
+    fr, uf, ud = _oracle_frags(w["cells"], w["pos"], eye, W, H)
     oimg, _, ocnt = oracle.forward(fr, w["pos"], w["col"], uf, 2.0, linear_depth=True)
-    op, oc = oracle.adjoint_exact(fr, w["pos"], w["col"], uf, 2.0, dimg, linear_depth=True)
     gp = mesh.get_vertex_positions().grad.cpu().numpy()
     gc = mesh.get_vertex_colors().grad.cpu().numpy()
     err = float(np.abs(img.cpu().numpy() - oimg).max())
     iimg, _, _ = oracle.forward(fr, w["pos"], w["col"], uf, 2.0)
-    print(f"PPLL: max|image - oracle| {err:.2e} (vs the INTERSECTION image {np.abs(iimg - oimg).max():.2e}); "
-          f"adjoint rel L2 pos {rel_l2(gp, op):.2e} col {rel_l2(gc, oc):.2e}", flush=True)
+    print(f"PPLL: max|image - oracle| {err:.2e} (vs the INTERSECTION image {np.abs(iimg - oimg).max():.2e})",
+          flush=True)
     assert np.array_equal(r.get_fragment_counts().cpu().numpy(), ocnt)
     assert err <= IMG_TOL
     assert float(np.abs(iimg - oimg).max()) > 1e-3
-    assert rel_l2(gc, oc) <= GRAD_TOL
-    assert rel_l2(gp, op) <= GRAD_TOL
+    check_gradients("PPLL grid12", gp, gc,
+                    adjoint_references(oracle, fr, w["pos"], w["col"], uf, ud, 2.0, dimg, linear_depth=True))
     r.set_adjoint_mode("reference")
     with pytest.raises(RuntimeError, match="Euclidean"):
         r.render_adjoint(torch.from_numpy(dimg).cuda(), False)

@@ -144,3 +144,35 @@ def test_projection_renderer_type_is_refused():
     import difftetvr_amd as d
     with pytest.raises(RuntimeError, match="PROJECTION"):
         d.Renderer(d.RendererType.PROJECTION)
+
+
+def test_unlink_tets():
+    from difftetvr_amd import meshgen
+    w = meshgen.grid_workload(3, seed=4)
+    cells, pos, col = meshgen.unlink_tets(w["cells"], w["pos"], w["col"])
+    assert cells.shape == w["cells"].shape and pos.shape == (cells.size, 3) and col.shape == (cells.size, 4)
+    assert np.array_equal(cells.ravel(), np.arange(cells.size))
+    # every tet keeps its corner positions and colours, in order
+    assert np.array_equal(pos[cells], w["pos"][w["cells"]])
+    assert np.array_equal(col[cells], w["col"][w["cells"]])
+
+
+def test_remove_transparent_tets():
+    from difftetvr_amd import meshgen
+    w = meshgen.grid_workload(4, seed=6)
+    col = w["col"].copy()
+    thr = float(np.quantile(col[:, 3], 0.7))
+    cells, pos, col2, _ = meshgen.remove_transparent_tets(w["cells"], w["pos"], col, thr)
+    keep = [t for t in range(w["cells"].shape[0]) if (col[w["cells"][t], 3] >= thr).any()]
+    assert 0 < len(keep) < w["cells"].shape[0]
+    assert cells.shape == (len(keep), 4)
+    # the kept tets in order, with their corners' positions and colours
+    assert np.array_equal(pos[cells], w["pos"][w["cells"][keep]])
+    assert np.array_equal(col2[cells], col[w["cells"][keep]])
+    # every kept vertex is used, in the original vertex order
+    used = np.unique(w["cells"][keep])
+    assert np.array_equal(pos, w["pos"][used])
+    # cell colours decide by the cell's own alpha
+    cc = np.random.default_rng(1).uniform(size=(w["cells"].shape[0], 4)).astype(np.float32)
+    c3, _, _, cc2 = meshgen.remove_transparent_tets(w["cells"], w["pos"], None, 0.5, cell_colors=cc)
+    assert np.array_equal(cc2, cc[cc[:, 3] >= 0.5]) and c3.shape[0] == cc2.shape[0]
