@@ -82,6 +82,8 @@ SIGNATURES = [
                                   ctypes.POINTER(DtvRenderParams), c_void_p, c_void_p]),
     ("dtv_render_adjoint", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_void_p, c_int32, c_int32, c_void_p,
                                           c_void_p, c_void_p]),
+    ("dtv_render_noalloc", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), ctypes.POINTER(DtvCamera),
+                                          ctypes.POINTER(DtvRenderParams), c_void_p, c_void_p, c_void_p]),
     ("dtv_render_batch", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvRenderItem), c_int32,
                                         ctypes.POINTER(DtvRenderParams), c_void_p]),
     ("dtv_render_adjoint_batch", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvAdjointItem), c_int32, c_int32, c_int32,

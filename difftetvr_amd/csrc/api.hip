@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -134,6 +135,43 @@ bool camera_uniforms(const dtv_camera& cam, double u[68]) {
 }
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+// ---- allocation-free rendering (dtv_render_noalloc) ---------------------------------------------------------
+// Every intermediate buffer of dtv_render_noalloc lives in this one array, which each GPU the library runs on
+// holds until the process ends.  Limits: kNaMaxTets tets, kNaMaxPixels pixels (kNaMaxTiles 8x8 tiles) and
+// kNaMaxRecords (tile, face) records per render.  Every call on a GPU reuses that GPU's copy (the buffers the
+// pipeline needs cleared are cleared by its own fill kernels), so calls that overlap on one GPU race.
+constexpr int64_t kNaMaxTets = 1 << 18;          // 262,144
+constexpr int64_t kNaMaxPixels = 1 << 18;        // e.g. 512 x 512
+constexpr int64_t kNaMaxTiles = 1 << 13;         // 8,192
+constexpr int64_t kNaMaxRecords = 3 << 18;       // 786,432
+constexpr size_t kNaScratchBytes = 56ull << 20;  // 56 MiB, >= the layout below at the limits (checked per call)
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each device (set up once per device)
+
+// Must run with `device` current.  The host-side name `scratch` is not a device address: kernels and copies
+// use the address looked up here.
+int setup_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(DTV_ERR_INVALID_ARGUMENT, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return DTV_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        return fail(DTV_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return DTV_OK;
+}
+
+__global__ void k_store_item(BatchItem item, BatchItem* __restrict__ dst) { *dst = item; }
+__global__ void k_copy_flag(const uint32_t* __restrict__ src, int32_t* __restrict__ dst) { *dst = (int32_t)*src; }
 
 }  // namespace
 
@@ -500,6 +538,8 @@ int dtv_ctx_create(int32_t device, dtv_ctx** out_ctx) {
     DTV_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_ctx_create: invalid device index");
     DTV_HIP(hipSetDevice(device));
+    const int rc = setup_noalloc_scratch(device);
+    if (rc != DTV_OK) return rc;
     dtv_ctx* c = new dtv_ctx();
     c->device = device;
     *out_ctx = c;
@@ -569,6 +609,112 @@ int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, c
     it.camera = *camera;
     it.out_image = out_image;
     return render_batch(view, &it, 1, params, reinterpret_cast<hipStream_t>(stream), "dtv_render");
+}
+
+int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* camera,
+                       const dtv_render_params* params, float* out_image, int32_t* out_overflow, void* stream) {
+    const std::string f = "dtv_render_noalloc";
+    if (!ctx || !mesh || !camera || !params || !out_image) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
+    const int tile = params->tile_size == 0 ? 8 : params->tile_size;
+    if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_size must be 8");
+    if (params->tile_order != 0 && params->tile_order != 1)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_order must be 0 (heaviest first) or 1 (row-major)");
+    if (params->segment_length != DTV_SEGMENT_EUCLIDEAN && params->segment_length != DTV_SEGMENT_LINEAR_DEPTH)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
+    if (!valid_camera(*camera)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid viewport, fovy or clip planes");
+    const dtv_mesh& m = *mesh;
+    if (m.num_cells < 0 || m.num_vertices < 0) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid mesh size");
+    if (m.num_cells > 0 && (!m.cells || !m.positions || !m.colors))
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL mesh buffer");
+    if (!aligned16(m.cells) || !aligned16(m.colors) || !aligned16(out_image))
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": cells, colors and out_image must be 16-byte aligned");
+    const int64_t P = (int64_t)camera->width * camera->height;
+    const int64_t tiles = (int64_t)((camera->width + 7) / 8) * ((camera->height + 7) / 8);
+    if (m.num_cells > kNaMaxTets || P > kNaMaxPixels || tiles > kNaMaxTiles)
+        return fail(DTV_ERR_INVALID_ARGUMENT,
+                    f + ": beyond the allocation-free limits (262,144 tets, 262,144 pixels, 8,192 8x8 tiles); "
+                        "use dtv_render");
+    BatchItem B{};
+    // forward only: no checkpoint slots (the forward stores none; the render cannot be differentiated)
+    dtv_render_params prm = *params;
+    if (!make_uniforms(*camera, prm, B.U)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": singular view matrix");
+    B.U.max_ckpt = 0;
+    B.cells = m.cells;
+    B.pos = m.positions;
+    B.col = m.colors;
+    B.image = out_image;
+    B.T = m.num_cells;
+    B.block0 = 0;
+    B.tile0 = 0;
+    B.pixel0 = 0;
+    const int64_t blocks = count_pairs_blocks(m.num_cells);
+    const int64_t nslots = 256 * (blocks > 0 ? blocks : 1);
+    const int64_t K = kNaMaxRecords;
+    int tile_bits = 0;
+    while ((1ll << tile_bits) < tiles) tile_bits++;
+    uint32_t mask = 0xFu;  // every depth digit: which of them vary is known only on the device
+    for (int p = 0; 8 * p < tile_bits; p++) mask |= 1u << (4 + p);
+    int npass = 0;
+    for (int p = 0; p < 8; p++) npass += (mask >> p) & 1;
+    // carve the scratch (256-byte aligned pieces)
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_item = take(sizeof(BatchItem)), o_small = take(384), o_counts = take(4 * (size_t)nslots),
+                 o_offsets = take(4 * (size_t)nslots), o_status = take(8 * (size_t)(scan_partitions(nslots) + 1)),
+                 o_bstats = take(16 * (size_t)(blocks + 1)), o_keys = take(8 * (size_t)K), o_keys2 = take(8 * (size_t)K),
+                 o_vals = take(4 * (size_t)K), o_vals2 = take(4 * (size_t)K), o_recs = take(32 * (size_t)K),
+                 o_hist = take(4 * 8 * 256 + 64), o_lookback = take(4 * 256 * (size_t)npass * (sort_partitions(K) + 1)),
+                 o_ranges = take(8 * (size_t)tiles), o_tfrags = take(4 * (size_t)tiles), o_order = take(4 * (size_t)tiles),
+                 o_color = take(16 * (size_t)P), o_count = take(4 * (size_t)P), o_last = take(4 * (size_t)P),
+                 o_nck = take(4 * (size_t)P);
+    if (off > kNaScratchBytes) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": scratch layout exceeds 56 MiB");
+    DTV_HIP(hipSetDevice(ctx->device));
+    char* base = static_cast<char*>(g_scratch_ptr[ctx->device]);
+    if (!base) return fail(DTV_ERR_HIP, f + ": scratch not set up on this device");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    BatchItem* dI = reinterpret_cast<BatchItem*>(base + o_item);
+    char* small = base + o_small;
+    uint32_t* scan_counter = reinterpret_cast<uint32_t*>(small + 64);
+    uint64_t* scan_total = reinterpret_cast<uint64_t*>(small + 128);
+    uint32_t* overflow = reinterpret_cast<uint32_t*>(small + 192);
+    SortRoute* route = reinterpret_cast<SortRoute*>(small + 320);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + o_counts);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(base + o_hist);
+    uint2* ranges = reinterpret_cast<uint2*>(base + o_ranges);
+    uint32_t* order = params->tile_order == 0 ? reinterpret_cast<uint32_t*>(base + o_order) : nullptr;
+    uint4* recs = reinterpret_cast<uint4*>(base + o_recs);
+    // the item reaches the device as a kernel argument: no host staging
+    hipLaunchKernelGGL(k_store_item, dim3(1), dim3(1), 0, s, B, dI);
+    launch_count_pairs(dI, 1, blocks, counts, reinterpret_cast<uint32_t*>(base + o_bstats), s);
+    scan_exclusive_u32(counts, reinterpret_cast<uint32_t*>(base + o_offsets), 256 * blocks,
+                       reinterpret_cast<uint64_t*>(base + o_status), scan_counter, scan_total, s);
+    fill_u32(overflow, 0u, 1, s);
+    launch_emit_pairs(dI, 1, blocks, reinterpret_cast<uint32_t*>(base + o_offsets), counts,
+                      reinterpret_cast<uint64_t*>(base + o_keys), reinterpret_cast<uint32_t*>(base + o_vals), K,
+                      overflow, s);
+    radix_sort_pairs(reinterpret_cast<uint64_t*>(base + o_keys), reinterpret_cast<uint32_t*>(base + o_vals),
+                     reinterpret_cast<uint64_t*>(base + o_keys2), reinterpret_cast<uint32_t*>(base + o_vals2), K,
+                     scan_total, mask, hist, reinterpret_cast<uint32_t*>(base + o_lookback), hist + 8 * 256, route, s);
+    launch_tile_ranges(route, K, scan_total, tiles, ranges, s);
+    if (order) launch_tile_order(ranges, nullptr, tiles, order, s);
+    launch_gather_records(dI, 1, route, K, scan_total, recs, s);
+    PixelState ps;
+    ps.ray_color = reinterpret_cast<float4*>(base + o_color);
+    ps.frag_count = reinterpret_cast<uint32_t*>(base + o_count);
+    ps.last_rec = reinterpret_cast<uint32_t*>(base + o_last);
+    ps.num_ckpt = reinterpret_cast<uint32_t*>(base + o_nck);
+    ps.ckpt_t = nullptr;  // max_ckpt = 0: never written
+    ps.ckpt_n = nullptr;
+    ps.counters = nullptr;
+    ps.tile_frags = reinterpret_cast<uint32_t*>(base + o_tfrags);
+    launch_render_forward(dI, 1, tiles, order, recs, ranges, ps, s);
+    if (out_overflow) hipLaunchKernelGGL(k_copy_flag, dim3(1), dim3(1), 0, s, overflow, out_overflow);
+    DTV_HIP(hipGetLastError());
+    return DTV_OK;
 }
 
 int dtv_render_batch(dtv_view* view, const dtv_render_item* items, int32_t num_items,

@@ -143,6 +143,18 @@ int dtv_camera_uniforms(const dtv_camera* camera, float* out68);
 int dtv_render(dtv_view* view, const dtv_mesh* mesh, const dtv_camera* camera, const dtv_render_params* params,
                float* out_image, void* stream);
 
+/* Forward render of one (mesh, camera) into out_image, allocating nothing: every intermediate buffer lives in one
+ * static device array of 56 MiB per GPU (held until the process ends), whose limits are 262,144 tets, 262,144
+ * pixels (8,192 8x8 tiles) and 786,432 (tile, face) records; sizes beyond them fail with DTV_ERR_INVALID_ARGUMENT.
+ * No host synchronisation and no host staging (capturable in a HIP graph).  *out_overflow (device int32, may be
+ * NULL) receives 1 when the render produced more records than the limit (that image is incomplete), else 0.
+ * The image equals dtv_render's.  Forward only: it keeps no state for an adjoint.  Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other streams or threads) race.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (HIP graph capture, serving loops, harnesses
+ * that forbid allocation). */
+int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* camera, const dtv_render_params* params,
+                       float* out_image, int32_t* out_overflow, void* stream);
+
 /* Renders num_items (1..4096) (mesh, camera) items as one batch: one binning grid over all their tets, one sort of
  * all (tile, face) records and one tile-renderer launch over all their tiles, so that many small or uneven views
  * fill the GPU together.  Each image equals what dtv_render gives for its item alone.  No reference counterpart:
