@@ -146,7 +146,7 @@ class TetMesh:
         self._use_gradients = False
         self._cells = None
         self._positions = None
-        self._colors = None
+        self._colors = None       # [V,4] (PER_VERTEX) or [T,4] (PER_CELL)
         self._boundary = None
         self._color_storage = ColorStorage.PER_VERTEX
 
@@ -159,24 +159,38 @@ class TetMesh:
     def get_use_gradients(self):
         return self._use_gradients
 
-    def set_tet_mesh_data(self, cells, vertex_positions, vertex_colors):
-        """Sets cells [T,4] (int), positions [V,3] and per-vertex colours [V,4] (array-likes or tensors)."""
+    def set_tet_mesh_data(self, cells, vertex_positions, vertex_colors=None, cell_colors=None):
+        """Sets cells [T,4] (int), positions [V,3] and either per-vertex colours [V,4] (ColorStorage.PER_VERTEX)
+        or per-cell colours [T,4] (ColorStorage.PER_CELL) (array-likes or tensors)."""
         dev = _device()
+        if (vertex_colors is None) == (cell_colors is None):
+            raise RuntimeError("set_tet_mesh_data: give exactly one of vertex_colors and cell_colors.")
         cells = torch.as_tensor(np.asarray(cells) if not torch.is_tensor(cells) else cells)
         cells = cells.to(device=dev, dtype=torch.int32).reshape(-1, 4).contiguous()
         pos = torch.as_tensor(vertex_positions if torch.is_tensor(vertex_positions) else np.asarray(vertex_positions))
         pos = pos.to(device=dev, dtype=torch.float32).reshape(-1, 3).contiguous()
-        col = torch.as_tensor(vertex_colors if torch.is_tensor(vertex_colors) else np.asarray(vertex_colors))
+        colors = vertex_colors if cell_colors is None else cell_colors
+        col = torch.as_tensor(colors if torch.is_tensor(colors) else np.asarray(colors))
         col = col.to(device=dev, dtype=torch.float32).reshape(-1, 4).contiguous()
-        if col.shape[0] != pos.shape[0]:
-            raise RuntimeError("set_tet_mesh_data: colours must be per vertex ([V,4]).")
+        if cell_colors is None and col.shape[0] != pos.shape[0]:
+            raise RuntimeError("set_tet_mesh_data: vertex colours must be [V,4].")
+        if cell_colors is not None and col.shape[0] != cells.shape[0]:
+            raise RuntimeError("set_tet_mesh_data: cell colours must be [T,4].")
         if cells.numel() and (int(cells.min()) < 0 or int(cells.max()) >= pos.shape[0]):
             raise RuntimeError("set_tet_mesh_data: cell index out of range.")
         self._cells = cells
         self._positions = pos.detach().clone()
         self._colors = col.detach().clone()
+        self._color_storage = ColorStorage.PER_VERTEX if cell_colors is None else ColorStorage.PER_CELL
         self._boundary = None
         self._setup_grads()
+
+    def _set_like(self, cells, pos, col):
+        """set_tet_mesh_data with this mesh's colour storage."""
+        if self._color_storage == ColorStorage.PER_CELL:
+            self.set_tet_mesh_data(cells, pos, cell_colors=col)
+        else:
+            self.set_tet_mesh_data(cells, pos, col)
 
     def _setup_grads(self):
         if self._use_gradients:
@@ -188,14 +202,15 @@ class TetMesh:
                 self._colors.grad = torch.zeros_like(self._colors)
 
     def set_hex_mesh_const(self, aabb, xs, ys, zs, const_color, color_storage=ColorStorage.PER_VERTEX):
-        """Tetrahedralised hex grid with xs*ys*zs vertices and a constant colour (TetMesh.cpp:1882-1936)."""
-        if color_storage != ColorStorage.PER_VERTEX:
-            raise RuntimeError("set_hex_mesh_const: only ColorStorage.PER_VERTEX is supported.")
+        """Tetrahedralised hex grid with xs*ys*zs vertices and a constant colour per vertex or per cell
+        (TetMesh.cpp:1882-1936)."""
         pos, cells = meshgen.hex_grid((aabb.min.x, aabb.min.y, aabb.min.z), (aabb.max.x, aabb.max.y, aabb.max.z),
                                       int(xs), int(ys), int(zs))
-        col = np.tile(np.array([const_color.x, const_color.y, const_color.z, const_color.w], np.float32),
-                      (pos.shape[0], 1))
-        self.set_tet_mesh_data(cells, pos, col)
+        c = np.array([const_color.x, const_color.y, const_color.z, const_color.w], np.float32)
+        if ColorStorage(color_storage) == ColorStorage.PER_CELL:
+            self.set_tet_mesh_data(cells, pos, cell_colors=np.tile(c, (cells.shape[0], 1)))
+        else:
+            self.set_tet_mesh_data(cells, pos, np.tile(c, (pos.shape[0], 1)))
 
     def load_test_data(self, test_case):
         """Built-in meshes (TetMesh::loadTestData, TetMesh.cpp:1016-1077)."""
@@ -207,14 +222,16 @@ class TetMesh:
 
     def load_from_file(self, file_path):
         from . import bintet
-        cells, pos, col = bintet.read(file_path)
-        self.set_tet_mesh_data(cells, pos, col)
+        cells, pos, col, cell_col = bintet.read(file_path)
+        self.set_tet_mesh_data(cells, pos, col, cell_col)
         return True
 
     def save_to_file(self, file_path):
         from . import bintet
+        col = self._colors.detach().cpu().numpy()
+        per_cell = self._color_storage == ColorStorage.PER_CELL
         bintet.write(file_path, self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(),
-                     self._colors.detach().cpu().numpy())
+                     None if per_cell else col, col if per_cell else None)
         return True
 
     # -- queries ------------------------------------------------------------------------------------------
@@ -236,11 +253,23 @@ class TetMesh:
         return self._positions
 
     def get_vertex_colors(self):
+        """float32 [V,4] (TetMesh::getVertexColorTensor, TetMesh.cpp:271-308)."""
         self._require()
+        if self._color_storage != ColorStorage.PER_VERTEX:
+            raise RuntimeError("Error in TetMesh::getVertexColorTensor: No vertex color data available.")
         return self._colors
 
     def get_cell_colors(self):
-        raise RuntimeError("get_cell_colors: the mesh uses ColorStorage.PER_VERTEX.")
+        """float32 [T,4] (TetMesh::getCellColorTensor, TetMesh.cpp:310-345)."""
+        self._require()
+        if self._color_storage != ColorStorage.PER_CELL:
+            raise RuntimeError("Error in TetMesh::getCellColorTensor: No cell color data available.")
+        return self._colors
+
+    def _color_tensor(self):
+        """The colour tensor of either storage (its .grad is the adjoint's colour target)."""
+        self._require()
+        return self._colors
 
     def get_cell_indices(self):
         self._require()
@@ -274,18 +303,21 @@ class TetMesh:
         """Gives every tet its own four vertices, undoing the shared-index representation (TetMesh.cpp:1950-1989).
         The mesh tensors are new afterwards."""
         self._require()
+        per_cell = self._color_storage == ColorStorage.PER_CELL
         cells, pos, col = meshgen.unlink_tets(self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(),
-                                              self._colors.detach().cpu().numpy())
-        self.set_tet_mesh_data(cells, pos, col)
+                                              None if per_cell else self._colors.detach().cpu().numpy())
+        self._set_like(cells, pos, self._colors.detach().cpu().numpy() if per_cell else col)
 
     def remove_transparent_tets(self, alpha_threshold):
         """Removes the tets none of whose vertices has alpha >= alpha_threshold, and the vertices no remaining tet
         uses (TetMesh.cpp:1991-2079).  The mesh tensors are new afterwards."""
         self._require()
-        cells, pos, col, _ = meshgen.remove_transparent_tets(
-            self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(), self._colors.detach().cpu().numpy(),
-            float(alpha_threshold))
-        self.set_tet_mesh_data(cells, pos, col)
+        per_cell = self._color_storage == ColorStorage.PER_CELL
+        col = self._colors.detach().cpu().numpy()
+        cells, pos, vcol, ccol = meshgen.remove_transparent_tets(
+            self._cells.cpu().numpy(), self._positions.detach().cpu().numpy(), None if per_cell else col,
+            float(alpha_threshold), cell_colors=col if per_cell else None)
+        self._set_like(cells, pos, ccol if per_cell else vcol)
 
     def set_force_use_ovm_representation(self):
         """Accepted for compatibility (Module.cpp:794): refinement works on the plain cell array."""
@@ -297,6 +329,9 @@ class TetMesh:
         does) and hands the mesh to ``volume_renderer``."""
         from . import refine
         self._require()
+        if self._color_storage != ColorStorage.PER_VERTEX:
+            raise RuntimeError("split_by_largest_gradient_magnitudes: subdivision interpolates vertex colours; "
+                               "the mesh uses ColorStorage.PER_CELL.")
         kind = SplitGradientType(split_gradient_type)
         src = self._positions if kind in (SplitGradientType.POSITION, SplitGradientType.ABS_POSITION) else self._colors
         if src.grad is None:
@@ -318,8 +353,10 @@ class TetMesh:
 
     def _desc(self):
         self._require()
-        return _lib.DtvMesh(self._cells.data_ptr(), self._positions.data_ptr(), self._colors.data_ptr(),
-                            self._cells.shape[0], self._positions.shape[0])
+        per_cell = self._color_storage == ColorStorage.PER_CELL
+        return _lib.DtvMesh(self._cells.data_ptr(), self._positions.data_ptr(),
+                            None if per_cell else self._colors.data_ptr(), self._cells.shape[0],
+                            self._positions.shape[0], self._colors.data_ptr() if per_cell else None)
 
 
 class Renderer:
@@ -498,7 +535,7 @@ class Renderer:
             raise RuntimeError("Error in render_adjoint: render() must be called first.")
         image_adjoint = image_adjoint.contiguous()
         mesh = self._mesh
-        pos, col = mesh.get_vertex_positions(), mesh.get_vertex_colors()
+        pos, col = mesh.get_vertex_positions(), mesh._color_tensor()
         if pos.grad is None:
             pos.grad = torch.zeros_like(pos)
         if col.grad is None:
@@ -641,7 +678,7 @@ class BatchRenderer(Renderer):
             if d.data_ptr() % 16:
                 d = d.clone()
             keep.append(d)
-            pos, col = mesh.get_vertex_positions(), mesh.get_vertex_colors()
+            pos, col = mesh.get_vertex_positions(), mesh._color_tensor()
             if pos.grad is None:
                 pos.grad = torch.zeros_like(pos)
             if col.grad is None:

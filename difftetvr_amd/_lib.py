@@ -24,7 +24,7 @@ c_void_p = ctypes.c_void_p
 
 class DtvMesh(ctypes.Structure):
     _fields_ = [("cells", c_void_p), ("positions", c_void_p), ("colors", c_void_p),
-                ("num_cells", c_int64), ("num_vertices", c_int64)]
+                ("num_cells", c_int64), ("num_vertices", c_int64), ("cell_colors", c_void_p)]
 
 
 class DtvCamera(ctypes.Structure):

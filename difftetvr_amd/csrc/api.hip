@@ -136,6 +136,20 @@ bool camera_uniforms(const dtv_camera& cam, double u[68]) {
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+// Sizes, buffers and alignment of a mesh; exactly one colour storage (per vertex or per cell).
+int check_mesh(const dtv_mesh& m, const std::string& f) {
+    if (m.num_cells < 0 || m.num_vertices < 0 || m.num_cells > (int64_t)(1u << 28))
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid mesh size");
+    if ((m.colors != nullptr) == (m.cell_colors != nullptr))
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": set exactly one of colors (per vertex) and cell_colors (per cell)");
+    if (m.num_cells > 0 && (!m.cells || !m.positions)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL mesh buffer");
+    if (!aligned16(m.cells) || !aligned16(m.colors) || !aligned16(m.cell_colors))
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": cells and colours must be 16-byte aligned");
+    return DTV_OK;
+}
+
+uint32_t colour_kind(const dtv_mesh& m) { return m.cell_colors ? kColoursPerCell : kColoursPerVertex; }
+
 }  // namespace
 
 // ---- allocation-free rendering (dtv_render_noalloc) ---------------------------------------------------------
@@ -184,6 +198,7 @@ struct dtv_view {
     // the batch of the last render (host copy; the device copy is dItems)
     std::vector<BatchItem> items;
     std::vector<int64_t> item_V;
+    uint32_t colour_kinds = 0;             // colour storages of the batch's items (kColoursPerVertex | PerCell)
     BatchItem* pinnedItems = nullptr;
     int pinned_items_cap = 0;
     hipEvent_t items_copied = nullptr;     // the last upload out of pinnedItems
@@ -287,6 +302,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     std::vector<BatchItem> items((size_t)n);
     std::vector<int64_t> item_V((size_t)n);
     int64_t blocks = 0, tiles = 0, pixels = 0;
+    uint32_t kinds = 0;
     for (int i = 0; i < n; i++) {
         const dtv_render_item& it = in[i];
         const dtv_mesh& m = it.mesh;
@@ -296,17 +312,15 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                 return fail(DTV_ERR_INVALID_ARGUMENT, f + ": viewport must be 1..16384 pixels per side");
             return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid fovy / clip planes");
         }
-        if (m.num_cells < 0 || m.num_vertices < 0 || m.num_cells > (int64_t)(1u << 29))
-            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid mesh size");
-        if (m.num_cells > 0 && (!m.cells || !m.positions || !m.colors))
-            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL mesh buffer");
-        if (!aligned16(m.cells) || !aligned16(m.colors) || !aligned16(it.out_image))
-            return fail(DTV_ERR_INVALID_ARGUMENT, f + ": cells, colors and out_image must be 16-byte aligned");
+        if (const int rc = check_mesh(m, f); rc != DTV_OK) return rc;
+        if (!aligned16(it.out_image)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": out_image must be 16-byte aligned");
         BatchItem& B = items[(size_t)i];
         if (!make_uniforms(it.camera, *params, B.U)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": singular view matrix");
         B.cells = m.cells;
         B.pos = m.positions;
         B.col = m.colors;
+        B.cell_col = m.cell_colors;
+        kinds |= colour_kind(m);
         B.image = it.out_image;
         B.T = m.num_cells;
         B.block0 = (uint32_t)blocks;
@@ -442,9 +456,10 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * pixels * kMaxCheckpoints));
     view->mark(DTV_STAGE_FORWARD, 0, s);
     launch_render_forward(dI, n, tiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
-                          view->pixel_state(), s);
+                          view->pixel_state(), kinds, s);
     view->mark(DTV_STAGE_FORWARD, 1, s);
     DTV_HIP(hipGetLastError());
+    view->colour_kinds = kinds;
     view->items.swap(items);
     view->item_V.swap(item_V);
     view->total_blocks = blocks;
@@ -465,6 +480,10 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         return fail(DTV_ERR_INVALID_ARGUMENT,
                     f + ": the reference adjoint mode restates the INTERSECTION renderer's adjoint and needs "
                         "Euclidean segment lengths");
+    if (adjoint_mode == DTV_ADJOINT_REFERENCE && (view->colour_kinds & kColoursPerCell))
+        return fail(DTV_ERR_INVALID_ARGUMENT,
+                    f + ": the reference adjoint mode restates the per-vertex colour adjoint; per-cell colours "
+                        "take the exact adjoint");
     for (int i = 0; i < n; i++) {
         if (!tg[i].dimage) continue;  // item not differentiated
         if (!tg[i].grad_pos || !tg[i].grad_col) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
@@ -510,7 +529,7 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     view->mark(DTV_STAGE_ADJOINT, 0, s);
     launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
                           adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps, use_abs_grad ? 1 : 0,
-                          adjoint_mode, s);
+                          adjoint_mode, view->colour_kinds, s);
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;
@@ -520,8 +539,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
 // the vertex count: the caller's mesh must be that one, unchanged in its buffers and sizes.
 bool same_mesh(const dtv_view* view, int i, const dtv_mesh& m) {
     const BatchItem& B = view->items[(size_t)i];
-    return m.cells == B.cells && m.positions == B.pos && m.colors == B.col && m.num_cells == B.T &&
-           m.num_vertices == view->item_V[(size_t)i];
+    return m.cells == B.cells && m.positions == B.pos && m.colors == B.col && m.cell_colors == B.cell_col &&
+           m.num_cells == B.T && m.num_vertices == view->item_V[(size_t)i];
 }
 
 }  // namespace
@@ -623,11 +642,8 @@ int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* cam
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
     if (!valid_camera(*camera)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid viewport, fovy or clip planes");
     const dtv_mesh& m = *mesh;
-    if (m.num_cells < 0 || m.num_vertices < 0) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid mesh size");
-    if (m.num_cells > 0 && (!m.cells || !m.positions || !m.colors))
-        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL mesh buffer");
-    if (!aligned16(m.cells) || !aligned16(m.colors) || !aligned16(out_image))
-        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": cells, colors and out_image must be 16-byte aligned");
+    if (const int rc = check_mesh(m, f); rc != DTV_OK) return rc;
+    if (!aligned16(out_image)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": out_image must be 16-byte aligned");
     const int64_t P = (int64_t)camera->width * camera->height;
     const int64_t tiles = (int64_t)((camera->width + 7) / 8) * ((camera->height + 7) / 8);
     if (m.num_cells > kNaMaxTets || P > kNaMaxPixels || tiles > kNaMaxTiles)
@@ -642,6 +658,7 @@ int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* cam
     B.cells = m.cells;
     B.pos = m.positions;
     B.col = m.colors;
+    B.cell_col = m.cell_colors;
     B.image = out_image;
     B.T = m.num_cells;
     B.block0 = 0;
@@ -711,7 +728,7 @@ int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* cam
     ps.ckpt_n = nullptr;
     ps.counters = nullptr;
     ps.tile_frags = reinterpret_cast<uint32_t*>(base + o_tfrags);
-    launch_render_forward(dI, 1, tiles, order, recs, ranges, ps, s);
+    launch_render_forward(dI, 1, tiles, order, recs, ranges, ps, colour_kind(m), s);
     if (out_overflow) hipLaunchKernelGGL(k_copy_flag, dim3(1), dim3(1), 0, s, overflow, out_overflow);
     DTV_HIP(hipGetLastError());
     return DTV_OK;

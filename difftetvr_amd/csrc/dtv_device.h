@@ -56,7 +56,8 @@ struct BatchItem {
     ViewUniforms U;
     const int32_t* cells;
     const float* pos;
-    const float* col;
+    const float* col;       // per-vertex colours [V * 4], or NULL when cell_col is set
+    const float* cell_col;  // per-cell colours [T * 4] (ColorStorage.PER_CELL), or NULL
     float* image;        // forward output [height * width * 4]
     int64_t T;           // tets
     int64_t pixel0;      // first pixel in the batch's per-pixel state
@@ -68,7 +69,7 @@ struct BatchItem {
 struct AdjointTarget {
     const float* dimage; // [height * width * 4]
     float* grad_pos;     // [V * 3], accumulated
-    float* grad_col;     // [V * 4], accumulated
+    float* grad_col;     // [V * 4] (per-cell items: [T * 4]), accumulated
 };
 
 // Item owning global block / tile `b` (the last item whose first block / tile <= b).  Uniform per workgroup.

@@ -78,11 +78,15 @@ struct PixelState {
     uint32_t* tile_frags;   // [tiles of the batch] fragments composited per tile (written by the forward)
 };
 // order (may be NULL = tile t in workgroup t): the tile each workgroup renders, from launch_tile_order.
+// colour_kinds: which colour storages the batch's items use (kColoursPerVertex | kColoursPerCell); one launch
+// per kind present, each rendering the tiles of its items.
+constexpr uint32_t kColoursPerVertex = 1, kColoursPerCell = 2;
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
-                           const uint4* recs, const uint2* ranges, PixelState ps, hipStream_t s);
+                           const uint4* recs, const uint2* ranges, PixelState ps, uint32_t colour_kinds,
+                           hipStream_t s);
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
-                           int mode, hipStream_t s);
+                           int mode, uint32_t colour_kinds, hipStream_t s);
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 

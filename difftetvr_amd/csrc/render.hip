@@ -72,6 +72,9 @@ __device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
     return f3(__shfl(mine.x, pl, 64), __shfl(mine.y, pl, 64), __shfl(mine.z, pl, 64));
 }
 
+// kCell: renders the items with per-cell colours (BatchItem::cell_col), else those with per-vertex colours; a
+// batch that holds both is rendered by one launch of each, every tile by the one matching its item.
+template <bool kCell>
 __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
                                                         const uint32_t* __restrict__ order,
                                                         const uint4* __restrict__ recs,
@@ -80,9 +83,10 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     __shared__ FragListLds F;
     const uint32_t gtile = global_tile_of_block(order);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    if ((B.cell_col != nullptr) != kCell) return;
     const ViewUniforms& U = B.U;
     const float* __restrict__ pos = B.pos;
-    const float* __restrict__ col = B.col;
+    const float* __restrict__ col = kCell ? B.cell_col : B.col;
     float4* __restrict__ out_image = reinterpret_cast<float4*>(B.image);
     const PixelState ps = item_pixel_state(ps_all, B);
     const int tile = (int)(gtile - B.tile0);
@@ -112,7 +116,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     RecordData cd;
     if (lane < kBatch && rg.x + lane < rg.y) {
         cur = load_record(recs, rg.x + lane);
-        fetch_record_data(pos, col, cur, cd);
+        fetch_record_data<kCell>(pos, col, cur, cd);
     }
     if (lane < kBatch && rg.x + kBatch + lane < rg.y) nxt = load_record(recs, rg.x + kBatch + lane);
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
         if (lane < cnt) store_record(U, cur, cd, L, lane);
         if (lane < kBatch && base + kBatch + lane < rg.y) {
             cur = nxt;
-            fetch_record_data(pos, col, cur, cd);
+            fetch_record_data<kCell>(pos, col, cur, cd);
         }
         if (lane < kBatch && base + 2 * kBatch + lane < rg.y) nxt = load_record(recs, base + 2 * kBatch + lane);
         __syncthreads();
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
                 const int pl = ent >> 5, j = ent & 31;
                 const float3_ rdp = f3(F.rd[0][pl], F.rd[1][pl], F.rd[2][pl]);
                 FragEval E;
-                eval_fragment(L, j, ro, rdp, att, seg_scale(U, rdp), E);
+                eval_fragment<kCell, false>(L, j, ro, rdp, att, seg_scale(U, rdp), E);
                 F.val[i] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
             }
             __syncthreads();
@@ -201,7 +205,8 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
 // (AdjointIntersectRasterization.glsl:111-362): per pixel from the blended output image, back to front, the
 // 'over' operator is inverted per sub-segment 3, 2, 1 (sub-segment 0 is skipped) with no checkpoints, and
 // each fragment's parameter gradients follow frag_backward_reference.
-template <bool kRef>
+// kCell: the items with per-cell colours (exact adjoint only), as k_render_forward.
+template <bool kRef, bool kCell>
 // Waves per SIMD: 3 for the exact adjoint (168 VGPRs of the 512 per lane, 13 KB of LDS), 2 for the reference
 // one, whose sub-segment arrays need more of both.
 __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const BatchItem* __restrict__ items,
@@ -216,9 +221,10 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
     const uint32_t gtile = global_tile_of_block(order);
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
+    if ((B.cell_col != nullptr) != kCell) return;
     const ViewUniforms& U = B.U;
     const float* __restrict__ pos = B.pos;
-    const float* __restrict__ col = B.col;
+    const float* __restrict__ col = kCell ? B.cell_col : B.col;
     const float4* __restrict__ dimage = reinterpret_cast<const float4*>(targets[item].dimage);
     if (dimage == nullptr) return;  // item not differentiated
     float* __restrict__ grad_pos = targets[item].grad_pos;
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
         const uint32_t b0 = rg.x + (uint32_t)(nbatch - 1) * kBatch;
         if (lane < kBatch && b0 + lane < rg.y) {
             nxt = load_record(recs, b0 + lane);
-            fetch_record_data(pos, col, nxt, nd);
+            fetch_record_data<kCell>(pos, col, nxt, nd);
         }
     }
     for (int bi = nbatch - 1; bi >= 0; bi--) {
@@ -325,7 +331,7 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                 const int pl = ent >> 5, j = ent & 31;
                 const float3_ rdp = lane_ray(my_rd, pl);
                 if (e < total) {
-                    eval_fragment(L, j, ro, rdp, att, seg_scale(U, rdp), E);
+                    eval_fragment<kCell, true>(L, j, ro, rdp, att, seg_scale(U, rdp), E);
                     A.val[lane] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
                     if constexpr (kRef) {
 #pragma unroll
@@ -406,6 +412,27 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
                     if constexpr (kRef) {
                         const float4 dacc[3] = {s_seg[0][lane], s_seg[1][lane], s_seg[2][lane]};
                         frag_backward_reference(E, L, j, ro, rdp, att, dacc, use_abs != 0, g);
+                    } else if constexpr (kCell) {
+                        // positions in the usual slots; the tet's colour gradient in vertex 0's colour slots
+                        float lam0[4], lam1[4], dcell[4];
+                        float3_ Q0, Q1;
+                        frag_backward_cell(E, L, j, rdp, att, seg_scale(U, rdp), A.val[lane], lam0, lam1, dcell, Q0,
+                                           Q1);
+                        if (use_abs) {
+#pragma unroll
+                            for (int ch = 0; ch < 4; ch++) dcell[ch] = fabsf(dcell[ch]);
+                            Q0 = f3(fabsf(Q0.x), fabsf(Q0.y), fabsf(Q0.z));
+                            Q1 = f3(fabsf(Q1.x), fabsf(Q1.y), fabsf(Q1.z));
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const float l0 = lam0[v], l1 = lam1[v];
+                            g[7 * v + 0] = l0 * Q0.x + l1 * Q1.x;
+                            g[7 * v + 1] = l0 * Q0.y + l1 * Q1.y;
+                            g[7 * v + 2] = l0 * Q0.z + l1 * Q1.z;
+                        }
+#pragma unroll
+                        for (int ch = 0; ch < 4; ch++) g[3 + ch] = dcell[ch];
                     } else {
                         float lam0[4], lam1[4], dcol0[4], dcol1[4];
                         float3_ Q0, Q1;
@@ -495,6 +522,21 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
             const int s0 = (2 * half) * 32 + j, s1 = s0 + 32;
             const int32_t* vids = &L.vid[0][0];
             const bool slot_live = j < cnt;
+            if constexpr (kCell) {
+                // the record's tet colour gradient (lane j holds it in vertex 0's colour slots) goes straight
+                // to the tet; only positions are merged by vertex below
+                if (half == 0 && slot_live) {
+                    const int64_t t = L.meta[j] >> 4;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (acc[3 + c] != 0.f) {
+                            atomic_add_f32(&grad_col[4 * t + c], acc[3 + c]);
+                            natom++;
+                        }
+                        acc[3 + c] = 0.f;
+                    }
+                }
+            }
             const int32_t v0 = slot_live ? vids[s0] : 0, v1 = slot_live ? vids[s1] : 0;
             int ent[2] = {-1, -1};
             bool owner[2] = {false, false};
@@ -540,7 +582,7 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
             }
             // The next batch's vertex data is requested before the atomics, so staging it does not wait for them.
             // Only lanes 0..31 hold that batch's record words.
-            if (bi > 0 && lane < kBatch) fetch_record_data(pos, col, nxt, nd);
+            if (bi > 0 && lane < kBatch) fetch_record_data<kCell>(pos, col, nxt, nd);
             uint32_t issued = 0;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
@@ -610,22 +652,32 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
 }
 
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
-                           const uint4* recs, const uint2* ranges, PixelState ps, hipStream_t s) {
+                           const uint4* recs, const uint2* ranges, PixelState ps, uint32_t colour_kinds,
+                           hipStream_t s) {
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_render_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order, recs, ranges,
-                       ps);
+    if (colour_kinds & kColoursPerVertex)
+        hipLaunchKernelGGL(k_render_forward<false>, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order,
+                           recs, ranges, ps);
+    if (colour_kinds & kColoursPerCell)
+        hipLaunchKernelGGL(k_render_forward<true>, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order,
+                           recs, ranges, ps);
 }
 
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
-                           int mode, hipStream_t s) {
+                           int mode, uint32_t colour_kinds, hipStream_t s) {
     if (ntiles <= 0) return;
-    if (mode == 1)
-        hipLaunchKernelGGL(k_render_adjoint<true>, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems,
-                           order, recs, ranges, ps, use_abs);
-    else
-        hipLaunchKernelGGL(k_render_adjoint<false>, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems,
-                           order, recs, ranges, ps, use_abs);
+    if (colour_kinds & kColoursPerVertex) {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_render_adjoint<true, false>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                               targets, nitems, order, recs, ranges, ps, use_abs);
+        else
+            hipLaunchKernelGGL((k_render_adjoint<false, false>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                               targets, nitems, order, recs, ranges, ps, use_abs);
+    }
+    if (colour_kinds & kColoursPerCell)  // exact adjoint only (the caller rejects mode 1 for such items)
+        hipLaunchKernelGGL((k_render_adjoint<false, true>), dim3((unsigned)ntiles), dim3(64), 0, s, items, targets,
+                           nitems, order, recs, ranges, ps, use_abs);
 }
 
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,

@@ -14,7 +14,7 @@ struct alignas(16) RecordLds {  // 16-byte aligned: vid[] is also read as int4
     float N[12][kBatch];      // face plane normals cross(p1-p0, p2-p0), [3*f + c]
     float k[4][kBatch];       // dot(N, camPos) - dot(N, p0) per face
     uint32_t mask[2][kBatch]; // coverage of the tile's pixels 0-31 / 32-63 (pixel = ly * 8 + lx = lane)
-    uint32_t meta[kBatch];    // bit 3: face normals point outward
+    uint32_t meta[kBatch];    // bit 3: face normals point outward; bits 4-31: tet index (num_cells <= 2^28)
     int32_t vid[4][kBatch];
 };
 
@@ -40,14 +40,22 @@ struct RecordData {
     float4 c[4];
 };
 
+// kCell: col holds one colour per tet (ColorStorage.PER_CELL), fetched into c[0]; else one per vertex (or
+// none, col NULL).
+template <bool kCell = false>
 __device__ __forceinline__ void fetch_record_data(const float* __restrict__ pos, const float* __restrict__ col,
                                                   const RecordIn& r, RecordData& d) {
     const int32_t vid[4] = {(int32_t)r.vid.x, (int32_t)r.vid.y, (int32_t)r.vid.z, (int32_t)r.vid.w};
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         d.p[v] = load_pos(pos, vid[v]);
-        d.c[v] = col != nullptr ? *reinterpret_cast<const float4*>(col + 4 * (int64_t)vid[v])
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
+        if constexpr (kCell) {
+            d.c[v] = v == 0 ? *reinterpret_cast<const float4*>(col + 4 * (int64_t)(r.rec.x >> 2))
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            d.c[v] = col != nullptr ? *reinterpret_cast<const float4*>(col + 4 * (int64_t)vid[v])
+                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
 }
 
@@ -70,7 +78,7 @@ __device__ __forceinline__ void store_record(const ViewUniforms& U, const Record
     L.mask[0][i] = rec.y;
     L.mask[1][i] = rec.z;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
-    uint32_t meta = 0;
+    uint32_t meta = (rec.x >> 2) << 4;
 #pragma unroll
     for (int g = 0; g < 4; g++) {
         const float3_ a = p[kFaceTable[g][0]], b = p[kFaceTable[g][1]], c = p[kFaceTable[g][2]];
@@ -315,6 +323,10 @@ __device__ __forceinline__ float seg_scale(const ViewUniforms& U, float3_ rd) {
     return U.linear_depth ? ((U.depth_axis[0] * rd.x + U.depth_axis[1] * rd.y) + U.depth_axis[2] * rd.z) : 1.0f;
 }
 
+// kCell: the tet's own colour (staged as vertex 0's) over the whole segment, accumulateConst
+// (IntersectRasterization.glsl:250-255, Common/ForwardCommon.glsl:37-40): F = ((1 - A) c, A), A = exp(-a t);
+// the barycentrics are then needed only by the adjoint (kBary).
+template <bool kCell = false, bool kBary = true>
 __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_ ro, float3_ rd, float att,
                                               float tscale, FragEval& E) {
     int fi, fo;
@@ -329,6 +341,27 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
     E.f1 = fo;
     E.t0 = tin;
     E.t1 = tout;
+    if constexpr (kCell) {
+        if constexpr (kBary) {
+            if (!have_bary) {
+                const float3_ P0 = add(ro, scale(rd, tin)), P1 = add(ro, scale(rd, tout));
+                bary_ref(lds_p(L, face_vertex(fi, 0), j), lds_p(L, face_vertex(fi, 1), j),
+                         lds_p(L, face_vertex(fi, 2), j), P0, bary[0], bary[1]);
+                bary_ref(lds_p(L, face_vertex(fo, 0), j), lds_p(L, face_vertex(fo, 1), j),
+                         lds_p(L, face_vertex(fo, 2), j), P1, bary[2], bary[3]);
+            }
+            E.u0 = bary[0];
+            E.v0 = bary[1];
+            E.u1 = bary[2];
+            E.v1 = bary[3];
+        }
+        const float4 c = lds_c(L, 0, j);
+        E.col0 = c;
+        E.col1 = c;
+        const float A = __expf(-(c.w * att) * ((tout - tin) * tscale));
+        E.F = make_float4((1.0f - A) * c.x, (1.0f - A) * c.y, (1.0f - A) * c.z, A);
+        return;
+    }
     if (!have_bary) {  // edges, vertices, silhouettes, camera inside the tet: the faces' own barycentrics
         const float3_ P0 = add(ro, scale(rd, tin)), P1 = add(ro, scale(rd, tout));
         bary_ref(lds_p(L, face_vertex(fi, 0), j), lds_p(L, face_vertex(fi, 1), j), lds_p(L, face_vertex(fi, 2), j),
@@ -476,6 +509,48 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     face_side_q(L, j, E.f0, rd, dcol0, -dLdt, Q0);
     face_side_q(L, j, E.f1, rd, dcol1, dLdt, Q1);
     // barycentric weights over the 4 local vertices (0 at the face's opposite vertex)
+    const float mu0[3] = {E.u0, E.v0, 1.0f - E.u0 - E.v0};
+    const float mu1[3] = {E.u1, E.v1, 1.0f - E.u1 - E.v1};
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        float l0 = 0.f, l1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            l0 = face_vertex(E.f0, k) == v ? mu0[k] : l0;
+            l1 = face_vertex(E.f1, k) == v ? mu1[k] : l1;
+        }
+        lam0[v] = l0;
+        lam1[v] = l1;
+    }
+}
+
+// Exact adjoint of one per-cell fragment (eval_fragment<true>) given dL/dF: dcell = dL/d(tet colour), and the
+// position weights as frag_backward_exact's (the hit points move; the face carries no colour gradient):
+// dL/dc = (1 - A) dF.rgb, dL/dA = dF.w - dF.rgb . c, dA/dt = -a A tscale, dA/d(density) = -att t A.
+__device__ __forceinline__ void frag_backward_cell(const FragEval& E, const RecordLds& L, int j, float3_ rd,
+                                                   float att, float tscale, float4 dF, float lam0[4], float lam1[4],
+                                                   float dcell[4], float3_& Q0, float3_& Q1) {
+#pragma clang fp contract(fast)
+    const float4 c = E.col0;
+    const float tt = (E.t1 - E.t0) * tscale;
+    const float a = c.w * att;
+    const float A = __expf(-a * tt);
+    const float dA = dF.w - (dF.x * c.x + dF.y * c.y + dF.z * c.z);
+    dcell[0] = (1.0f - A) * dF.x;
+    dcell[1] = (1.0f - A) * dF.y;
+    dcell[2] = (1.0f - A) * dF.z;
+    dcell[3] = -(att * tt * A) * dA;
+    const float dLdt = -(a * A) * dA * tscale;
+    {
+        const int f = E.f0;
+        const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);
+        Q0 = scale(n, fast_div(-dLdt, cdot(n, rd)));
+    }
+    {
+        const int f = E.f1;
+        const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);
+        Q1 = scale(n, fast_div(dLdt, cdot(n, rd)));
+    }
     const float mu0[3] = {E.u0, E.v0, 1.0f - E.u0 - E.v0};
     const float mu1[3] = {E.u1, E.v1, 1.0f - E.u1 - E.v1};
 #pragma unroll

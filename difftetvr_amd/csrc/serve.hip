@@ -244,8 +244,8 @@ int dtv_server_destroy(dtv_server* S) {
 
 int dtv_server_add_mesh(dtv_server* S, const dtv_mesh* mesh, int32_t* out_mesh_id) {
     if (!S || !mesh || !out_mesh_id) return sfail(DTV_ERR_INVALID_ARGUMENT, "dtv_server_add_mesh: NULL argument");
-    if (mesh->num_cells < 0 || mesh->num_vertices < 0 ||
-        (mesh->num_cells > 0 && (!mesh->cells || !mesh->positions || !mesh->colors)))
+    if (mesh->num_cells < 0 || mesh->num_vertices < 0 || (mesh->colors != nullptr) == (mesh->cell_colors != nullptr) ||
+        (mesh->num_cells > 0 && (!mesh->cells || !mesh->positions)))
         return sfail(DTV_ERR_INVALID_ARGUMENT, "dtv_server_add_mesh: invalid mesh");
     std::lock_guard<std::mutex> lk(S->mu);
     S->meshes.push_back(*mesh);

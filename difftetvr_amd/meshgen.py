@@ -193,11 +193,11 @@ def grid_workload(n, seed=0, jitter=0.2):
 def unlink_tets(cells, positions, colors):
     """Every tet gets its own four vertices (TetMesh::unlinkTets, TetMesh.cpp:1950-1989): vertex 4t + i is the
     i-th corner of tet t, with that corner's position and colour.  Returns (cells [T,4], positions [4T,3],
-    colors [4T,4])."""
+    colors [4T,4], or None when colors is None: per-cell colours stay as they are)."""
     cells = np.asarray(cells, np.int64).reshape(-1, 4)
     flat = cells.reshape(-1)
     return (np.arange(flat.size, dtype=np.int32).reshape(-1, 4), np.asarray(positions)[flat],
-            np.asarray(colors)[flat])
+            None if colors is None else np.asarray(colors)[flat])
 
 
 def remove_transparent_tets(cells, positions, colors, alpha_threshold, cell_colors=None):

@@ -6,6 +6,8 @@ returns no gradients itself (pyutils.py:41-56).
 """
 import torch
 
+from . import ColorStorage
+
 
 class DifferentiableRenderingFunction(torch.autograd.Function):
     @staticmethod
@@ -43,5 +45,9 @@ class DifferentiableRenderer(torch.nn.Module):
 
     def forward(self):
         tet_mesh = self.renderer.get_tet_mesh()
+        if tet_mesh.get_color_storage() == ColorStorage.PER_VERTEX:
+            colors = tet_mesh.get_vertex_colors()
+        else:
+            colors = tet_mesh.get_cell_colors()
         return DifferentiableRenderingFunction.apply(self.renderer, self.tet_regularizer, self.use_abs_grad,
-                                                     tet_mesh.get_vertex_positions(), tet_mesh.get_vertex_colors())
+                                                     tet_mesh.get_vertex_positions(), colors)

@@ -84,7 +84,7 @@ class Server:
         keep = [out]
         if target is not None:
             mesh = self._meshes[int(mesh_id)]
-            pos, col = mesh.get_vertex_positions(), mesh.get_vertex_colors()
+            pos, col = mesh.get_vertex_positions(), mesh._color_tensor()
             if pos.grad is None:
                 pos.grad = torch.zeros_like(pos)
             if col.grad is None:

@@ -30,16 +30,20 @@ extern "C" {
 typedef struct dtv_ctx dtv_ctx;   /* per-device scratch and settings */
 typedef struct dtv_view dtv_view; /* state of one rendered view, kept for its adjoint */
 
-/* Tet mesh, per-vertex colours.  Layouts of TetMesh::getVertexPositionTensor/getVertexColorTensor
+/* Tet mesh.  Layouts of TetMesh::getVertexPositionTensor/getVertexColorTensor/getCellColorTensor
  * (src/Tet/TetMesh.cpp:236-367); cells are int32 vertex indices, 4 per tet, with the reference's orientation
  * convention dot(cross(p1-p0, p2-p0), p3-p0) < 0 (src/Tet/TetMesh.cpp:532-541) (other orientations render the
- * same). Colour alpha is the density before attenuation. */
+ * same). Colour alpha is the density before attenuation.  Exactly one of colors (ColorStorage.PER_VERTEX:
+ * colours interpolated over each segment, 4 sub-segments) and cell_colors (ColorStorage.PER_CELL: one colour per
+ * tet, accumulateConst over the segment, IntersectRasterization.glsl:250-255) is set; the adjoint's colour
+ * gradient then has the same layout. */
 typedef struct dtv_mesh {
     const int32_t* cells;    /* device [num_cells * 4] */
     const float* positions;  /* device [num_vertices * 3] */
-    const float* colors;     /* device [num_vertices * 4] */
+    const float* colors;     /* device [num_vertices * 4], or NULL */
     int64_t num_cells;
     int64_t num_vertices;
+    const float* cell_colors; /* device [num_cells * 4], or NULL */
 } dtv_mesh;
 
 /* Camera: view matrix as Renderer.set_view_matrix (src/Module/Module.cpp:904-911, 16 floats column-major),
@@ -96,8 +100,8 @@ typedef struct dtv_render_item {
 } dtv_render_item;
 
 /* Adjoint input and gradient targets of one batch item: dimage [height, width, 4]; gradients accumulate (+=) into
- * grad_positions [V,3] and grad_colors [V,4] (items that share a mesh may share them).  mesh must equal the
- * item's mesh as rendered (same buffers and sizes; checked). */
+ * grad_positions [V,3] and grad_colors ([V,4], or [T,4] for a mesh with cell_colors) (items that share a mesh may
+ * share them).  mesh must equal the item's mesh as rendered (same buffers and sizes; checked). */
 typedef struct dtv_adjoint_item {
     dtv_mesh mesh;
     const float* dimage;
@@ -170,7 +174,7 @@ int dtv_render_adjoint_batch(dtv_view* view, const dtv_adjoint_item* items, int3
                              int32_t adjoint_mode, void* stream);
 
 /* Adjoint of the last dtv_render of this view: accumulates (+=) into grad_positions [V,3] and
- * grad_colors [V,4].  Replaces Renderer.render_adjoint (Module.cpp:922-959) ->
+ * grad_colors [V,4] ([T,4] for cell colours).  Replaces Renderer.render_adjoint (Module.cpp:922-959) ->
  * TetMeshRendererIntersection::renderAdjoint (:709-715).  The mesh must be the one rendered (same cells,
  * positions and colours buffers and sizes; else DTV_ERR_INVALID_ARGUMENT). */
 int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage, int32_t use_abs_grad,
@@ -282,7 +286,7 @@ typedef struct dtv_server_request {
     float* out_image;           /* device [height, width, 4], written when the request completes */
     const float* target_image;  /* optional device [height, width, 4]: also run loss + adjoint */
     float* grad_positions;      /* with target_image: device [V,3], accumulated */
-    float* grad_colors;         /* with target_image: device [V,4], accumulated */
+    float* grad_colors;         /* with target_image: device [V,4] ([T,4] for cell colours), accumulated */
 } dtv_server_request;
 
 typedef struct dtv_server_stats {

@@ -48,14 +48,15 @@ def _load():
     lib.orc_frags_export.argtypes = [c.c_void_p, _i64p, _i32p]
     lib.orc_frags_export_tris.argtypes = [c.c_void_p, _u32p, _i32p]
     lib.orc_frags_free.argtypes = [c.c_void_p]
-    lib.orc_forward_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
-                                  c.c_int, _f32p, _f32p, _i32p]
-    lib.orc_forward_d.argtypes = [c.c_void_p, _i32p, _f64p, _f64p, _f64p, c.c_double, _f64p, c.c_int, c.c_double,
-                                  c.c_int, c.c_int, c.c_void_p, _f64p, _f64p, _i32p]
-    lib.orc_adjoint_exact_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
+    vp = c.c_void_p  # optional arrays (None = NULL)
+    lib.orc_forward_f.argtypes = [vp, _i32p, _f32p, vp, vp, _f32p, c.c_float, _f32p, c.c_int, c.c_float, c.c_int,
+                                  _f32p, _f32p, _i32p]
+    lib.orc_forward_d.argtypes = [vp, _i32p, _f64p, vp, vp, _f64p, c.c_double, _f64p, c.c_int, c.c_double, c.c_int,
+                                  c.c_int, vp, _f64p, _f64p, _i32p]
+    lib.orc_adjoint_exact_f.argtypes = [vp, _i32p, _f32p, vp, vp, _f32p, c.c_float, _f32p, c.c_int, c.c_float,
                                         c.c_int, _f32p, c.c_int, _f32p, _f32p]
-    lib.orc_adjoint_exact_d.argtypes = [c.c_void_p, _i32p, _f64p, _f64p, _f64p, c.c_double, _f64p, c.c_int,
-                                        c.c_double, c.c_int, c.c_int, _f64p, c.c_int, _f64p, _f64p]
+    lib.orc_adjoint_exact_d.argtypes = [vp, _i32p, _f64p, vp, vp, _f64p, c.c_double, _f64p, c.c_int, c.c_double,
+                                        c.c_int, c.c_int, _f64p, c.c_int, _f64p, _f64p]
     lib.orc_adjoint_reference_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, _f32p, c.c_int,
                                             _f32p, _f32p]
     lib.orc_accumulate_linear_const_d.argtypes = [c.c_double, _f64p, _f64p, c.c_double, _f64p]
@@ -126,41 +127,56 @@ class Fragments:
         return keys[:n], ids[:n]
 
 
+def _colour_args(colors, cell_colors, dtype):
+    """(per-vertex colours, per-cell colours) as NULL-able pointers, plus the arrays that keep them alive."""
+    if (colors is None) == (cell_colors is None):
+        raise ValueError("give exactly one of colors (per vertex) and cell_colors (per cell)")
+    arr = np.ascontiguousarray(colors if cell_colors is None else cell_colors, dtype).reshape(-1)
+    ptr = arr.ctypes.data_as(ctypes.c_void_p)
+    return (ptr, None, arr) if cell_colors is None else (None, ptr, arr)
+
+
 def forward(frags, positions, colors, cam, attenuation, clear=(0, 0, 0, 0), use_ert=False, ert_alpha=1.0 - 1e-4,
-            dtype=np.float32, linear_depth=False, f32_decisions=False, decision_positions=None):
+            dtype=np.float32, linear_depth=False, f32_decisions=False, decision_positions=None, cell_colors=None):
     """Renders; returns (image[H,W,4], ray colour C[H,W,4] before the clear blend, fragment count[H,W]).
     linear_depth: segment lengths as differences of linear eye depths (the PPLL renderer's) instead of the
     Euclidean lengths of the INTERSECTION renderer.  f32_decisions (float64 only; cam = the float64 uniforms):
     the ray-tet decisions of the FP32 evaluation, the continuous maths in FP64; decision_positions (float32
-    [V,3]) takes those decisions at other positions (decisions frozen, e.g. for differences along one branch)."""
+    [V,3]) takes those decisions at other positions (decisions frozen, e.g. for differences along one branch).
+    cell_colors [T,4] (with colors None): one colour per tet (ColorStorage.PER_CELL)."""
     lib = _load()
     H, W = frags.height, frags.width
     img = np.zeros((H, W, 4), dtype)
     C = np.zeros((H, W, 4), dtype)
     cnt = np.zeros((H, W), np.int32)
+    vcol, ccol, keep = _colour_args(colors, cell_colors, dtype)
     dp = None if decision_positions is None else np.ascontiguousarray(decision_positions, np.float32).reshape(-1)
     extra = () if dtype == np.float32 else (int(bool(f32_decisions)),
                                             None if dp is None else dp.ctypes.data_as(ctypes.c_void_p))
     fn = lib.orc_forward_f if dtype == np.float32 else lib.orc_forward_d
-    fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1),
-       np.ascontiguousarray(colors, dtype).reshape(-1), np.ascontiguousarray(cam, dtype), attenuation,
-       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)), *extra, img, C, cnt)
+    fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1), vcol, ccol,
+       np.ascontiguousarray(cam, dtype), attenuation, np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha,
+       int(bool(linear_depth)), *extra, img, C, cnt)
+    del keep
     return img, C, cnt
 
 
 def adjoint_exact(frags, positions, colors, cam, attenuation, dimg, clear=(0, 0, 0, 0), use_ert=False,
-                  ert_alpha=1.0 - 1e-4, use_abs_grad=False, dtype=np.float32, linear_depth=False, f32_decisions=False):
-    """Exact gradient of sum(dimg * image) w.r.t. vertex positions [V,3] and colours [V,4]."""
+                  ert_alpha=1.0 - 1e-4, use_abs_grad=False, dtype=np.float32, linear_depth=False, f32_decisions=False,
+                  cell_colors=None):
+    """Exact gradient of sum(dimg * image) w.r.t. vertex positions [V,3] and colours: [V,4] per vertex, or [T,4]
+    per cell when cell_colors is given (colors None)."""
     lib = _load()
     V = np.asarray(positions).reshape(-1, 3).shape[0]
+    vcol, ccol, keep = _colour_args(colors, cell_colors, dtype)
     gpos = np.zeros((V, 3), dtype)
-    gcol = np.zeros((V, 4), dtype)
+    gcol = np.zeros((V if cell_colors is None else keep.size // 4, 4), dtype)
     fn = lib.orc_adjoint_exact_f if dtype == np.float32 else lib.orc_adjoint_exact_d
-    fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1),
-       np.ascontiguousarray(colors, dtype).reshape(-1), np.ascontiguousarray(cam, dtype), attenuation,
-       np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha, int(bool(linear_depth)),
-       *(() if dtype == np.float32 else (int(bool(f32_decisions)),)),
+    fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1), vcol, ccol,
+       np.ascontiguousarray(cam, dtype), attenuation, np.ascontiguousarray(clear, dtype), int(use_ert), ert_alpha,
+       int(bool(linear_depth)), *(() if dtype == np.float32 else (int(bool(f32_decisions)),)),
        np.ascontiguousarray(dimg, dtype).reshape(-1), int(use_abs_grad), gpos.reshape(-1), gcol.reshape(-1))
+    del keep
     return gpos, gcol
 
 

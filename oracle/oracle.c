@@ -328,30 +328,32 @@ void orc_frags_free(orc_frags* fr) {
  * f32_decisions (double versions): the ray-tet decisions (hit, entry and exit faces, camera inside) are taken
  * from the FP32 evaluation (FP32 camera and ray, positions rounded to float), the continuous maths in FP64: the
  * FP64 counterpart of the FP32 function, free of the edge-grazing rays where FP32 and FP64 decide differently. */
-void orc_forward_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* camu,
-                   float att, const float* clear, int use_ert, float ert_alpha, int linear_depth, float* img, float* C,
-                   int32_t* cnt) {
-    forward_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, 0, NULL, img, C, cnt);
+/* cell_col (may be NULL): per-cell colours [T*4]; col is then not read and gcol is [T*4]. */
+void orc_forward_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* cell_col,
+                   const float* camu, float att, const float* clear, int use_ert, float ert_alpha, int linear_depth,
+                   float* img, float* C, int32_t* cnt) {
+    forward_f(fr, cells, pos, col, cell_col, camu, att, clear, use_ert, ert_alpha, linear_depth, 0, NULL, img, C, cnt);
 }
 /* dec_pos (may be NULL, with f32_decisions): the float positions the decisions are taken at. */
-void orc_forward_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col, const double* camu,
-                   double att, const double* clear, int use_ert, double ert_alpha, int linear_depth, int f32_decisions,
-                   const float* dec_pos, double* img, double* C, int32_t* cnt) {
-    forward_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, f32_decisions, dec_pos, img, C,
-              cnt);
+void orc_forward_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col,
+                   const double* cell_col, const double* camu, double att, const double* clear, int use_ert,
+                   double ert_alpha, int linear_depth, int f32_decisions, const float* dec_pos, double* img, double* C,
+                   int32_t* cnt) {
+    forward_d(fr, cells, pos, col, cell_col, camu, att, clear, use_ert, ert_alpha, linear_depth, f32_decisions, dec_pos,
+              img, C, cnt);
 }
-void orc_adjoint_exact_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col, const float* camu,
-                         float att, const float* clear, int use_ert, float ert_alpha, int linear_depth, const float* dimg,
-                         int use_abs, float* gpos, float* gcol) {
-    adjoint_exact_f(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, 0, dimg, use_abs, gpos,
-                    gcol);
+void orc_adjoint_exact_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col,
+                         const float* cell_col, const float* camu, float att, const float* clear, int use_ert,
+                         float ert_alpha, int linear_depth, const float* dimg, int use_abs, float* gpos, float* gcol) {
+    adjoint_exact_f(fr, cells, pos, col, cell_col, camu, att, clear, use_ert, ert_alpha, linear_depth, 0, dimg, use_abs,
+                    gpos, gcol);
 }
 void orc_adjoint_exact_d(const orc_frags* fr, const int32_t* cells, const double* pos, const double* col,
-                         const double* camu, double att, const double* clear, int use_ert, double ert_alpha,
-                         int linear_depth, int f32_decisions, const double* dimg, int use_abs, double* gpos,
-                         double* gcol) {
-    adjoint_exact_d(fr, cells, pos, col, camu, att, clear, use_ert, ert_alpha, linear_depth, f32_decisions, dimg,
-                    use_abs, gpos, gcol);
+                         const double* cell_col, const double* camu, double att, const double* clear, int use_ert,
+                         double ert_alpha, int linear_depth, int f32_decisions, const double* dimg, int use_abs,
+                         double* gpos, double* gcol) {
+    adjoint_exact_d(fr, cells, pos, col, cell_col, camu, att, clear, use_ert, ert_alpha, linear_depth, f32_decisions,
+                    dimg, use_abs, gpos, gcol);
 }
 void orc_adjoint_reference_f(const orc_frags* fr, const int32_t* cells, const float* pos, const float* col,
                              const float* camu, float att, const float* img, const float* dimg, int use_abs, float* gpos,

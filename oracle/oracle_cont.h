@@ -131,15 +131,28 @@ typedef struct {
     R a[4];               /* density * attenuation per sub-segment */
     SFX(v3) sc0[4], sc1[4];
     R tscale;             /* segment length per unit ray length: 1, or the linear-depth factor (PPLL) */
+    int per_cell;         /* one colour per tet: accumulateConst over the whole segment */
 } SFX(frag);
 
-static void SFX(load_tet)(const int32_t* cells, const R* pos, const R* col, int64_t tet, SFX(v3) P[4], SFX(v4) C[4], int32_t vid[4]) {
+/* Tet `tet`'s vertices and colours.  cell_col (may be NULL): per-cell colours [T*4] (ColorStorage.PER_CELL,
+ * TetMesh.cpp:310-343); every C[i] is then the tet's colour and col is not read. */
+static void SFX(load_tet)(const int32_t* cells, const R* pos, const R* col, const R* cell_col, int64_t tet,
+                          SFX(v3) P[4], SFX(v4) C[4], int32_t vid[4]) {
     for (int i = 0; i < 4; i++) {
         int32_t g = cells[tet * 4 + i];
         vid[i] = g;
         P[i] = SFX(mk3)(pos[3 * (int64_t)g], pos[3 * (int64_t)g + 1], pos[3 * (int64_t)g + 2]);
-        C[i] = SFX(mk4)(col[4 * (int64_t)g], col[4 * (int64_t)g + 1], col[4 * (int64_t)g + 2], col[4 * (int64_t)g + 3]);
+        if (cell_col)
+            C[i] = SFX(mk4)(cell_col[4 * tet], cell_col[4 * tet + 1], cell_col[4 * tet + 2], cell_col[4 * tet + 3]);
+        else
+            C[i] = SFX(mk4)(col[4 * (int64_t)g], col[4 * (int64_t)g + 1], col[4 * (int64_t)g + 2], col[4 * (int64_t)g + 3]);
     }
+}
+
+/* accumulateConst: Common/ForwardCommon.glsl:37-40 (alpha = 1 - exp(-a t), colour alpha * c). */
+static SFX(v4) SFX(acc_const)(R t, SFX(v4) c, R a) {
+    R A = (R)1.0 - REXP_(-a * t);
+    return SFX(mk4)(A * c.x, A * c.y, A * c.z, A);
 }
 
 /* Segment length per unit length along the ray.  INTERSECTION: the Euclidean distance between the entry and
@@ -163,9 +176,10 @@ static R SFX(face_t)(SFX(v3) ro, SFX(v3) rd, SFX(v3) p0, SFX(v3) p1, SFX(v3) p2)
 /* dec (may be NULL): {hit, entry face, exit face, entry clamped to 0} decided elsewhere (the FP32 decisions of
  * an FP64 evaluation); the distances are then this precision's t of those faces. */
 static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX(v4) C[4], R att, R tscale,
-                           const int* dec, SFX(frag) * F) {
+                           int per_cell, const int* dec, SFX(frag) * F) {
     memset(F, 0, sizeof(*F));
     F->tscale = tscale;
+    F->per_cell = per_cell;
     if (dec) {
         F->hit = dec[0];
         if (!F->hit) return;
@@ -193,6 +207,13 @@ static void SFX(frag_eval)(SFX(v3) ro, SFX(v3) rd, const SFX(v3) P[4], const SFX
     F->col1 = SFX(mk4)((C[b[0]].x * u + C[b[1]].x * v) + C[b[2]].x * w, (C[b[0]].y * u + C[b[1]].y * v) + C[b[2]].y * w,
                        (C[b[0]].z * u + C[b[1]].z * v) + C[b[2]].z * w, (C[b[0]].w * u + C[b[1]].w * v) + C[b[2]].w * w);
     R t = F->t1 - F->t0;
+    if (per_cell) {
+        /* IntersectRasterization.glsl:250-255: the tet's colour, accumulateConst over the segment length */
+        F->col0 = F->col1 = C[0];
+        F->a[0] = C[0].w * att;
+        F->state[4] = SFX(acc_const)(t * tscale, C[0], F->a[0]);
+        return;
+    }
     R tSeg = (t / (R)4.0) * tscale;
     SFX(v4) rc = SFX(mk4)(0, 0, 0, 0);
     F->state[0] = rc;
@@ -237,7 +258,8 @@ static const int* decide_f32(const cam_f* camf, v3_f rdf, const v3_d Pt[4], cons
 /* Forward over the per-pixel fragment lists.  Front-to-back 'over' equals the reference's back-to-front
  * premultiplied blend (TetMeshRendererIntersection.cpp:258-276); the clear colour is blended last.
  * Early ray termination follows PPLL/Forward.glsl:168-173 (checked before each fragment). */
-static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* camu, R att,
+static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* cell_col,
+                         const R* camu, R att,
                          const R clear[4], int use_ert, R ert_alpha, int linear_depth, int f32_decisions,
                          const float* dec_pos, R* out_img, R* out_C, int32_t* out_cnt) {
     SFX(cam) cam;
@@ -266,7 +288,7 @@ static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos
             if (use_ert && rc.w > ert_alpha) break;
             int32_t id = fr->ids[k];
             SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
-            SFX(load_tet)(cells, pos, col, id >> 2, Pt, Ct, vid);
+            SFX(load_tet)(cells, pos, col, cell_col, id >> 2, Pt, Ct, vid);
             SFX(frag) F;
             int dec[4];
             const int* decp = NULL;
@@ -276,7 +298,7 @@ static void SFX(forward)(const orc_frags* fr, const int32_t* cells, const R* pos
             (void)dec;
             (void)dec_pos;
 #endif
-            SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, decp, &F);
+            SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, cell_col != NULL, decp, &F);
             cnt++;
             if (!F.hit) continue;
             SFX(v4) f = F.state[4];
@@ -347,6 +369,34 @@ static SFX(v3) SFX(face_q)(SFX(v3) N, SFX(v3) d, R g[4][3], const R dC[4], R sgn
  * gradient contributions into gp[4][3], gc[4][4] (tet-local), one contribution per face corner. */
 static void SFX(frag_backward_exact)(const SFX(frag) * F, SFX(v3) rd, const SFX(v3) Pt[4], const SFX(v4) Ct[4], R att,
                                     const R dF[4], int use_abs, R gp[4][3], R gc[4][4]) {
+    if (F->per_cell) {
+        /* accumulateConst: F = (alpha c, alpha), alpha = 1 - A, A = exp(-a t t_s): dL/dc = alpha dF.rgb,
+         * dL/dalpha = dF.a + dF.rgb . c, dalpha/dt = a A t_s, dalpha/d(density) = att t t_s A.  The tet's colour
+         * gradient goes to gc[0]; the positions move only the hit points, with no colour gradient in the face. */
+        R tt = (F->t1 - F->t0) * F->tscale;
+        R a = F->a[0];
+        R A = REXP_(-a * tt);
+        SFX(v4) c = F->col0;
+        R sa = dF[3] + (dF[0] * c.x + dF[1] * c.y + dF[2] * c.z);
+        R dcell[4] = {((R)1.0 - A) * dF[0], ((R)1.0 - A) * dF[1], ((R)1.0 - A) * dF[2], att * tt * A * sa};
+        for (int ch = 0; ch < 4; ch++) gc[0][ch] += use_abs ? RABS_(dcell[ch]) : dcell[ch];
+        R dLdt = a * A * sa * F->tscale;
+        R zero_g[4][3] = {{0}};
+        R zero_dc[4] = {0, 0, 0, 0};
+        for (int side = 0; side < 2; side++) {
+            const int* fv = ORC_FT[side == 0 ? F->f0 : F->f1];
+            R u = side == 0 ? F->u0 : F->u1;
+            R v = side == 0 ? F->v0 : F->v1;
+            R mu[3] = {u, v, (R)1.0 - u - v};
+            SFX(v3) N = SFX(face_normal)(Pt[fv[0]], Pt[fv[1]], Pt[fv[2]]);
+            SFX(v3) Q = SFX(face_q)(N, rd, zero_g, zero_dc, side == 0 ? -dLdt : dLdt);
+            for (int k = 0; k < 3; k++) {
+                R cgp[3] = {mu[k] * Q.x, mu[k] * Q.y, mu[k] * Q.z};
+                for (int ch = 0; ch < 3; ch++) gp[fv[k]][ch] += use_abs ? RABS_(cgp[ch]) : cgp[ch];
+            }
+        }
+        return;
+    }
     R dR[4] = {dF[0], dF[1], dF[2], dF[3]};
     R dcol0[4] = {0, 0, 0, 0}, dcol1[4] = {0, 0, 0, 0};
     R dLdt = 0;
@@ -409,7 +459,15 @@ static void SFX(frag_backward_exact)(const SFX(frag) * F, SFX(v3) rd, const SFX(
     }
 }
 
-static void SFX(scatter_tet_grads)(const int32_t vid[4], R gp[4][3], R gc[4][4], R* gpos, R* gcol) {
+/* tet >= 0: per-cell colours, gc[0] is the tet's colour gradient (gcol [T*4]). */
+static void SFX(scatter_tet_grads)(const int32_t vid[4], R gp[4][3], R gc[4][4], int64_t tet, R* gpos, R* gcol) {
+    if (tet >= 0) {
+        for (int ch = 0; ch < 4; ch++) {
+            R val = gc[0][ch];
+#pragma omp atomic
+            gcol[4 * tet + ch] += val;
+        }
+    }
     for (int i = 0; i < 4; i++) {
         int64_t g = vid[i];
         for (int ch = 0; ch < 3; ch++) {
@@ -417,6 +475,7 @@ static void SFX(scatter_tet_grads)(const int32_t vid[4], R gp[4][3], R gc[4][4],
 #pragma omp atomic
             gpos[3 * g + ch] += val;
         }
+        if (tet >= 0) continue;
         for (int ch = 0; ch < 4; ch++) {
             R val = gc[i][ch];
 #pragma omp atomic
@@ -427,7 +486,8 @@ static void SFX(scatter_tet_grads)(const int32_t vid[4], R gp[4][3], R gc[4][4],
 
 /* Exact adjoint over all pixels.  gpos/gcol are accumulated into (like the reference's .grad buffers,
  * TetMesh.cpp:243-249).  dimg is dL/d(blended image). */
-static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* camu, R att,
+static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const R* pos, const R* col, const R* cell_col,
+                               const R* camu, R att,
                                const R clear[4], int use_ert, R ert_alpha, int linear_depth, int f32_decisions,
                                const R* dimg, int use_abs, R* gpos, R* gcol) {
     SFX(cam) cam;
@@ -468,12 +528,12 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
                 if (use_ert && rc.w > ert_alpha) break;
                 states[k] = rc;
                 SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
-                SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
+                SFX(load_tet)(cells, pos, col, cell_col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
                 SFX(frag) F;
 #if IS_DOUBLE_
                 if (f32_decisions) decp = decide_f32(&camf, rdf, Pt, NULL, vid, dec);
 #endif
-                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, decp, &F);
+                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, cell_col != NULL, decp, &F);
                 m = k + 1;
                 if (!F.hit) continue;
                 SFX(v4) f = F.state[4];
@@ -487,12 +547,12 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
             R dC[4] = {di[0], di[1], di[2], di[3] * ((R)1.0 - clear[3]) - (di[0] * clear[0] + di[1] * clear[1] + di[2] * clear[2])};
             for (int64_t k = m - 1; k >= 0; k--) {
                 SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
-                SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
+                SFX(load_tet)(cells, pos, col, cell_col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
                 SFX(frag) F;
 #if IS_DOUBLE_
                 if (f32_decisions) decp = decide_f32(&camf, rdf, Pt, NULL, vid, dec);
 #endif
-                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, decp, &F);
+                SFX(frag_eval)(ro, rd, Pt, Ct, att, tscale, cell_col != NULL, decp, &F);
                 if (!F.hit) continue;
                 SFX(v4) ci = states[k];
                 SFX(v4) f = F.state[4];
@@ -501,7 +561,8 @@ static void SFX(adjoint_exact)(const orc_frags* fr, const int32_t* cells, const 
                 dC[3] = dC[3] * ((R)1.0 - f.w) - (dC[0] * f.x + dC[1] * f.y + dC[2] * f.z);
                 R gp[4][3] = {{0}}, gc[4][4] = {{0}};
                 SFX(frag_backward_exact)(&F, rd, Pt, Ct, att, dF, use_abs, gp, gc);
-                SFX(scatter_tet_grads)(vid, gp, gc, gpos, gcol);
+                SFX(scatter_tet_grads)(vid, gp, gc, cell_col ? (int64_t)(fr->ids[fr->off[px] + k] >> 2) : -1, gpos,
+                                       gcol);
             }
         }
         free(states);
@@ -591,9 +652,9 @@ static void SFX(adjoint_reference)(const orc_frags* fr, const int32_t* cells, co
         R dro[4] = {dimg[4 * px], dimg[4 * px + 1], dimg[4 * px + 2], dimg[4 * px + 3]};
         for (int64_t k = n - 1; k >= 0; k--) {
             SFX(v3) Pt[4]; SFX(v4) Ct[4]; int32_t vid[4];
-            SFX(load_tet)(cells, pos, col, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
+            SFX(load_tet)(cells, pos, col, NULL, fr->ids[fr->off[px] + k] >> 2, Pt, Ct, vid);
             SFX(frag) F;
-            SFX(frag_eval)(ro, rd, Pt, Ct, att, (R)1.0, NULL, &F);
+            SFX(frag_eval)(ro, rd, Pt, Ct, att, (R)1.0, 0, NULL, &F);
             if (!F.hit) continue;
             const int* a0 = ORC_FT[F.f0];
             const int* a1 = ORC_FT[F.f1];
@@ -662,7 +723,7 @@ static void SFX(adjoint_reference)(const orc_frags* fr, const int32_t* cells, co
                     for (int ch = 0; ch < 4; ch++) gc[fv[k]][ch] += use_abs ? RABS_(gc6[sd][k][ch]) : gc6[sd][k][ch];
                 }
             }
-            SFX(scatter_tet_grads)(vid, gp, gc, gpos, gcol);
+            SFX(scatter_tet_grads)(vid, gp, gc, -1, gpos, gcol);
         }
     }
 }

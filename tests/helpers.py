@@ -43,8 +43,11 @@ def adjoint_references(oracle, frags, pos, col, cam_f, cam_d, att, dimg, **kw):
     """The oracle's exact adjoint in FP32, and in FP64 with the FP32 evaluation's ray-tet decisions (the FP64
     counterpart of the same function: no edge-grazing ray decides differently)."""
     op, oc = oracle.adjoint_exact(frags, pos, col, cam_f, att, dimg, **kw)
-    mp, mc = oracle.adjoint_exact(frags, np.asarray(pos, np.float64), np.asarray(col, np.float64), cam_d, att,
-                                  np.asarray(dimg, np.float64), dtype=np.float64, f32_decisions=True, **kw)
+    kw64 = dict(kw)
+    if kw64.get("cell_colors") is not None:  # per-cell colours (col None)
+        kw64["cell_colors"] = np.asarray(kw64["cell_colors"], np.float64)
+    mp, mc = oracle.adjoint_exact(frags, np.asarray(pos, np.float64), None if col is None else np.asarray(col, np.float64),
+                                  cam_d, att, np.asarray(dimg, np.float64), dtype=np.float64, f32_decisions=True, **kw64)
     return op, oc, mp, mc
 
 

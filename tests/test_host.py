@@ -81,11 +81,23 @@ def test_bintet_roundtrip(tmp_path):
     col = np.random.default_rng(0).uniform(size=(pos.shape[0], 4)).astype(np.float32)
     p = str(tmp_path / "m.bintet")
     bintet.write(p, cells, pos, col)
-    c2, p2, col2 = bintet.read(p)
-    assert np.array_equal(c2, cells) and np.array_equal(p2, pos) and np.array_equal(col2, col)
+    c2, p2, col2, cc2 = bintet.read(p)
+    assert np.array_equal(c2, cells) and np.array_equal(p2, pos) and np.array_equal(col2, col) and cc2 is None
     raw = open(p, "rb").read()
     assert raw[:4] == (1).to_bytes(4, "little")
     assert raw[4:8] == (cells.size).to_bytes(4, "little")
+    # version 2: cell colours after an empty vertex colour array
+    ccol = np.random.default_rng(1).uniform(size=(cells.shape[0], 4)).astype(np.float32)
+    p_2 = str(tmp_path / "m2.bintet")
+    bintet.write(p_2, cells, pos, None, ccol)
+    raw2 = open(p_2, "rb").read()
+    assert raw2[:4] == (2).to_bytes(4, "little")
+    tail = 4 + 4 + cells.size * 4 + 4 + pos.size * 4
+    assert raw2[tail:tail + 4] == (0).to_bytes(4, "little")
+    assert raw2[tail + 4:tail + 8] == (cells.shape[0]).to_bytes(4, "little")
+    assert len(raw2) == tail + 8 + ccol.size * 4
+    c3, p3, col3, cc3 = bintet.read(p_2)
+    assert np.array_equal(c3, cells) and np.array_equal(p3, pos) and col3 is None and np.array_equal(cc3, ccol)
     bad = str(tmp_path / "bad.bintet")
     open(bad, "wb").write((7).to_bytes(4, "little") + raw[4:])
     with pytest.raises(RuntimeError, match="Invalid version"):
