@@ -44,7 +44,7 @@ constexpr int kAdjFrags = 1024;
 struct AdjFragLds {
     uint16_t entry[kAdjFrags];  // (pixel lane << 5) | record index, one part of a batch
     float w[32][29];            // gradient rows of half a round (29: conflict-free column access)
-    float4 val[kRoundAdj];      // F, then dL/dF; w = kMissMarker when the ray missed the tet
+    float4 val[kRoundAdj];      // F, then dL/dF (.w: d/d(alpha), alpha = 1 - F.w); w = kMissMarker on a miss
     uint32_t cnt[kBatch];       // half-round counting sort by record: fragments per record, then
     uint32_t start[kBatch];     // their first row
 };

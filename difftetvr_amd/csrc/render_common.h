@@ -524,9 +524,11 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     }
 }
 
-// Exact adjoint of one per-cell fragment (eval_fragment<true>) given dL/dF: dcell = dL/d(tet colour), and the
-// position weights as frag_backward_exact's (the hit points move; the face carries no colour gradient):
-// dL/dc = (1 - A) dF.rgb, dL/dA = dF.w - dF.rgb . c, dA/dt = -a A tscale, dA/d(density) = -att t A.
+// Exact adjoint of one per-cell fragment (eval_fragment<true>) given dL/dF, whose .w is the derivative with
+// respect to the fragment's alpha 1 - A (as throughout the adjoint): dcell = dL/d(tet colour), and the position
+// weights as frag_backward_exact's (the hit points move; the face carries no colour gradient).  With
+// F = (alpha c, alpha), alpha = 1 - exp(-a t): dL/dc = alpha dF.rgb, dL/dalpha = dF.w + dF.rgb . c,
+// dalpha/dt = a A tscale, dalpha/d(density) = att t A.
 __device__ __forceinline__ void frag_backward_cell(const FragEval& E, const RecordLds& L, int j, float3_ rd,
                                                    float att, float tscale, float4 dF, float lam0[4], float lam1[4],
                                                    float dcell[4], float3_& Q0, float3_& Q1) {
@@ -535,12 +537,12 @@ __device__ __forceinline__ void frag_backward_cell(const FragEval& E, const Reco
     const float tt = (E.t1 - E.t0) * tscale;
     const float a = c.w * att;
     const float A = __expf(-a * tt);
-    const float dA = dF.w - (dF.x * c.x + dF.y * c.y + dF.z * c.z);
+    const float dal = dF.w + (dF.x * c.x + dF.y * c.y + dF.z * c.z);
     dcell[0] = (1.0f - A) * dF.x;
     dcell[1] = (1.0f - A) * dF.y;
     dcell[2] = (1.0f - A) * dF.z;
-    dcell[3] = -(att * tt * A) * dA;
-    const float dLdt = -(a * A) * dA * tscale;
+    dcell[3] = (att * tt * A) * dal;
+    const float dLdt = (a * A) * dal * tscale;
     {
         const int f = E.f0;
         const float3_ n = f3(L.N[3 * f][j], L.N[3 * f + 1][j], L.N[3 * f + 2][j]);
