@@ -9,9 +9,11 @@ Renderer types: INTERSECTION renders with that renderer's maths (Euclidean segme
 linear sub-segments, face-centroid depth order); PPLL (the reference's default) with the same tile renderer and
 PPLL's segment length, the difference of linear eye depths (PPLL/Forward.glsl:53,188), in the same face-centroid
 order (PPLL sorts each pixel's faces by depth; the two orders agree wherever the centroid order is
-visibility-correct along the ray); PROJECTION is not available and raises.  Gradients are the exact derivative of
-the forward (``set_adjoint_mode("reference")`` selects the INTERSECTION renderer's own adjoint), coarse-to-fine
-refinement chooses prism diagonals by vertex index, and only GPU tensors are supported.
+visibility-correct along the ray); PROJECTION renders projected tetrahedra (3 or 4 screen triangles per tet with
+a thickness at the thick vertex, blended in centroid-depth order) and has no early ray termination.  Gradients
+are the exact derivative of the forward (``set_adjoint_mode("reference")`` selects the INTERSECTION renderer's
+own adjoint), coarse-to-fine refinement chooses prism diagonals by vertex index, and only GPU tensors are
+supported.
 """
 import ctypes
 import enum
@@ -363,9 +365,6 @@ class Renderer:
     """Tet-mesh volume renderer (TetMeshVolumeRenderer, src/Renderer/TetMeshVolumeRenderer.hpp:84-280)."""
 
     def __init__(self, renderer_type=RendererType.PPLL):
-        if RendererType(renderer_type) == RendererType.PROJECTION:
-            raise RuntimeError("RendererType.PROJECTION (projected tetrahedra) is not available in difftetvr_amd; "
-                               "use RendererType.PPLL or RendererType.INTERSECTION.")
         ctx = _ctx()
         lib = _lib.load()
         self._renderer_type = RendererType(renderer_type)
@@ -498,6 +497,8 @@ class Renderer:
         # PPLL composites between consecutive faces by their linear depths (PPLL/Forward.glsl:53,188)
         p.segment_length = (_lib.DTV_SEGMENT_LINEAR_DEPTH if self._renderer_type == RendererType.PPLL
                             else _lib.DTV_SEGMENT_EUCLIDEAN)
+        p.renderer = (_lib.DTV_RENDERER_PROJECTION if self._renderer_type == RendererType.PROJECTION
+                      else _lib.DTV_RENDERER_TILE)
         return p
 
     def camera_uniforms(self):

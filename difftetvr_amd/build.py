@@ -11,8 +11,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["api.hip", "binning.hip", "sort.hip", "render.hip", "optim.hip", "regularizer.hip", "serve.hip",
-           "split.hip"]
+SOURCES = ["api.hip", "binning.hip", "sort.hip", "render.hip", "proj.hip", "optim.hip", "regularizer.hip",
+           "serve.hip", "split.hip"]
 LIB = os.path.join(HERE, "libdifftetvr_amd.so")
 ARCH = os.environ.get("DTV_OFFLOAD_ARCH", "gfx950")
 

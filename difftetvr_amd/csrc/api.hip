@@ -213,6 +213,8 @@ struct dtv_view {
         lookback, ranges;
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_t, ckpt_n;
     DevBuf tile_frags, tile_order;  // per tile: forward's fragments; launch orders of the forward and adjoint
+    DevBuf pt_info, pt_vert, pt_col, tri_grad;  // Projection renderer: per-tet triangles, per-corner gradients
+    bool projection = false;        // the last render ran the Projection renderer
     bool cost_ordered = true;       // the last render launched its tiles heaviest first
     int64_t planned_cap = 0;  // > 0: planned (synchronisation-free) rendering with this record capacity
     // state of the last render
@@ -299,6 +301,9 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_order must be 0 (heaviest first) or 1 (row-major)");
     if (params->segment_length != DTV_SEGMENT_EUCLIDEAN && params->segment_length != DTV_SEGMENT_LINEAR_DEPTH)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
+    if (params->renderer != DTV_RENDERER_TILE && params->renderer != DTV_RENDERER_PROJECTION)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": renderer must be 0 (tile) or 1 (projection)");
+    const bool proj = params->renderer == DTV_RENDERER_PROJECTION;
     std::vector<BatchItem> items((size_t)n);
     std::vector<int64_t> item_V((size_t)n);
     int64_t blocks = 0, tiles = 0, pixels = 0;
@@ -316,6 +321,10 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
         if (!aligned16(it.out_image)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": out_image must be 16-byte aligned");
         BatchItem& B = items[(size_t)i];
         if (!make_uniforms(it.camera, *params, B.U)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": singular view matrix");
+        if (proj) {  // no early ray termination, no segment lengths
+            B.U.use_ert = 0;
+            B.U.linear_depth = 0;
+        }
         B.cells = m.cells;
         B.pos = m.positions;
         B.col = m.colors;
@@ -372,8 +381,17 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
     uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
     DTV_HIP(view->block_stats.ensure(sizeof(uint32_t) * 4 * (blocks + 1)));
+    if (proj) {
+        DTV_HIP(view->pt_info.ensure(sizeof(uint32_t) * nslots));
+        DTV_HIP(view->pt_vert.ensure(sizeof(float4) * 5 * nslots));
+        DTV_HIP(view->pt_col.ensure(sizeof(float4) * 5 * nslots));
+    }
     view->mark(DTV_STAGE_COUNT_PAIRS, 0, s);
-    launch_count_pairs(dI, n, blocks, view->counts.as<uint32_t>(), view->block_stats.as<uint32_t>(), s);
+    if (proj)
+        launch_pt_count(dI, n, blocks, view->pt_info.as<uint32_t>(), view->pt_vert.as<float4>(),
+                        view->pt_col.as<float4>(), view->counts.as<uint32_t>(), view->block_stats.as<uint32_t>(), s);
+    else
+        launch_count_pairs(dI, n, blocks, view->counts.as<uint32_t>(), view->block_stats.as<uint32_t>(), s);
     view->mark(DTV_STAGE_COUNT_PAIRS, 1, s);
     view->mark(DTV_STAGE_SCAN, 0, s);
     scan_exclusive_u32(view->counts.as<uint32_t>(), view->offsets.as<uint32_t>(), 256 * blocks,
@@ -411,8 +429,13 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->vals2.ensure(sizeof(uint32_t) * (K > 0 ? K : 1)));
     DTV_HIP(view->recs.ensure(2 * sizeof(uint4) * (K > 0 ? K : 1)));
     view->mark(DTV_STAGE_EMIT, 0, s);
-    launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
-                      view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), K, overflow, s);
+    if (proj)
+        launch_pt_emit(dI, n, blocks, view->pt_info.as<uint32_t>(), view->pt_vert.as<float4>(),
+                       view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(), view->keys.as<uint64_t>(),
+                       view->vals.as<uint32_t>(), K, overflow, s);
+    else
+        launch_emit_pairs(dI, n, blocks, view->offsets.as<uint32_t>(), view->counts.as<uint32_t>(),
+                          view->keys.as<uint64_t>(), view->vals.as<uint32_t>(), K, overflow, s);
     view->mark(DTV_STAGE_EMIT, 1, s);
     uint32_t mask = 0;
     // planned: a pass per depth digit (which of them vary is not known on the host; constant ones exit at once)
@@ -444,7 +467,11 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     view->mark(DTV_STAGE_RANGES, 0, s);
     launch_tile_ranges(route, K, n_dev, tiles, view->ranges.as<uint2>(), s);
     if (fwd_order) launch_tile_order(view->ranges.as<uint2>(), nullptr, tiles, view->tile_order.as<uint32_t>(), s);
-    launch_gather_records(dI, n, route, K, n_dev, view->recs.as<uint4>(), s);
+    if (proj)
+        launch_pt_gather(dI, n, view->pt_info.as<uint32_t>(), view->pt_vert.as<float4>(), route, K, n_dev,
+                         view->recs.as<uint4>(), s);
+    else
+        launch_gather_records(dI, n, route, K, n_dev, view->recs.as<uint4>(), s);
     view->mark(DTV_STAGE_RANGES, 1, s);
 
     // 6. render
@@ -455,11 +482,17 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->ckpt_t.ensure(sizeof(float) * pixels * kMaxCheckpoints));
     DTV_HIP(view->ckpt_n.ensure(sizeof(uint32_t) * pixels * kMaxCheckpoints));
     view->mark(DTV_STAGE_FORWARD, 0, s);
-    launch_render_forward(dI, n, tiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
-                          view->pixel_state(), kinds, s);
+    if (proj)
+        launch_pt_forward(dI, n, tiles, fwd_order, view->pt_info.as<uint32_t>(), view->pt_vert.as<float4>(),
+                          view->pt_col.as<float4>(), view->sorted_recs(), view->ranges.as<uint2>(),
+                          view->pixel_state(), s);
+    else
+        launch_render_forward(dI, n, tiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
+                              view->pixel_state(), kinds, s);
     view->mark(DTV_STAGE_FORWARD, 1, s);
     DTV_HIP(hipGetLastError());
     view->colour_kinds = kinds;
+    view->projection = proj;
     view->items.swap(items);
     view->item_V.swap(item_V);
     view->total_blocks = blocks;
@@ -480,6 +513,10 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         return fail(DTV_ERR_INVALID_ARGUMENT,
                     f + ": the reference adjoint mode restates the INTERSECTION renderer's adjoint and needs "
                         "Euclidean segment lengths");
+    if (adjoint_mode == DTV_ADJOINT_REFERENCE && view->projection)
+        return fail(DTV_ERR_INVALID_ARGUMENT,
+                    f + ": the Projection renderer's adjoint is the exact derivative of its forward "
+                        "(DTV_ADJOINT_EXACT)");
     if (adjoint_mode == DTV_ADJOINT_REFERENCE && (view->colour_kinds & kColoursPerCell))
         return fail(DTV_ERR_INVALID_ARGUMENT,
                     f + ": the reference adjoint mode restates the per-vertex colour adjoint; per-cell colours "
@@ -527,9 +564,18 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         adj_order = o;
     }
     view->mark(DTV_STAGE_ADJOINT, 0, s);
-    launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
-                          adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps, use_abs_grad ? 1 : 0,
-                          adjoint_mode, view->colour_kinds, s);
+    if (view->projection) {
+        const int64_t tg_floats = 21 * 4 * 256 * view->total_blocks;
+        DTV_HIP(view->tri_grad.ensure(sizeof(float) * (size_t)(tg_floats > 0 ? tg_floats : 1)));
+        launch_pt_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n, view->total_tiles,
+                          view->total_blocks, adj_order, view->pt_info.as<uint32_t>(), view->pt_vert.as<float4>(),
+                          view->pt_col.as<float4>(), view->sorted_recs(), view->ranges.as<uint2>(), ps,
+                          view->tri_grad.as<float>(), tg_floats, use_abs_grad ? 1 : 0, s);
+    } else {
+        launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n,
+                              view->total_tiles, adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps,
+                              use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s);
+    }
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;
@@ -597,7 +643,8 @@ int dtv_view_destroy(dtv_view* v) {
     DevBuf* bufs[] = {&v->dItems, &v->dTargets, &v->counts, &v->offsets, &v->scan_status, &v->small, &v->block_stats, &v->keys,
                       &v->vals, &v->keys2,
                       &v->vals2, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
-                      &v->num_ckpt, &v->ckpt_t, &v->ckpt_n, &v->tile_frags, &v->tile_order};
+                      &v->num_ckpt, &v->ckpt_t, &v->ckpt_n, &v->tile_frags, &v->tile_order, &v->pt_info,
+                      &v->pt_vert, &v->pt_col, &v->tri_grad};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < DTV_NUM_STAGES; i++)
         for (int k = 0; k < 2; k++)
@@ -640,6 +687,8 @@ int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* cam
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_order must be 0 (heaviest first) or 1 (row-major)");
     if (params->segment_length != DTV_SEGMENT_EUCLIDEAN && params->segment_length != DTV_SEGMENT_LINEAR_DEPTH)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
+    if (params->renderer != DTV_RENDERER_TILE)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": renders with the tile renderer only (renderer 0)");
     if (!valid_camera(*camera)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid viewport, fovy or clip planes");
     const dtv_mesh& m = *mesh;
     if (const int rc = check_mesh(m, f); rc != DTV_OK) return rc;
@@ -790,6 +839,9 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
     if (!view || !mesh || !out_ids || max_per_pixel <= 0)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: invalid argument");
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_view_export_fragments: view has not been rendered");
+    if (view->projection)
+        return fail(DTV_ERR_INVALID_ARGUMENT,
+                    "dtv_view_export_fragments: the Projection renderer's fragments are triangles, not tet faces");
     if (view->nitems() != 1 || mesh->cells != view->items[0].cells || mesh->num_cells != view->items[0].T)
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: mesh differs from the rendered one");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -10,148 +10,9 @@
 // (one thread per record, so the per-pixel edge tests are spread evenly over the GPU).
 #include "dtv_kernels.h"
 #include "dtv_prims.h"
+#include "raster.h"
 
 namespace dtv {
-
-struct FaceBins {
-    int32_t tx0, tx1, ty0, ty1;  // tile range of the face's pixel-centre bounding box (empty: tx0 > tx1)
-    ScreenTri S;
-};
-
-__device__ __forceinline__ FaceBins face_bins(const ViewUniforms& U, const ProjectedTet& T, int f) {
-    FaceBins b;
-    b.tx0 = 0;
-    b.tx1 = -1;
-    b.ty0 = 0;
-    b.ty1 = -1;
-    face_screen(U, T, f, b.S);
-    int32_t x0, x1, y0, y1;
-    tri_pixel_bbox(U, b.S, x0, x1, y0, y1);
-    if (x0 > x1 || y0 > y1) return b;
-    b.tx0 = x0 >> 3;
-    b.tx1 = x1 >> 3;
-    b.ty0 = y0 >> 3;
-    b.ty1 = y1 >> 3;
-    return b;
-}
-
-// Edge e of the oriented screen triangle: E(px, py) = dx (py - Y_a) - dy (px - X_a) in 1/256-pixel units; a
-// pixel centre is inside when every E > 0, or E == 0 on a tie edge (dy < 0, or dy == 0 and dx > 0).
-struct TileEdge {
-    long long C;       // E at the centre of the tile's first pixel
-    long long sx, sy;  // change of E per pixel step in x and y
-    bool tie;
-};
-
-__device__ __forceinline__ TileEdge tile_edge(const ScreenTri& S, int e, int tx, int ty) {
-    const int a = e, b = e == 2 ? 0 : e + 1;
-    const long long dx = S.X[b] - S.X[a], dy = S.Y[b] - S.Y[a];
-    const long long px0 = 256LL * (8 * tx) + 128, py0 = 256LL * (8 * ty) + 128;
-    TileEdge t;
-    t.C = dx * (py0 - S.Y[a]) - dy * (px0 - S.X[a]);
-    t.sx = -dy * 256;
-    t.sy = dx * 256;
-    t.tie = dy < 0 || (dy == 0 && dx > 0);
-    return t;
-}
-
-// False when some edge function is negative at every pixel centre of tile (tx, ty): a record of that tile
-// could not cover a pixel.  (The maximum of E over the tile's centre lattice sits at a corner.)
-__device__ __forceinline__ bool tile_may_cover(const ScreenTri& S, int tx, int ty) {
-#pragma unroll
-    for (int e = 0; e < 3; e++) {
-        const TileEdge t = tile_edge(S, e, tx, ty);
-        const long long mx = t.C + (t.sx > 0 ? 7 * t.sx : 0) + (t.sy > 0 ? 7 * t.sy : 0);
-        if (mx < 0) return false;
-    }
-    return true;
-}
-
-// Faces whose bounding box spans more tiles than this keep every tile of it (bounded per-thread work).
-constexpr int kMaxTestedTiles = 64;
-
-__device__ __forceinline__ bool test_tiles(const FaceBins& b) {
-    return (b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1) <= kMaxTestedTiles;
-}
-
-__device__ __forceinline__ uint32_t face_record_count(const FaceBins& b) {
-    if (b.tx0 > b.tx1 || b.ty0 > b.ty1) return 0u;
-    if (!test_tiles(b)) return (uint32_t)((b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1));
-    uint32_t c = 0;
-    for (int ty = b.ty0; ty <= b.ty1; ty++)
-        for (int tx = b.tx0; tx <= b.tx1; tx++) c += tile_may_cover(b.S, tx, ty) ? 1u : 0u;
-    return c;
-}
-
-// Exact coverage of tile (tx, ty): bit ly * 8 + lx is set when the centre of pixel (8 tx + lx, 8 ty + ly) is
-// inside the triangle; pixels outside the viewport are clear.  Edges whose extremes over the tile decide every
-// pixel at once are not evaluated per pixel.
-__device__ __forceinline__ uint64_t tile_coverage(const ScreenTri& S, int tx, int ty, int width, int height) {
-    const int nx = min(8, width - 8 * tx), ny = min(8, height - 8 * ty);
-    const uint64_t row = nx >= 8 ? 0xFFull : ((1ull << nx) - 1ull);
-    uint64_t m = 0;
-    for (int ly = 0; ly < ny; ly++) m |= row << (8 * ly);
-#pragma unroll 1
-    for (int e = 0; e < 3; e++) {
-        const TileEdge t = tile_edge(S, e, tx, ty);
-        const long long ex = 7 * t.sx, ey = 7 * t.sy;
-        const long long emax = t.C + (ex > 0 ? ex : 0) + (ey > 0 ? ey : 0);
-        const long long emin = t.C + (ex < 0 ? ex : 0) + (ey < 0 ? ey : 0);
-        if (emax < 0 || (emax == 0 && !t.tie)) return 0ull;
-        if (emin > 0 || (emin == 0 && t.tie)) continue;  // the edge admits every pixel of the tile
-        uint64_t em = 0;
-        // sx = -256 dy and sy = 256 dx, so E = 256 W + r with W = q + ly dx - lx dy, q = floor(C / 256),
-        // r = C mod 256: E > 0 or (E == 0 on a tie edge)  <=>  W >= wmin.  The edge crosses the tile, so
-        // emin <= 0 <= emax bounds |E| by 7 (|sx| + |sy|) and |W| by 7 (|dx| + |dy|) + 1 < 2^31 for screen
-        // coordinates below 2^26 (the guard band keeps them below 2^25.2): exact in 32-bit integers.
-        const long long dx = t.sy >> 8, dy = -(t.sx >> 8);
-        if (dx > -(1ll << 27) && dx < (1ll << 27) && dy > -(1ll << 27) && dy < (1ll << 27)) {
-            const int dxi = (int)dx, dyi = (int)dy;
-            const int r = (int)(t.C & 255);
-            const int wmin = (r > 0 || t.tie) ? 0 : 1;
-            const int q = (int)(t.C >> 8);
-            // only values at the tile's 64 pixel centres are formed (no step past the tile can overflow)
-#pragma unroll
-            for (int ly = 0; ly < 8; ly++) {
-                const int Wr = q + ly * dxi;
-                uint32_t rm = 0;
-#pragma unroll
-                for (int lx = 0; lx < 8; lx++) rm |= (uint32_t)(Wr - lx * dyi >= wmin) << lx;
-                em |= (uint64_t)rm << (8 * ly);
-            }
-        } else {
-            long long Er = t.C;
-#pragma unroll 1
-            for (int ly = 0; ly < 8; ly++) {
-                long long E = Er;
-                for (int lx = 0; lx < 8; lx++) {
-                    if (E > 0 || (E == 0 && t.tie)) em |= 1ull << (8 * ly + lx);
-                    E += t.sx;
-                }
-                Er += t.sy;
-            }
-        }
-        m &= em;
-        if (m == 0) return 0ull;
-    }
-    return m;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_and_u32(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v &= __shfl_xor(v, d, 64);
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
-    return v;
-}
 
 // One thread per tet slot of the batch's block grid; counts[slot] = records of the tet (all its faces).
 __global__ __launch_bounds__(256) void k_count_pairs(const BatchItem* __restrict__ items, int nitems,

@@ -90,6 +90,25 @@ void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets,
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 
+// proj.hip: the Projection renderer.  Per-tet buffers are indexed by the tet's slot in the batch's block grid
+// (block0 * 256 + tet): pt_info [slots], pt_vert / pt_col [5 * slots] (the five vertices' NDC xyz + thickness,
+// colour); tri_grad [21 * 4 * slots] (the adjoint's per-triangle-corner sums).  Records: recs[2i + 1] =
+// {triangle slot id (tet * 4 + k, item-local tet), coverage mask low, high, 0}.
+void launch_pt_count(const BatchItem* items, int nitems, int64_t nblocks, uint32_t* pt_info, float4* pt_vert,
+                     float4* pt_col, uint32_t* counts, uint32_t* block_stats, hipStream_t s);
+void launch_pt_emit(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* pt_info,
+                    const float4* pt_vert, const uint32_t* offsets, const uint32_t* counts, uint64_t* keys,
+                    uint32_t* vals, int64_t cap, uint32_t* overflow, hipStream_t s);
+void launch_pt_gather(const BatchItem* items, int nitems, const uint32_t* pt_info, const float4* pt_vert,
+                      const SortRoute* route, int64_t n, const uint64_t* n_dev, uint4* out, hipStream_t s);
+void launch_pt_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
+                       const uint32_t* pt_info, const float4* pt_vert, const float4* pt_col, const uint4* recs,
+                       const uint2* ranges, PixelState ps, hipStream_t s);
+void launch_pt_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
+                       int64_t nblocks, const uint32_t* order, const uint32_t* pt_info, const float4* pt_vert,
+                       const float4* pt_col, const uint4* recs, const uint2* ranges, PixelState ps, float* tri_grad,
+                       int64_t tri_grad_floats, int use_abs, hipStream_t s);
+
 // optim.hip
 struct AdamGroupDev {
     float* param;

@@ -1,0 +1,922 @@
+// Projection renderer (projected tetrahedra) on gfx950.
+//
+// Each tet in view becomes 3 or 4 screen triangles, Shirley and Tuchman's classes, with the thickness at the
+// thick vertex taken from the crossing of two edges' projections (GenerateTrianglesVTK.glsl:129-403).  The
+// triangles go through the tile renderer's binning, radix sort and per-tile compositing: one (tile, triangle)
+// record per 8x8 tile the triangle may cover, keyed by the NDC z of the triangle's centroid
+// (ComputeTriangleDepths.glsl:99-108), composited front to back.  The reference blends the same list back to
+// front with premultiplied alpha, which sums the same terms.  Each fragment contributes (c.rgb alpha, alpha),
+// alpha = 1 - exp(-c.a d), where colour c and thickness d are interpolated linearly in screen space
+// (ProjectedRasterization.glsl:100-105; the vertices carry w = 1).
+//
+// The adjoint walks each pixel back to front as the tet renderer's does (transmittance checkpoints and
+// division).  It sums each record's gradients with respect to its triangle's corner NDC xy, colours and
+// thicknesses (wave reductions into LDS, then one atomic per value into the triangle's slot).  A per-tet pass
+// carries them to the vertex positions and colours through the triangle construction, by forward-mode
+// derivatives of the same arithmetic.
+#include "raster.h"
+#include "render_common.h"
+
+namespace dtv {
+
+// ---------------------------------------------------------------------------------------------------------
+// Decisions of one tet.  pt_info word: bits 0-2 triangle count (0, 3, 4), bit 3 class 2, bit 4 flip, then the
+// two edges seg1[0], seg1[1], seg2[0], seg2[1] (2 bits each, bits 5-12).
+struct PtDec {
+    int ntri;
+    int s10, s11, s20, s21;
+    bool cls2, flip;
+};
+
+__device__ __forceinline__ uint32_t pt_pack(const PtDec& D) {
+    return (uint32_t)D.ntri | (D.cls2 ? 8u : 0u) | (D.flip ? 16u : 0u) | ((uint32_t)D.s10 << 5) |
+           ((uint32_t)D.s11 << 7) | ((uint32_t)D.s20 << 9) | ((uint32_t)D.s21 << 11);
+}
+__device__ __forceinline__ PtDec pt_unpack(uint32_t w) {
+    PtDec D;
+    D.ntri = (int)(w & 7u);
+    D.cls2 = (w >> 3) & 1u;
+    D.flip = (w >> 4) & 1u;
+    D.s10 = (int)((w >> 5) & 3u);
+    D.s11 = (int)((w >> 7) & 3u);
+    D.s20 = (int)((w >> 9) & 3u);
+    D.s21 = (int)((w >> 11) & 3u);
+    return D;
+}
+
+// Corner j of triangle k: projected-tetrahedra vertex 0-3 (the tet's) or 4 (the crossing); the triangle fans
+// of GenerateTrianglesVTK.glsl:334-400.
+__device__ __forceinline__ int pt_corner(const PtDec& D, int k, int j) {
+    const int a0 = D.flip ? D.s11 : D.s10, a1 = D.flip ? D.s10 : D.s11;
+    if (D.cls2) {  // fan {4, a0, b0, a1, b1, a0}
+        const int i = j == 0 ? 0 : k + j;
+        return i == 0 ? 4 : (i == 1 || i == 5) ? a0 : i == 2 ? D.s20 : i == 3 ? a1 : D.s21;
+    }
+    const int i = j == 0 ? 0 : k + j;  // fan {a1, a0, b0, b1, a0}
+    return i == 0 ? a1 : (i == 1 || i == 4) ? a0 : i == 2 ? D.s20 : D.s21;
+}
+
+// Float decisions (GenerateTrianglesVTK.glsl:140-273).  Beyond the reference: a tet with a vertex behind the
+// near plane or beyond the guard band is culled (its triangles would be undefined there).
+__device__ __forceinline__ PtDec pt_decide(const ViewUniforms& U, const float P[4][3]) {
+    PtDec D{};
+    D.ntri = 0;
+    float ndc[4][3];
+    bool cull = false;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float c[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            c[r] = ((U.viewProj[0 * 4 + r] * P[i][0] + U.viewProj[1 * 4 + r] * P[i][1]) +
+                    U.viewProj[2 * 4 + r] * P[i][2]) + U.viewProj[3 * 4 + r] * 1.0f;
+        if (!(c[3] >= U.znear)) cull = true;
+#pragma unroll
+        for (int k = 0; k < 3; k++) ndc[i][k] = c[k] / c[3];
+        if (!(fabsf(ndc[i][0]) <= kGuard) || !(fabsf(ndc[i][1]) <= kGuard)) cull = true;
+    }
+    if (cull) return D;
+    bool gx = true, lx = true, gy = true, ly = true, gz = true, lz = false;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        gx = gx && ndc[i][0] > 1.0f;
+        lx = lx && ndc[i][0] < -1.0f;
+        gy = gy && ndc[i][1] > 1.0f;
+        ly = ly && ndc[i][1] < -1.0f;
+        gz = gz && ndc[i][2] > 1.0f;
+        lz = lz || ndc[i][2] < 0.0f;
+    }
+    if (gx || lx || gy || ly || gz || lz) return D;
+    const float v1x = ndc[1][0] - ndc[0][0], v1y = ndc[1][1] - ndc[0][1];
+    const float v2x = ndc[2][0] - ndc[0][0], v2y = ndc[2][1] - ndc[0][1];
+    const float v3x = ndc[3][0] - ndc[0][0], v3y = ndc[3][1] - ndc[0][1];
+    const float fd1 = v3x * v2y - v3y * v2x;
+    const float fd2 = v1x * v3y - v1y * v3x;
+    const float fd3 = v2x * v1y - v2y * v1x;
+    if ((fd1 * fd2 >= 0.0f) && ((fd1 != 0.0f) || (fd2 != 0.0f))) {
+        D.s10 = 0; D.s11 = 3; D.s20 = 1; D.s21 = 2;
+    } else if (fd1 * fd3 >= 0.0f) {
+        D.s10 = 0; D.s11 = 2; D.s20 = 1; D.s21 = 3;
+    } else {
+        D.s10 = 0; D.s11 = 1; D.s20 = 2; D.s21 = 3;
+    }
+    const float Ax = ndc[D.s11][0] - ndc[D.s10][0], Ay = ndc[D.s11][1] - ndc[D.s10][1];
+    const float Bx = ndc[D.s21][0] - ndc[D.s20][0], By = ndc[D.s21][1] - ndc[D.s20][1];
+    const float Cx = ndc[D.s20][0] - ndc[D.s10][0], Cy = ndc[D.s20][1] - ndc[D.s10][1];
+    const float den = Ax * By - Ay * Bx;
+    if (fabsf(den) < 1e-8f) return D;  // degenerate
+    const float alpha = (By * Cx - Bx * Cy) / den;
+    D.cls2 = (alpha >= 0.0f) && (alpha <= 1.0f);
+    D.flip = !D.cls2 && alpha <= 0.0f;
+    D.ntri = D.cls2 ? 4 : 3;
+    return D;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Values of the five vertices (NDC xyz, colour, thickness) given the decisions, templated on the scalar: float
+// for the render (bit-identical to the decisions' arithmetic), Dual<3> for the adjoint's position derivatives.
+template <int N>
+struct Dual {
+    float v;
+    float d[N];
+};
+template <int N>
+__device__ __forceinline__ Dual<N> dconst(float v) {
+    Dual<N> r;
+    r.v = v;
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = 0.f;
+    return r;
+}
+#define DTV_DUAL_BIN(OP, VAL, DER)                                                        \
+    template <int N>                                                                      \
+    __device__ __forceinline__ Dual<N> operator OP(const Dual<N>& a, const Dual<N>& b) {  \
+        Dual<N> r;                                                                        \
+        r.v = VAL;                                                                        \
+        _Pragma("unroll") for (int i = 0; i < N; i++) r.d[i] = DER;                       \
+        return r;                                                                         \
+    }
+DTV_DUAL_BIN(+, a.v + b.v, a.d[i] + b.d[i])
+DTV_DUAL_BIN(-, a.v - b.v, a.d[i] - b.d[i])
+DTV_DUAL_BIN(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+DTV_DUAL_BIN(/, a.v / b.v, (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v))
+#undef DTV_DUAL_BIN
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(float s, const Dual<N>& a) {
+    Dual<N> r;
+    r.v = s * a.v;
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = s * a.d[i];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, float s) {
+    return s * a;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, float s) {
+    Dual<N> r = a;
+    r.v = a.v + s;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(float s, const Dual<N>& a) {
+    Dual<N> r;
+    r.v = s - a.v;
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = -a.d[i];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, float s) {
+    Dual<N> r = a;
+    r.v = a.v - s;
+    return r;
+}
+__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+template <int N>
+__device__ __forceinline__ Dual<N> tsqrt(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = sqrtf(a.v);
+    const float k = r.v > 0.f ? 0.5f / r.v : 0.f;
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = k * a.d[i];
+    return r;
+}
+__device__ __forceinline__ void tzero(float& x) { x = 0.f; }
+template <int N>
+__device__ __forceinline__ void tzero(Dual<N>& a) {
+    a = dconst<N>(0.f);
+}
+
+template <class T>
+struct PtVerts {
+    T ndc[5][3];
+    T col[5][4];
+    T dep[5];
+};
+
+// GetCorrectedDepth without the linear depth correction, which the renderer never enables
+// (GenerateTrianglesVTK.glsl:83-126, TetMeshRendererProjection.cpp:800-808).
+template <class T>
+__device__ __forceinline__ T pt_corrected_depth(const float* iP, T x, T y, T z1, T z2) {
+    T e1[4], e2[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        e1[r] = ((iP[0 * 4 + r] * x + iP[1 * 4 + r] * y) + iP[2 * 4 + r] * z1) + iP[3 * 4 + r] * 1.0f;
+        e2[r] = ((iP[0 * 4 + r] * x + iP[1 * 4 + r] * y) + iP[2 * 4 + r] * z2) + iP[3 * 4 + r] * 1.0f;
+    }
+    const T d0 = e1[0] / e1[3] - e2[0] / e2[3];
+    const T d1 = e1[1] / e1[3] - e2[1] / e2[3];
+    const T d2 = e1[2] / e1[3] - e2[2] / e2[3];
+    return tsqrt((d0 * d0 + d1 * d1) + d2 * d2);
+}
+
+template <class T>
+__device__ __forceinline__ void pt_vertices(const PtDec& D, const ViewUniforms& U, const T P[4][3], const T C[4][4],
+                                            bool per_cell, PtVerts<T>& V) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        T c[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            c[r] = ((U.viewProj[0 * 4 + r] * P[i][0] + U.viewProj[1 * 4 + r] * P[i][1]) +
+                    U.viewProj[2 * 4 + r] * P[i][2]) + U.viewProj[3 * 4 + r] * 1.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) V.ndc[i][k] = c[k] / c[3];
+#pragma unroll
+        for (int k = 0; k < 4; k++) V.col[i][k] = C[i][k];
+        tzero(V.dep[i]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) tzero(V.ndc[4][k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) tzero(V.col[4][k]);
+    tzero(V.dep[4]);
+    int a0 = D.s10, a1 = D.s11;
+    const int b0 = D.s20, b1 = D.s21;
+    T A[3], B[3], Cv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        A[k] = V.ndc[a1][k] - V.ndc[a0][k];
+        B[k] = V.ndc[b1][k] - V.ndc[b0][k];
+        Cv[k] = V.ndc[b0][k] - V.ndc[a0][k];
+    }
+    const T den = A[0] * B[1] - A[1] * B[0];
+    T alpha = (B[1] * Cv[0] - B[0] * Cv[1]) / den;
+    const T beta = (A[1] * Cv[0] - A[0] * Cv[1]) / den;
+    if (D.cls2) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) V.ndc[4][k] = V.ndc[a0][k] + alpha * A[k];
+        const T depth =
+            pt_corrected_depth(U.invProj, V.ndc[4][0], V.ndc[4][1], V.ndc[4][2], V.ndc[b0][2] + beta * B[2]);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            V.col[4][k] = per_cell ? C[0][k]
+                                   : 0.5f * (((C[a0][k] + alpha * (C[a1][k] - C[a0][k])) + C[b0][k]) +
+                                             beta * (C[b1][k] - C[b0][k]));
+        V.dep[4] = depth * U.attenuation;
+    } else {
+        if (D.flip) {  // segment 1 reversed: the thick point is its second vertex
+            const int t = a0;
+            a0 = a1;
+            a1 = t;
+            alpha = 1.0f - alpha;
+        }
+        const T edgez = V.ndc[b0][2] + beta * B[2];
+        const T pointz = V.ndc[a0][2];
+        const T facez = (edgez + (alpha - 1.0f) * pointz) / alpha;
+        const T depth = pt_corrected_depth(U.invProj, V.ndc[a1][0], V.ndc[a1][1], V.ndc[a1][2], facez);
+        if (!per_cell) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const T edgec = C[b0][k] + beta * (C[b1][k] - C[b0][k]);
+                const T pointc = C[a0][k];
+                const T facec = (edgec + (alpha - 1.0f) * pointc) / alpha;
+                V.col[a1][k] = 0.5f * (facec + C[a1][k]);
+            }
+        }
+        V.dep[a1] = depth * U.attenuation;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Per-tet state between the passes: pt_vert[5 * t + v] = {NDC x, y, z, thickness}, pt_col[5 * t + v] = colour.
+
+__device__ __forceinline__ void pt_load_tet(const BatchItem& B, int64_t tet, float P[4][3], float C[4][4],
+                                            int32_t vid[4]) {
+    const int4 idx = *reinterpret_cast<const int4*>(B.cells + 4 * tet);
+    vid[0] = idx.x;
+    vid[1] = idx.y;
+    vid[2] = idx.z;
+    vid[3] = idx.w;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float3_ p = load_pos(B.pos, vid[i]);
+        P[i][0] = p.x;
+        P[i][1] = p.y;
+        P[i][2] = p.z;
+        const float4 c = B.cell_col ? *reinterpret_cast<const float4*>(B.cell_col + 4 * tet)
+                                    : *reinterpret_cast<const float4*>(B.col + 4 * (int64_t)vid[i]);
+        C[i][0] = c.x;
+        C[i][1] = c.y;
+        C[i][2] = c.z;
+        C[i][3] = c.w;
+    }
+}
+
+// Fixed-point screen triangle (oriented to positive area) and centroid key of triangle k.
+__device__ __forceinline__ void pt_screen(const ViewUniforms& U, const float4* __restrict__ pv, const PtDec& D, int k,
+                                          ScreenTri& S, uint32_t& key) {
+    const float hw = 0.5f * (float)U.width, hh = 0.5f * (float)U.height;
+    float z[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const float4 q = pv[pt_corner(D, k, j)];
+        const float xf = q.x * hw + hw;
+        const float yf = q.y * hh + hh;
+        S.X[j] = (int32_t)rintf(xf * 256.0f);
+        S.Y[j] = (int32_t)rintf(yf * 256.0f);
+        z[j] = q.z;
+    }
+    key = __float_as_uint(((z[0] + z[1]) + z[2]) / 3.0f);
+    int64_t area = (int64_t)(S.X[1] - S.X[0]) * (S.Y[2] - S.Y[0]) - (int64_t)(S.Y[1] - S.Y[0]) * (S.X[2] - S.X[0]);
+    if (area < 0) {
+        const int32_t tx = S.X[1], ty = S.Y[1];
+        S.X[1] = S.X[2];
+        S.Y[1] = S.Y[2];
+        S.X[2] = tx;
+        S.Y[2] = ty;
+        area = -area;
+    }
+    S.area2 = area;
+}
+
+// One thread per tet slot: decisions, the five vertices, and the record count of its triangles.
+__global__ __launch_bounds__(256) void k_pt_count(const BatchItem* __restrict__ items, int nitems,
+                                                   uint32_t* __restrict__ pt_info, float4* __restrict__ pt_vert,
+                                                   float4* __restrict__ pt_col, uint32_t* __restrict__ counts,
+                                                   uint4* __restrict__ block_stats) {
+    const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
+    const ViewUniforms& U = B.U;
+    const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
+    uint32_t c = 0, ntri = 0, kand = 0xFFFFFFFFu, kor = 0u;
+    if (tet < B.T) {
+        float P[4][3], C[4][4];
+        int32_t vid[4];
+        pt_load_tet(B, tet, P, C, vid);
+        const PtDec D = pt_decide(U, P);
+        const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;  // the tet's slot in the batch
+        pt_info[g] = pt_pack(D);
+        if (D.ntri) {
+            PtVerts<float> V;
+            pt_vertices(D, U, P, C, B.cell_col != nullptr, V);
+            float4 pv[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                pv[v] = make_float4(V.ndc[v][0], V.ndc[v][1], V.ndc[v][2], V.dep[v]);
+                pt_vert[5 * g + v] = pv[v];
+                pt_col[5 * g + v] = make_float4(V.col[v][0], V.col[v][1], V.col[v][2], V.col[v][3]);
+            }
+            for (int k = 0; k < D.ntri; k++) {
+                ScreenTri S;
+                uint32_t key;
+                pt_screen(U, pv, D, k, S, key);
+                const uint32_t cf = face_record_count(screen_bins(U, S));
+                c += cf;
+                ntri++;
+                if (cf) {
+                    kand &= key;
+                    kor |= key;
+                }
+            }
+        }
+    }
+    counts[(int64_t)blockIdx.x * 256 + threadIdx.x] = c;
+    __shared__ uint4 s_part[4];
+    const uint32_t wtri = wave_sum_u32(ntri), wand = wave_and_u32(kand), wor = wave_or_u32(kor);
+    if (lane_id() == 0) s_part[threadIdx.x / 64] = make_uint4(wtri, wand, wor, 0u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint4 r = s_part[0];
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) {
+            r.x += s_part[w].x;
+            r.y &= s_part[w].y;
+            r.z |= s_part[w].z;
+        }
+        block_stats[blockIdx.x] = r;
+    }
+}
+
+// Same grid: the tet's records at offsets[slot], triangles in order, tiles row-major; vals = tet * 4 + k.
+__global__ __launch_bounds__(256) void k_pt_emit(const BatchItem* __restrict__ items, int nitems,
+                                                  const uint32_t* __restrict__ pt_info,
+                                                  const float4* __restrict__ pt_vert,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ counts, uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals, uint64_t cap,
+                                                  uint32_t* __restrict__ overflow) {
+    const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
+    const ViewUniforms& U = B.U;
+    const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
+    if (tet >= B.T) return;
+    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t left = counts[slot];
+    if (left == 0) return;
+    uint32_t o = offsets[slot];
+    const PtDec D = pt_unpack(pt_info[slot]);
+    float4 pv[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) pv[v] = pt_vert[5 * slot + v];
+    for (int k = 0; k < D.ntri && left > 0; k++) {
+        ScreenTri S;
+        uint32_t key;
+        pt_screen(U, pv, D, k, S, key);
+        const FaceBins b = screen_bins(U, S);
+        const bool test = test_tiles(b);
+        const uint32_t id = (uint32_t)(tet * 4 + k);
+        for (int ty = b.ty0; ty <= b.ty1 && left > 0; ty++)
+            for (int tx = b.tx0; tx <= b.tx1 && left > 0; tx++) {
+                if (test && !tile_may_cover(b.S, tx, ty)) continue;
+                if (o < cap) {
+                    keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | (uint64_t)key;
+                    vals[o] = id;
+                } else {
+                    *overflow = 1u;
+                }
+                o++;
+                left--;
+            }
+    }
+}
+
+// Sorted record i: out[2i + 1] = {triangle id, coverage mask low, high, 0} (out[2i] is not used).
+__global__ __launch_bounds__(256) void k_pt_gather(const BatchItem* __restrict__ items, int nitems,
+                                                    const uint32_t* __restrict__ pt_info,
+                                                    const float4* __restrict__ pt_vert,
+                                                    const SortRoute* __restrict__ route, int64_t n_cap,
+                                                    const uint64_t* __restrict__ n_dev, uint4* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= run_count(n_cap, n_dev)) return;
+    const uint32_t id = route->vals_out[i];
+    const uint32_t gtile = (uint32_t)(route->keys_out[i] >> 32);
+    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    const ViewUniforms& U = B.U;
+    const int tile = (int)(gtile - B.tile0);
+    const int64_t g = (int64_t)B.block0 * 256 + (id >> 2);
+    const PtDec D = pt_unpack(pt_info[g]);
+    ScreenTri S;
+    uint32_t key;
+    pt_screen(U, pt_vert + 5 * g, D, (int)(id & 3), S, key);
+    const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
+    out[2 * i + 1] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Tile compositing.  A batch of 32 records is staged in LDS: each triangle's corner NDC xy, colours,
+// thicknesses, 1 / (twice its NDC area) and coverage.
+struct PtRecLds {
+    float X[3][kBatch], Y[3][kBatch];
+    float c[12][kBatch];  // corner j colour channel ch at [4 j + ch]
+    float d[3][kBatch];
+    float invD[kBatch];
+    uint32_t mask[2][kBatch];
+    uint32_t id[kBatch];
+};
+
+// slot0: the item's first tet slot in the batch (block0 * 256).
+__device__ __forceinline__ void pt_stage(const float4* __restrict__ pt_vert, const float4* __restrict__ pt_col,
+                                         const uint32_t* __restrict__ pt_info, int64_t slot0, uint4 rec, PtRecLds& L,
+                                         int i) {
+    const uint32_t id = rec.x;
+    const int64_t tet = slot0 + (id >> 2);
+    const PtDec D = pt_unpack(pt_info[tet]);
+    float X[3], Y[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int v = pt_corner(D, (int)(id & 3), j);
+        const float4 q = pt_vert[5 * tet + v];
+        const float4 c = pt_col[5 * tet + v];
+        X[j] = q.x;
+        Y[j] = q.y;
+        L.X[j][i] = q.x;
+        L.Y[j][i] = q.y;
+        L.d[j][i] = q.w;
+        L.c[4 * j + 0][i] = c.x;
+        L.c[4 * j + 1][i] = c.y;
+        L.c[4 * j + 2][i] = c.z;
+        L.c[4 * j + 3][i] = c.w;
+    }
+    L.invD[i] = 1.0f / ((X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]));
+    L.mask[0][i] = rec.y;
+    L.mask[1][i] = rec.z;
+    L.id[i] = (uint32_t)(4 * tet) + (id & 3u);  // the triangle's slot in the batch
+}
+
+// One fragment: barycentrics of the pixel centre (px, py) (NDC), interpolated colour c and thickness d,
+// A = exp(-c.a d), F = ((1 - A) c.rgb, A) (premultiplied RGB, transmittance).
+struct PtFrag {
+    float l[3];
+    float c[4];
+    float d;
+    float A;
+};
+__device__ __forceinline__ void pt_eval(const PtRecLds& L, int j, float px, float py, PtFrag& f) {
+    const float X0 = L.X[0][j], X1 = L.X[1][j], X2 = L.X[2][j];
+    const float Y0 = L.Y[0][j], Y1 = L.Y[1][j], Y2 = L.Y[2][j];
+    const float iD = L.invD[j];
+    f.l[0] = ((X1 - px) * (Y2 - py) - (X2 - px) * (Y1 - py)) * iD;
+    f.l[1] = ((X2 - px) * (Y0 - py) - (X0 - px) * (Y2 - py)) * iD;
+    f.l[2] = ((X0 - px) * (Y1 - py) - (X1 - px) * (Y0 - py)) * iD;
+    {
+#pragma clang fp contract(fast)
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++)
+            f.c[ch] = (f.l[0] * L.c[ch][j] + f.l[1] * L.c[4 + ch][j]) + f.l[2] * L.c[8 + ch][j];
+        f.d = (f.l[0] * L.d[0][j] + f.l[1] * L.d[1][j]) + f.l[2] * L.d[2][j];
+    }
+    f.A = __expf(-f.c[3] * f.d);
+}
+
+__device__ __forceinline__ bool pt_covers(const PtRecLds& L, int j, int lane) {
+    return (L.mask[lane >> 5][j] >> (lane & 31)) & 1u;
+}
+
+constexpr float kPtTinyT = 0x1p-100f;  // transmittance below which a fragment's input is not recovered
+
+__global__ __launch_bounds__(64) void k_pt_forward(const BatchItem* __restrict__ items, int nitems,
+                                                    const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ pt_info,
+                                                    const float4* __restrict__ pt_vert,
+                                                    const float4* __restrict__ pt_col, const uint4* __restrict__ recs,
+                                                    const uint2* __restrict__ ranges, PixelState ps_all) {
+    __shared__ PtRecLds L;
+    const uint32_t gtile = global_tile_of_block(order);
+    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    const ViewUniforms& U = B.U;
+    float4* __restrict__ out_image = reinterpret_cast<float4*>(B.image);
+    const PixelState ps = item_pixel_state(ps_all, B);
+    const int tile = (int)(gtile - B.tile0);
+    const int lane = threadIdx.x;
+    const int x = (tile % U.tiles_x) * kTileEdge + (lane & 7), y = (tile / U.tiles_x) * kTileEdge + (lane >> 3);
+    const bool valid = x < U.width && y < U.height;
+    const float px = 2.0f * (((float)x + 0.5f) / (float)U.width) - 1.0f;
+    const float py = 2.0f * (((float)y + 0.5f) / (float)U.height) - 1.0f;
+    const uint2 rg = ranges[gtile];
+    const int P = U.width * U.height;
+    const int pix = valid ? pixel_index(U, x, y) : 0;
+    float4 C = make_float4(0.f, 0.f, 0.f, 1.0f);  // premultiplied RGB, transmittance
+    uint32_t n = 0, nck = 0, nlast = 0, last = rg.x;
+    for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
+        const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
+        if (lane < cnt)
+            pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, recs[2 * (int64_t)(base + lane) + 1], L,
+                     lane);
+        __syncthreads();
+        for (int j = 0; j < cnt; j++) {
+            if (!valid || !pt_covers(L, j, lane)) continue;
+            PtFrag f;
+            pt_eval(L, j, px, py, f);
+            const float Tout = C.w * f.A;
+            // checkpoint rules of the tet renderer: before an underflow, and periodically
+            if (C.w >= kPtTinyT && Tout < kPtTinyT && nlast != n && (int)nck < U.max_ckpt) {
+                ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
+                ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                nck++;
+                nlast = n;
+            }
+            {
+#pragma clang fp contract(fast)
+                const float a = 1.0f - f.A;
+                C.x = C.x + C.w * (a * f.c[0]);
+                C.y = C.y + C.w * (a * f.c[1]);
+                C.z = C.z + C.w * (a * f.c[2]);
+            }
+            C.w = Tout;
+            n++;
+            last = base + (uint32_t)j + 1;
+            if ((int)(n - nlast) >= U.ckpt_interval && C.w >= kPtTinyT && (int)nck < U.max_ckpt - 1) {
+                ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
+                ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                nck++;
+                nlast = n;
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        const float T = C.w;
+        out_image[pix] = make_float4(C.x + T * U.clear[0], C.y + T * U.clear[1], C.z + T * U.clear[2],
+                                     (1.0f - T) + T * U.clear[3]);
+        ps.ray_color[pix] = C;
+        ps.frag_count[pix] = n;
+        ps.last_rec[pix] = last;
+        ps.num_ckpt[pix] = nck;
+    }
+    const uint32_t tile_total = wave_inclusive_scan(n);
+    if (lane == 63) ps.tile_frags[gtile] = tile_total;
+}
+
+// Adjoint: per pixel back to front; per record the 21 gradient sums (corner j: -lambda_j G.x, -lambda_j G.y,
+// lambda_j dL/dc, lambda_j dL/dd) into tri_grad[21 * id] with one atomic per nonzero value.
+__global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__ items,
+                                                    const AdjointTarget* __restrict__ targets, int nitems,
+                                                    const uint32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ pt_info,
+                                                    const float4* __restrict__ pt_vert,
+                                                    const float4* __restrict__ pt_col, const uint4* __restrict__ recs,
+                                                    const uint2* __restrict__ ranges, PixelState ps_all,
+                                                    float* __restrict__ tri_grad, int use_abs) {
+    __shared__ PtRecLds L;
+    __shared__ float s_acc[kBatch][21];
+    const uint32_t gtile = global_tile_of_block(order);
+    const int item = item_of_tile(items, nitems, gtile);
+    const BatchItem& B = items[item];
+    const ViewUniforms& U = B.U;
+    const float4* __restrict__ dimage = reinterpret_cast<const float4*>(targets[item].dimage);
+    if (dimage == nullptr) return;
+    const PixelState ps = item_pixel_state(ps_all, B);
+    const int tile = (int)(gtile - B.tile0);
+    const int lane = threadIdx.x;
+    const int x = (tile % U.tiles_x) * kTileEdge + (lane & 7), y = (tile / U.tiles_x) * kTileEdge + (lane >> 3);
+    const bool valid = x < U.width && y < U.height;
+    const float px = 2.0f * (((float)x + 0.5f) / (float)U.width) - 1.0f;
+    const float py = 2.0f * (((float)y + 0.5f) / (float)U.height) - 1.0f;
+    const uint2 rg = ranges[gtile];
+    const int P = U.width * U.height;
+    const int pix = valid ? pixel_index(U, x, y) : 0;
+    float Cw = 1.0f;
+    float dC[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t n = 0, last = 0;
+    int jck = -1;
+    uint32_t ck_n = 0xFFFFFFFFu, ck_n2 = 0xFFFFFFFFu;
+    float ck_t = 1.0f;
+    if (valid) {
+        Cw = ps.ray_color[pix].w;
+        n = ps.frag_count[pix];
+        last = ps.last_rec[pix];
+        jck = (int)ps.num_ckpt[pix] - 1;
+        if (jck >= 0) {
+            ck_n = ps.ckpt_n[(int64_t)jck * P + pix];
+            ck_t = ps.ckpt_t[(int64_t)jck * P + pix];
+            if (jck >= 1) ck_n2 = ps.ckpt_n[(int64_t)(jck - 1) * P + pix];
+        }
+        const float4 di = dimage[pix];
+        dC[0] = di.x;
+        dC[1] = di.y;
+        dC[2] = di.z;
+        dC[3] = di.w * (1.0f - U.clear[3]) - (di.x * U.clear[0] + di.y * U.clear[1] + di.z * U.clear[2]);
+    }
+    if (ballot(valid && n > 0) == 0ull) return;
+    const uint32_t nrec = rg.y - rg.x;
+    const int nbatch = (int)((nrec + kBatch - 1) / kBatch);
+    for (int bi = nbatch - 1; bi >= 0; bi--) {
+        const uint32_t base = rg.x + (uint32_t)bi * kBatch;
+        const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
+        if (lane < cnt)
+            pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, recs[2 * (int64_t)(base + lane) + 1], L,
+                     lane);
+        for (int e = lane; e < kBatch * 21; e += 64) (&s_acc[0][0])[e] = 0.f;
+        __syncthreads();
+        for (int j = cnt - 1; j >= 0; j--) {
+            const bool live = valid && n > 0 && base + (uint32_t)j < last && pt_covers(L, j, lane);
+            if (ballot(live) == 0ull) continue;
+            float g[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            float lam[3] = {0.f, 0.f, 0.f};
+            if (live) {
+                PtFrag f;
+                pt_eval(L, j, px, py, f);
+                if (jck >= 0 && ck_n == n) {
+                    Cw = ck_t;
+                    jck--;
+                    ck_n = ck_n2;
+                    if (jck >= 0) {
+                        ck_t = ps.ckpt_t[(int64_t)jck * P + pix];
+                        ck_n2 = jck >= 1 ? ps.ckpt_n[(int64_t)(jck - 1) * P + pix] : 0xFFFFFFFFu;
+                    }
+                }
+                float Tin;
+                if (jck >= 0 && ck_n == n - 1) {
+                    Tin = ck_t;
+                } else if (n == 1) {
+                    Tin = 1.0f;
+                } else {
+                    Tin = Cw < kPtTinyT ? 0.0f : div_nr(Cw, f.A);
+                }
+                {
+#pragma clang fp contract(fast)
+                const float a = 1.0f - f.A;
+                // dL/dF (alpha form in .w), then the composite's adjoint moves past this fragment
+                const float dF[4] = {Tin * dC[0], Tin * dC[1], Tin * dC[2], Tin * dC[3]};
+                dC[3] = dC[3] * f.A - (dC[0] * (a * f.c[0]) + dC[1] * (a * f.c[1]) + dC[2] * (a * f.c[2]));
+                Cw = Tin;
+                n--;
+                // F = (alpha c.rgb, alpha), alpha = 1 - exp(-c.a d)
+                const float dal = dF[3] + (dF[0] * f.c[0] + dF[1] * f.c[1] + dF[2] * f.c[2]);
+                float dc[4] = {a * dF[0], a * dF[1], a * dF[2], dal * f.d * f.A};
+                const float dd = dal * f.c[3] * f.A;
+                // AbsGS-style absolute colour gradients: the per-cell variant only, as the reference's
+                // (AdjointProjectedRasterization.glsl:214-222)
+                const bool abs_col = use_abs && B.cell_col != nullptr;
+                // dL/dlambda_k = dc . c_k + dd d_k; moving corner k by delta (NDC xy) changes lambda by
+                // -lambda_k grad(lambda) . delta, so dL/d(corner k) = -lambda_k G, G = sum_k dL/dlambda_k grad lambda_k
+                float gl[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    gl[k] = ((dc[0] * L.c[4 * k][j] + dc[1] * L.c[4 * k + 1][j]) +
+                             (dc[2] * L.c[4 * k + 2][j] + dc[3] * L.c[4 * k + 3][j])) + dd * L.d[k][j];
+                const float X0 = L.X[0][j], X1 = L.X[1][j], X2 = L.X[2][j];
+                const float Y0 = L.Y[0][j], Y1 = L.Y[1][j], Y2 = L.Y[2][j];
+                const float iD = L.invD[j];
+                const float Gx = (gl[0] * (Y1 - Y2) + gl[1] * (Y2 - Y0) + gl[2] * (Y0 - Y1)) * iD;
+                const float Gy = (gl[0] * (X2 - X1) + gl[1] * (X0 - X2) + gl[2] * (X1 - X0)) * iD;
+                g[0] = -Gx;
+                g[1] = -Gy;
+                g[2] = abs_col ? fabsf(dc[0]) : dc[0];
+                g[3] = abs_col ? fabsf(dc[1]) : dc[1];
+                g[4] = abs_col ? fabsf(dc[2]) : dc[2];
+                g[5] = abs_col ? fabsf(dc[3]) : dc[3];
+                g[6] = dd;
+                lam[0] = f.l[0];
+                lam[1] = f.l[1];
+                lam[2] = f.l[2];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int q = 0; q < 7; q++) {
+                    const float s = wave_sum(lam[k] * g[q]);
+                    if (lane == 0) s_acc[j][7 * k + q] = s;
+                }
+        }
+        __syncthreads();
+        for (int e = lane; e < cnt * 21; e += 64) {
+            const int j = e / 21, q = e - 21 * j;
+            const float v = s_acc[j][q];
+            if (v != 0.f) atomic_add_f32(&tri_grad[21 * (int64_t)L.id[j] + q], v);
+        }
+        __syncthreads();
+    }
+}
+
+// Per tet: the gradients of its triangles' corners, summed per vertex, carried through the construction to
+// the positions (forward-mode derivatives, one pass per tet vertex) and colours (linear weights; per cell: the
+// sum over the corners).
+__global__ __launch_bounds__(256) void k_pt_backprop(const BatchItem* __restrict__ items,
+                                                      const AdjointTarget* __restrict__ targets, int nitems,
+                                                      const uint32_t* __restrict__ pt_info,
+                                                      const float* __restrict__ tri_grad) {
+    const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
+    const int item = item_of_block(items, nitems, blockIdx.x);
+    const ViewUniforms& U = B.U;
+    const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
+    if (tet >= B.T || targets[item].dimage == nullptr) return;
+    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const PtDec D = pt_unpack(pt_info[slot]);
+    if (D.ntri == 0) return;
+    float gv[5][7];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+        for (int q = 0; q < 7; q++) gv[v][q] = 0.f;
+    for (int k = 0; k < D.ntri; k++) {
+        const float* tg = tri_grad + 21 * (slot * 4 + k);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int v = pt_corner(D, k, j);
+#pragma unroll
+            for (int q = 0; q < 7; q++) {
+                const float val = tg[7 * j + q];
+#pragma unroll
+                for (int w = 0; w < 5; w++) gv[w][q] += w == v ? val : 0.f;
+            }
+        }
+    }
+    float P[4][3], C[4][4];
+    int32_t vid[4];
+    pt_load_tet(B, tet, P, C, vid);
+    const bool per_cell = B.cell_col != nullptr;
+    float* __restrict__ grad_pos = targets[item].grad_pos;
+    float* __restrict__ grad_col = targets[item].grad_col;
+    // the special vertex: 4 (class 2) or the thick vertex (class 1)
+    const int sv = D.cls2 ? 4 : (D.flip ? D.s10 : D.s11);
+    // positions: d(outputs)/d(P[i]) by Dual<3>, outputs = corner NDC xy (all vertices), the special vertex's
+    // thickness and (per vertex colours) colour
+#pragma unroll 1
+    for (int i = 0; i < 4; i++) {
+        Dual<3> Pd[4][3], Cd[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                Pd[a][k] = dconst<3>(P[a][k]);
+                if (a == i) Pd[a][k].d[k] = 1.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) Cd[a][k] = dconst<3>(C[a][k]);
+        }
+        PtVerts<Dual<3>> V;
+        pt_vertices(D, U, Pd, Cd, per_cell, V);
+        float gp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float s = 0.f;
+#pragma unroll
+            for (int v = 0; v < 5; v++) s += gv[v][0] * V.ndc[v][0].d[c] + gv[v][1] * V.ndc[v][1].d[c];
+            s += gv[sv][6] * V.dep[sv].d[c];
+            if (!per_cell) {
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++) s += gv[sv][2 + ch] * V.col[sv][ch].d[c];
+            }
+            gp[c] = s;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            if (gp[c] != 0.f) atomic_add_f32(&grad_pos[3 * (int64_t)vid[i] + c], gp[c]);
+    }
+    // colours: the corners' colours are the vertices' except the special vertex's, a linear combination
+    float gc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) gc[a][ch] = a == sv ? 0.f : gv[a][2 + ch];
+    if (per_cell) {
+        float s[4];
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) {
+            s[ch] = gv[0][2 + ch] + gv[1][2 + ch] + gv[2][2 + ch] + gv[3][2 + ch] + gv[4][2 + ch];
+            if (s[ch] != 0.f) atomic_add_f32(&grad_col[4 * tet + ch], s[ch]);
+        }
+        return;
+    }
+    // alpha, beta of the float construction (the weights of the special colour)
+    PtVerts<float> Vf;
+    pt_vertices(D, U, P, C, false, Vf);
+    float An[2], Bn[2], Cn[2];
+    float ndc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        ndc[a][0] = Vf.ndc[a][0];
+        ndc[a][1] = Vf.ndc[a][1];
+    }
+    An[0] = ndc[D.s11][0] - ndc[D.s10][0];
+    An[1] = ndc[D.s11][1] - ndc[D.s10][1];
+    Bn[0] = ndc[D.s21][0] - ndc[D.s20][0];
+    Bn[1] = ndc[D.s21][1] - ndc[D.s20][1];
+    Cn[0] = ndc[D.s20][0] - ndc[D.s10][0];
+    Cn[1] = ndc[D.s20][1] - ndc[D.s10][1];
+    const float den = An[0] * Bn[1] - An[1] * Bn[0];
+    float alpha = (Bn[1] * Cn[0] - Bn[0] * Cn[1]) / den;
+    const float beta = (An[1] * Cn[0] - An[0] * Cn[1]) / den;
+#pragma unroll
+    for (int ch = 0; ch < 4; ch++) {
+        const float gs = gv[sv][2 + ch];
+        if (D.cls2) {  // col4 = 0.5 (C_a0 + alpha (C_a1 - C_a0) + C_b0 + beta (C_b1 - C_b0))
+            gc[D.s10][ch] += 0.5f * (1.0f - alpha) * gs;
+            gc[D.s11][ch] += 0.5f * alpha * gs;
+            gc[D.s20][ch] += 0.5f * (1.0f - beta) * gs;
+            gc[D.s21][ch] += 0.5f * beta * gs;
+        } else {  // col[a1] = 0.5 ((C_b0 + beta (C_b1 - C_b0) + (alpha' - 1) C_a0) / alpha' + C_a1), flipped
+            const int a0 = D.flip ? D.s11 : D.s10, a1 = D.flip ? D.s10 : D.s11;
+            const float al = D.flip ? 1.0f - alpha : alpha;
+            gc[a1][ch] += 0.5f * gs;
+            gc[a0][ch] += 0.5f * gs * (al - 1.0f) / al;
+            gc[D.s20][ch] += 0.5f * gs * (1.0f - beta) / al;
+            gc[D.s21][ch] += 0.5f * gs * beta / al;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) {
+            const float v = gc[a][ch];
+            if (v != 0.f) atomic_add_f32(&grad_col[4 * (int64_t)vid[a] + ch], v);
+        }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+void launch_pt_count(const BatchItem* items, int nitems, int64_t nblocks, uint32_t* pt_info, float4* pt_vert,
+                     float4* pt_col, uint32_t* counts, uint32_t* block_stats, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_pt_count, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, pt_info, pt_vert, pt_col,
+                       counts, reinterpret_cast<uint4*>(block_stats));
+}
+
+void launch_pt_emit(const BatchItem* items, int nitems, int64_t nblocks, const uint32_t* pt_info,
+                    const float4* pt_vert, const uint32_t* offsets, const uint32_t* counts, uint64_t* keys,
+                    uint32_t* vals, int64_t cap, uint32_t* overflow, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_pt_emit, dim3((unsigned)nblocks), dim3(256), 0, s, items, nitems, pt_info, pt_vert, offsets,
+                       counts, keys, vals, (uint64_t)cap, overflow);
+}
+
+void launch_pt_gather(const BatchItem* items, int nitems, const uint32_t* pt_info, const float4* pt_vert,
+                      const SortRoute* route, int64_t n, const uint64_t* n_dev, uint4* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pt_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items, nitems, pt_info,
+                       pt_vert, route, n, n_dev, out);
+}
+
+void launch_pt_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
+                       const uint32_t* pt_info, const float4* pt_vert, const float4* pt_col, const uint4* recs,
+                       const uint2* ranges, PixelState ps, hipStream_t s) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_pt_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order, pt_info, pt_vert,
+                       pt_col, recs, ranges, ps);
+}
+
+void launch_pt_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
+                       int64_t nblocks, const uint32_t* order, const uint32_t* pt_info, const float4* pt_vert,
+                       const float4* pt_col, const uint4* recs, const uint2* ranges, PixelState ps, float* tri_grad,
+                       int64_t tri_grad_floats, int use_abs, hipStream_t s) {
+    if (ntiles <= 0) return;
+    fill_u32(reinterpret_cast<uint32_t*>(tri_grad), 0u, tri_grad_floats, s);
+    hipLaunchKernelGGL(k_pt_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, order, pt_info,
+                       pt_vert, pt_col, recs, ranges, ps, tri_grad, use_abs);
+    if (nblocks > 0)
+        hipLaunchKernelGGL(k_pt_backprop, dim3((unsigned)nblocks), dim3(256), 0, s, items, targets, nitems, pt_info,
+                           tri_grad);
+}
+
+}  // namespace dtv

@@ -72,7 +72,20 @@ typedef struct dtv_render_params {
                                           adjoint by the forward's fragment count), 1 = row-major */
     int32_t segment_length;            /* DTV_SEGMENT_EUCLIDEAN (INTERSECTION renderer) or
                                           DTV_SEGMENT_LINEAR_DEPTH (PPLL renderer) */
+    int32_t renderer;                  /* DTV_RENDERER_TILE (0) or DTV_RENDERER_PROJECTION (1) */
 } dtv_render_params;
+
+/* Which renderer dtv_render / dtv_render_batch runs.  TILE: per-pixel ray-tet segments of the INTERSECTION and
+ * PPLL renderers (segment_length chooses the segment length).  PROJECTION: the Projection renderer's projected
+ * tetrahedra (TetMeshRendererProjection::render, src/Renderer/TetMeshRendererProjection.cpp:811-903): each tet
+ * in view becomes 3 or 4 screen triangles with a thickness at the thick vertex
+ * (Data/Shaders/Projection/GenerateTrianglesVTK.glsl), sorted by centroid depth and blended with
+ * alpha = 1 - exp(-density * thickness) of linearly interpolated colour and thickness
+ * (ProjectedRasterization.glsl).  It has no early ray termination (use_early_ray_termination and
+ * segment_length are ignored), and its adjoint (dtv_render_adjoint, DTV_ADJOINT_EXACT only) replaces
+ * TetMeshRendererProjection::renderAdjoint (:905-971) with the exact derivative of that forward. */
+#define DTV_RENDERER_TILE 0
+#define DTV_RENDERER_PROJECTION 1
 
 /* Length of a ray's segment through a tet.  EUCLIDEAN: the distance between the entry and exit points, as the
  * INTERSECTION renderer (Data/Shaders/Intersect/IntersectRasterization.glsl:211).  LINEAR_DEPTH: the difference

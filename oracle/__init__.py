@@ -59,6 +59,12 @@ def _load():
                                         c.c_int, c.c_int, _f64p, c.c_int, _f64p, _f64p]
     lib.orc_adjoint_reference_f.argtypes = [c.c_void_p, _i32p, _f32p, _f32p, _f32p, c.c_float, _f32p, _f32p, c.c_int,
                                             _f32p, _f32p]
+    lib.orc_proj_fragments.argtypes = [_i32p, c.c_int64, _f32p, vp, vp, _f32p, c.c_float, c.c_int, c.c_int]
+    lib.orc_proj_fragments.restype = c.c_void_p
+    lib.orc_proj_forward_f.argtypes = [vp, _i32p, _f32p, vp, vp, _f32p, c.c_float, _f32p, _f32p, _f32p]
+    lib.orc_proj_forward_d.argtypes = [vp, _i32p, _f64p, vp, vp, _f64p, c.c_double, _f64p, _f64p, _f64p]
+    lib.orc_proj_tet_d.argtypes = [_f64p, _f64p, c.c_int, _f64p, _f32p, c.c_float, c.c_double, c.POINTER(c.c_int),
+                                   c.POINTER(c.c_int), _i32p, _f64p, _f64p, _f64p]
     lib.orc_accumulate_linear_const_d.argtypes = [c.c_double, _f64p, _f64p, c.c_double, _f64p]
     lib.orc_ray_tet_d.argtypes = [_f64p, _f64p, _f64p, c.POINTER(c.c_int), c.POINTER(c.c_int),
                                   c.POINTER(c.c_double), c.POINTER(c.c_double)]
@@ -193,6 +199,54 @@ def adjoint_reference(frags, positions, colors, cam, attenuation, img, dimg, use
                                 np.ascontiguousarray(dimg, np.float32).reshape(-1), int(use_abs_grad),
                                 gpos.reshape(-1), gcol.reshape(-1))
     return gpos, gcol
+
+
+class ProjFragments(Fragments):
+    """The Projection renderer's ordered per-pixel lists (fragment id = tet * 4 + triangle), built with the float
+    decisions of the mesh at `positions`: projected-tetrahedra triangles, centroid keys, sort, raster.  Kept
+    fixed, they also freeze every discrete choice for finite differences of proj_forward."""
+
+    def __init__(self, cells, positions, colors, cam_f, znear, width, height, cell_colors=None):
+        self._lib = _load()
+        self.cells = np.ascontiguousarray(cells, np.int32).reshape(-1)
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        vcol, ccol, keep = _colour_args(colors, cell_colors, np.float32)
+        self.width, self.height = width, height
+        self.handle = self._lib.orc_proj_fragments(self.cells, self.cells.size // 4, pos, vcol, ccol,
+                                                   np.ascontiguousarray(cam_f, np.float32), znear, width, height)
+        del keep
+
+
+def proj_forward(frags, positions, colors, cam, attenuation, clear=(0, 0, 0, 0), dtype=np.float32,
+                 cell_colors=None):
+    """Projection renderer image [H,W,4] and composite before the clear blend, over the lists of `frags`
+    (ProjFragments), values at `positions` / colours in precision dtype (cam: uniforms of that precision)."""
+    lib = _load()
+    H, W = frags.height, frags.width
+    img = np.zeros((H, W, 4), dtype)
+    C = np.zeros((H, W, 4), dtype)
+    vcol, ccol, keep = _colour_args(colors, cell_colors, dtype)
+    fn = lib.orc_proj_forward_f if dtype == np.float32 else lib.orc_proj_forward_d
+    fn(frags.handle, frags.cells, np.ascontiguousarray(positions, dtype).reshape(-1), vcol, ccol,
+       np.ascontiguousarray(cam, dtype), attenuation, np.ascontiguousarray(clear, dtype), img.reshape(-1),
+       C.reshape(-1))
+    del keep
+    return img, C
+
+
+def proj_tet(P, C, cam_d, cam_f, znear, attenuation, per_cell=False):
+    """One tet's projected-tetrahedra vertices in double (float decisions): dict with ntri, cls2, corners [4,3]
+    (-1 beyond ntri), ndc [5,3], col [5,4], dep [5]."""
+    c = ctypes
+    ntri, cls2 = c.c_int(), c.c_int()
+    corners = np.zeros(12, np.int32)
+    ndc, col, dep = np.zeros(15), np.zeros(20), np.zeros(5)
+    _load().orc_proj_tet_d(np.ascontiguousarray(P, np.float64).reshape(-1),
+                           np.ascontiguousarray(C, np.float64).reshape(-1), int(per_cell),
+                           np.ascontiguousarray(cam_d, np.float64), np.ascontiguousarray(cam_f, np.float32), znear,
+                           attenuation, c.byref(ntri), c.byref(cls2), corners, ndc, col, dep)
+    return {"ntri": ntri.value, "cls2": bool(cls2.value), "corners": corners.reshape(4, 3),
+            "ndc": ndc.reshape(5, 3), "col": col.reshape(5, 4), "dep": dep}
 
 
 def accumulate_linear_const(t, c0, c1, a):

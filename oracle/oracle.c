@@ -41,6 +41,14 @@
 /* Face f uses vertices ORC_FT[f][0..2]; ORC_FT[f][3] is the opposite vertex.  Common/TetFaceTable.glsl:29-34. */
 static const int ORC_FT[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {0, 2, 3, 1}, {2, 1, 3, 0}};
 
+/* Discrete choices of the Projection renderer for one tet (pt_decide). */
+typedef struct {
+    int ntri;              /* triangles: 0 (culled or degenerate), 3 (class 1) or 4 (class 2) */
+    int seg1[2], seg2[2];  /* the two edges whose projections are intersected, as first chosen */
+    int cls2;              /* the projections cross inside edge 1: class 2, vertex 4 is the crossing */
+    int flip;              /* class 1 with the crossing before edge 1's first vertex: edge 1 is reversed */
+} pt_dec;
+
 typedef struct orc_frags {
     int W, H;
     int64_t* off;   /* [W*H + 1] CSR offsets of the per-pixel fragment lists */
@@ -48,6 +56,8 @@ typedef struct orc_frags {
     int64_t ntris;  /* visible (generated) triangles */
     uint32_t* tri_key;
     int32_t* tri_id;
+    pt_dec* pt_dec;   /* Projection renderer lists only: per-tet decisions (ids are then tet*4 + triangle) */
+    int64_t T;
 } orc_frags;
 
 /* ------------------------------------------------------------------------------------------------------ */
@@ -290,7 +300,7 @@ void orc_frags_export_tris(const orc_frags* fr, uint32_t* keys, int32_t* ids) {
 }
 void orc_frags_free(orc_frags* fr) {
     if (!fr) return;
-    free(fr->off); free(fr->ids); free(fr->tri_key); free(fr->tri_id); free(fr);
+    free(fr->off); free(fr->ids); free(fr->tri_key); free(fr->tri_id); free(fr->pt_dec); free(fr);
 }
 
 /* ------------------------------------------------------------------------------------------------------ */
@@ -303,6 +313,7 @@ void orc_frags_free(orc_frags* fr) {
 #define RABS_ fabsf
 #define IS_DOUBLE_ 0
 #include "oracle_cont.h"
+#include "oracle_proj.h"
 #undef IS_DOUBLE_
 #undef R
 #undef SFX
@@ -317,6 +328,7 @@ void orc_frags_free(orc_frags* fr) {
 #define RABS_ fabs
 #define IS_DOUBLE_ 1
 #include "oracle_cont.h"
+#include "oracle_proj.h"
 #undef IS_DOUBLE_
 #undef R
 #undef SFX
@@ -419,4 +431,280 @@ void orc_tet_regularizer_d(const int32_t* cells, int64_t T, const double* pos, d
 }
 double orc_tet_regularizer_loss_d(const int32_t* cells, int64_t T, const double* pos, double lambda, double beta) {
     return tet_regularizer_loss_d(cells, T, pos, lambda, beta);
+}
+
+/* ------------------------------------------------------------------------------------------------------ */
+/* Projection renderer (projected tetrahedra): GenerateTrianglesVTK.glsl:129-403, ComputeTriangleDepths.glsl
+ * :99-108, ProjectedRasterization.glsl.  Each tet in view becomes 3 or 4 triangles whose corners carry NDC
+ * xyz, a colour and a thickness (0 except at the thick vertex); the triangles are sorted by the NDC z of their
+ * centroid and blended back to front (premultiplied), i.e. composited front to back in ascending order; each
+ * fragment contributes (c.rgb alpha, alpha), alpha = 1 - exp(-c.a d), with c and d interpolated linearly in
+ * screen space.  Differences from the reference, stated: a tet with a vertex behind the near plane or beyond
+ * the guard band is culled (the reference's NDC tests leave such a tet in and its triangles are then
+ * undefined); equal keys are ordered by triangle id = tet * 4 + k (the reference by its atomic append order);
+ * coverage is the tile renderer's fixed-point pixel-centre rule, for either winding (the pipeline culls none). */
+
+/* Corner j of triangle k of a tet with decisions D: its projected-tetrahedra vertex (0-3 tet vertex, 4 the
+ * crossing).  The triangle fans of GenerateTrianglesVTK.glsl:334-400. */
+static int pt_corner(const pt_dec* D, int k, int j) {
+    int a0 = D->seg1[0], a1 = D->seg1[1];
+    if (D->flip) { int t = a0; a0 = a1; a1 = t; }
+    const int b0 = D->seg2[0], b1 = D->seg2[1];
+    if (D->cls2) {
+        const int idx[6] = {4, a0, b0, a1, b1, a0};
+        return j == 0 ? idx[0] : idx[k + j];
+    }
+    const int idx[5] = {a1, a0, b0, b1, a0};
+    return j == 0 ? idx[0] : idx[k + j];
+}
+
+/* Float decisions for tet `tet` (GenerateTrianglesVTK.glsl:140-273), arithmetic as the GPU's. */
+static void pt_decide(const int32_t* cells, const float* pos, int64_t tet, const float* u, float znear, pt_dec* D) {
+    const float* VP = u;
+    float ndc[4][3];
+    int cull_w = 0;
+    for (int i = 0; i < 4; i++) {
+        int32_t g = cells[tet * 4 + i];
+        float p[3] = {pos[3 * (int64_t)g], pos[3 * (int64_t)g + 1], pos[3 * (int64_t)g + 2]};
+        float c[4];
+        for (int r = 0; r < 4; r++)
+            c[r] = ((VP[0 * 4 + r] * p[0] + VP[1 * 4 + r] * p[1]) + VP[2 * 4 + r] * p[2]) + VP[3 * 4 + r] * 1.0f;
+        if (!(c[3] >= znear)) cull_w = 1;
+        for (int k = 0; k < 3; k++) ndc[i][k] = c[k] / c[3];
+        if (!(fabsf(ndc[i][0]) <= ORC_GUARD) || !(fabsf(ndc[i][1]) <= ORC_GUARD)) cull_w = 1;
+    }
+    D->ntri = 0;
+    D->cls2 = D->flip = 0;
+    if (cull_w) return;
+    int all_gx = 1, all_lx = 1, all_gy = 1, all_ly = 1, all_gz = 1, any_lz = 0;
+    for (int i = 0; i < 4; i++) {
+        all_gx &= ndc[i][0] > 1.0f;
+        all_lx &= ndc[i][0] < -1.0f;
+        all_gy &= ndc[i][1] > 1.0f;
+        all_ly &= ndc[i][1] < -1.0f;
+        all_gz &= ndc[i][2] > 1.0f;
+        any_lz |= ndc[i][2] < 0.0f;
+    }
+    if (all_gx || all_lx || all_gy || all_ly || all_gz || any_lz) return;
+    float v1x = ndc[1][0] - ndc[0][0], v1y = ndc[1][1] - ndc[0][1];
+    float v2x = ndc[2][0] - ndc[0][0], v2y = ndc[2][1] - ndc[0][1];
+    float v3x = ndc[3][0] - ndc[0][0], v3y = ndc[3][1] - ndc[0][1];
+    float fd1 = v3x * v2y - v3y * v2x;
+    float fd2 = v1x * v3y - v1y * v3x;
+    float fd3 = v2x * v1y - v2y * v1x;
+    if ((fd1 * fd2 >= 0.0f) && ((fd1 != 0.0f) || (fd2 != 0.0f))) {
+        D->seg1[0] = 0; D->seg1[1] = 3; D->seg2[0] = 1; D->seg2[1] = 2;
+    } else if (fd1 * fd3 >= 0.0f) {
+        D->seg1[0] = 0; D->seg1[1] = 2; D->seg2[0] = 1; D->seg2[1] = 3;
+    } else {
+        D->seg1[0] = 0; D->seg1[1] = 1; D->seg2[0] = 2; D->seg2[1] = 3;
+    }
+    const float* P1 = ndc[D->seg1[0]];
+    const float* P2 = ndc[D->seg1[1]];
+    const float* P3 = ndc[D->seg2[0]];
+    const float* P4 = ndc[D->seg2[1]];
+    float Ax = P2[0] - P1[0], Ay = P2[1] - P1[1];
+    float Bx = P4[0] - P3[0], By = P4[1] - P3[1];
+    float Cx = P3[0] - P1[0], Cy = P3[1] - P1[1];
+    float den = Ax * By - Ay * Bx;
+    if (fabsf(den) < 1e-8f) return;  /* degenerate */
+    float alpha = (By * Cx - Bx * Cy) / den;
+    D->cls2 = (alpha >= 0.0f) && (alpha <= 1.0f);
+    D->flip = !D->cls2 && alpha <= 0.0f;
+    D->ntri = D->cls2 ? 4 : 3;
+}
+
+/* Triangles of all tets in float (decisions, keys, fixed-point screen triangles): out[4 t + k] for k < nt[t]. */
+static void pt_triangles(const int32_t* cells, int64_t T, const float* pos, const float* col, const float* cell_col,
+                         const float* u, float znear, int W, int H, pt_dec* dec, orc_tri* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < T; t++) {
+        pt_dec* D = &dec[t];
+        pt_decide(cells, pos, t, u, znear, D);
+        if (!D->ntri) continue;
+        float P[4][3], C[4][4], ndc[5][3], cc[5][4], dep[5];
+        for (int i = 0; i < 4; i++) {
+            int32_t g = cells[t * 4 + i];
+            for (int k = 0; k < 3; k++) P[i][k] = pos[3 * (int64_t)g + k];
+            for (int k = 0; k < 4; k++) C[i][k] = cell_col ? cell_col[4 * t + k] : col[4 * (int64_t)g + k];
+        }
+        pt_vertices_f(D, u, P, C, cell_col != NULL, 1.0f, ndc, cc, dep);
+        const float hw = 0.5f * (float)W, hh = 0.5f * (float)H;
+        for (int k = 0; k < D->ntri; k++) {
+            orc_tri* tr = &out[4 * t + k];
+            tr->id = (int32_t)(t * 4 + k);
+            int cv[3];
+            for (int j = 0; j < 3; j++) cv[j] = pt_corner(D, k, j);
+            float z = ((ndc[cv[0]][2] + ndc[cv[1]][2]) + ndc[cv[2]][2]) / 3.0f;
+            tr->key = fbits(z);
+            for (int j = 0; j < 3; j++) {
+                float xf = ndc[cv[j]][0] * hw + hw;
+                float yf = ndc[cv[j]][1] * hh + hh;
+                tr->X[j] = (int32_t)rintf(xf * 256.0f);
+                tr->Y[j] = (int32_t)rintf(yf * 256.0f);
+            }
+            int64_t area = (int64_t)(tr->X[1] - tr->X[0]) * (tr->Y[2] - tr->Y[0]) -
+                           (int64_t)(tr->Y[1] - tr->Y[0]) * (tr->X[2] - tr->X[0]);
+            if (area < 0) {
+                int32_t tx = tr->X[1], ty = tr->Y[1];
+                tr->X[1] = tr->X[2]; tr->Y[1] = tr->Y[2]; tr->X[2] = tx; tr->Y[2] = ty;
+            }
+            int32_t mnx = tr->X[0], mxx = tr->X[0], mny = tr->Y[0], mxy = tr->Y[0];
+            for (int j = 1; j < 3; j++) {
+                if (tr->X[j] < mnx) mnx = tr->X[j];
+                if (tr->X[j] > mxx) mxx = tr->X[j];
+                if (tr->Y[j] < mny) mny = tr->Y[j];
+                if (tr->Y[j] > mxy) mxy = tr->Y[j];
+            }
+            int64_t x0 = -floordiv256(-(int64_t)mnx + 128), x1 = floordiv256((int64_t)mxx - 128);
+            int64_t y0 = -floordiv256(-(int64_t)mny + 128), y1 = floordiv256((int64_t)mxy - 128);
+            if (x0 < 0) x0 = 0;
+            if (y0 < 0) y0 = 0;
+            if (x1 > W - 1) x1 = W - 1;
+            if (y1 > H - 1) y1 = H - 1;
+            if (area == 0) { x0 = 1; x1 = 0; }
+            tr->x0 = (int32_t)x0; tr->x1 = (int32_t)x1; tr->y0 = (int32_t)y0; tr->y1 = (int32_t)y1;
+        }
+    }
+}
+
+/* Ordered per-pixel lists of the sorted triangles `all[0..n)` covering each pixel (CSR into fr). */
+static void raster_lists(const orc_tri* all, int64_t n, int W, int H, orc_frags* fr) {
+    const int64_t P = (int64_t)W * H;
+    int64_t* cnt = (int64_t*)calloc((size_t)P + 1, sizeof(int64_t));
+    const int nb = 64;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nb; b++) {
+        int r0 = (int)((int64_t)H * b / nb), r1 = (int)((int64_t)H * (b + 1) / nb);
+        for (int64_t i = 0; i < n; i++) {
+            const orc_tri* t = &all[i];
+            int ya = t->y0 > r0 ? t->y0 : r0, yb = t->y1 < r1 - 1 ? t->y1 : r1 - 1;
+            for (int y = ya; y <= yb; y++)
+                for (int x = t->x0; x <= t->x1; x++)
+                    if (covers(t, x, y)) cnt[(int64_t)y * W + x + 1]++;
+        }
+    }
+    for (int64_t i = 0; i < P; i++) cnt[i + 1] += cnt[i];
+    fr->off = cnt;
+    fr->ids = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cnt[P] > 0 ? cnt[P] : 1));
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)P);
+    memcpy(fill, cnt, sizeof(int64_t) * (size_t)P);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nb; b++) {
+        int r0 = (int)((int64_t)H * b / nb), r1 = (int)((int64_t)H * (b + 1) / nb);
+        for (int64_t i = 0; i < n; i++) {
+            const orc_tri* t = &all[i];
+            int ya = t->y0 > r0 ? t->y0 : r0, yb = t->y1 < r1 - 1 ? t->y1 : r1 - 1;
+            for (int y = ya; y <= yb; y++)
+                for (int x = t->x0; x <= t->x1; x++)
+                    if (covers(t, x, y)) fr->ids[fill[(int64_t)y * W + x]++] = t->id;
+        }
+    }
+    free(fill);
+}
+
+/* Projected-tetrahedra fragment lists: the decisions and triangles of the mesh at float positions `pos`
+ * (triangle ids tet * 4 + k), sorted by (key, id), rasterised into per-pixel lists front to back. */
+orc_frags* orc_proj_fragments(const int32_t* cells, int64_t T, const float* pos, const float* col, const float* cell_col,
+                              const float* camu, float znear, int W, int H) {
+    orc_frags* fr = (orc_frags*)calloc(1, sizeof(orc_frags));
+    fr->W = W; fr->H = H;
+    pt_dec* dec = (pt_dec*)malloc(sizeof(pt_dec) * (size_t)(T > 0 ? T : 1));
+    orc_tri* all = (orc_tri*)malloc(sizeof(orc_tri) * 4 * (size_t)(T > 0 ? T : 1));
+    pt_triangles(cells, T, pos, col, cell_col, camu, znear, W, H, dec, all);
+    int64_t n = 0;
+    for (int64_t t = 0; t < T; t++)
+        for (int k = 0; k < dec[t].ntri; k++) all[n++] = all[4 * t + k];
+    qsort(all, (size_t)n, sizeof(orc_tri), tri_cmp);
+    fr->ntris = n;
+    fr->tri_key = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    fr->tri_id = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) { fr->tri_key[i] = all[i].key; fr->tri_id[i] = all[i].id; }
+    raster_lists(all, n, W, H, fr);
+    fr->pt_dec = dec;
+    fr->T = T;
+    free(all);
+    return fr;
+}
+
+#define PT_FORWARD(SUF, R_, REXPF)                                                                                    \
+    /* Composites the lists of fr (from orc_proj_fragments) with values in precision R_ at positions pos. */     \
+    void orc_proj_forward##SUF(const orc_frags* fr, const int32_t* cells, const R_* pos, const R_* col,            \
+                               const R_* cell_col, const R_* camu, R_ att, const R_* clear, R_* img, R_* outC) {   \
+        const int W = fr->W, H = fr->H;                                                                           \
+        const int64_t T = fr->T;                                                                                  \
+        R_(*vn)[5][3] = (R_(*)[5][3])malloc(sizeof(R_) * 15 * (size_t)(T > 0 ? T : 1));                           \
+        R_(*vc)[5][4] = (R_(*)[5][4])malloc(sizeof(R_) * 20 * (size_t)(T > 0 ? T : 1));                           \
+        R_(*vd)[5] = (R_(*)[5])malloc(sizeof(R_) * 5 * (size_t)(T > 0 ? T : 1));                                  \
+        _Pragma("omp parallel for schedule(static)") for (int64_t t = 0; t < T; t++) {                           \
+            if (!fr->pt_dec[t].ntri) continue;                                                                    \
+            R_ P[4][3], C[4][4];                                                                                  \
+            for (int i = 0; i < 4; i++) {                                                                         \
+                int32_t g = cells[t * 4 + i];                                                                     \
+                for (int k = 0; k < 3; k++) P[i][k] = pos[3 * (int64_t)g + k];                                    \
+                for (int k = 0; k < 4; k++) C[i][k] = cell_col ? cell_col[4 * t + k] : col[4 * (int64_t)g + k];   \
+            }                                                                                                     \
+            pt_vertices##SUF(&fr->pt_dec[t], camu, (const R_(*)[3])P, (const R_(*)[4])C, cell_col != NULL, att,  \
+                             vn[t], vc[t], vd[t]);                                                                \
+        }                                                                                                         \
+        const int64_t NP = (int64_t)W * H;                                                                        \
+        _Pragma("omp parallel for schedule(dynamic, 64)") for (int64_t px = 0; px < NP; px++) {                  \
+            int x = (int)(px % W), y = (int)(px / W);                                                             \
+            R_ pxn = (R_)2.0 * (((R_)x + (R_)0.5) / (R_)W) - (R_)1.0;                                             \
+            R_ pyn = (R_)2.0 * (((R_)y + (R_)0.5) / (R_)H) - (R_)1.0;                                             \
+            R_ rc[4] = {0, 0, 0, 0};                                                                              \
+            for (int64_t q = fr->off[px]; q < fr->off[px + 1]; q++) {                                             \
+                int32_t id = fr->ids[q];                                                                          \
+                int64_t t = id >> 2;                                                                              \
+                int k = id & 3;                                                                                   \
+                R_ X[3], Y[3], cc[3][4], dd[3];                                                                   \
+                for (int j = 0; j < 3; j++) {                                                                     \
+                    int v = pt_corner(&fr->pt_dec[t], k, j);                                                      \
+                    X[j] = vn[t][v][0];                                                                           \
+                    Y[j] = vn[t][v][1];                                                                           \
+                    for (int c = 0; c < 4; c++) cc[j][c] = vc[t][v][c];                                           \
+                    dd[j] = vd[t][v];                                                                             \
+                }                                                                                                 \
+                v4##SUF f = pt_fragment##SUF(X, Y, (const R_(*)[4])cc, dd, pxn, pyn);                              \
+                rc[0] = rc[0] + ((R_)1.0 - rc[3]) * f.x;                                                          \
+                rc[1] = rc[1] + ((R_)1.0 - rc[3]) * f.y;                                                          \
+                rc[2] = rc[2] + ((R_)1.0 - rc[3]) * f.z;                                                          \
+                rc[3] = rc[3] + ((R_)1.0 - rc[3]) * f.w;                                                          \
+            }                                                                                                     \
+            R_ Tr = (R_)1.0 - rc[3];                                                                              \
+            for (int c = 0; c < 4; c++) {                                                                         \
+                if (outC) outC[4 * px + c] = rc[c];                                                               \
+                img[4 * px + c] = rc[c] + Tr * clear[c];                                                          \
+            }                                                                                                     \
+        }                                                                                                         \
+        free(vn);                                                                                                 \
+        free(vc);                                                                                                 \
+        free(vd);                                                                                                 \
+    }
+PT_FORWARD(_f, float, expf)
+PT_FORWARD(_d, double, exp)
+
+/* One tet's projected-tetrahedra vertices in double with float decisions (closed-form checks).  P: 4x3 world
+ * positions, C: 4x4 colours; out: ntri, cls2, corners[4][3] (PT vertex of each triangle corner, -1 beyond ntri),
+ * ndc[5][3], col[5][4], dep[5]. */
+void orc_proj_tet_d(const double* P, const double* C, int per_cell, const double* camu_d, const float* camu_f,
+                    float znear, double att, int* ntri, int* cls2, int* corners, double* ndc, double* col,
+                    double* dep) {
+    int32_t cells[4] = {0, 1, 2, 3};
+    float pf[12];
+    for (int i = 0; i < 12; i++) pf[i] = (float)P[i];
+    pt_dec D;
+    pt_decide(cells, pf, 0, camu_f, znear, &D);
+    *ntri = D.ntri;
+    *cls2 = D.cls2;
+    for (int k = 0; k < 4; k++)
+        for (int j = 0; j < 3; j++) corners[3 * k + j] = k < D.ntri ? pt_corner(&D, k, j) : -1;
+    if (!D.ntri) return;
+    double Pa[4][3], Ca[4][4];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 3; k++) Pa[i][k] = P[3 * i + k];
+        for (int k = 0; k < 4; k++) Ca[i][k] = C[4 * i + k];
+    }
+    pt_vertices_d(&D, camu_d, (const double(*)[3])Pa, (const double(*)[4])Ca, per_cell, att, (double(*)[3])ndc,
+                  (double(*)[4])col, dep);
 }

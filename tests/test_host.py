@@ -151,11 +151,15 @@ def test_module_fails_loudly_without_gpu():
         d.TetMesh()
 
 
-def test_projection_renderer_type_is_refused():
-    """RendererType.PROJECTION is not built: asking for it raises instead of rendering something else."""
-    import difftetvr_amd as d
-    with pytest.raises(RuntimeError, match="PROJECTION"):
-        d.Renderer(d.RendererType.PROJECTION)
+def test_render_params_layout_matches_header():
+    """dtv_render_params in include/difftetvr_amd.h and its ctypes mirror list the same fields in order (the
+    renderer selector last), and the selector values agree."""
+    from difftetvr_amd import _lib
+    txt = open(os.path.join(ROOT, "include", "difftetvr_amd.h")).read()
+    body = txt[txt.index("typedef struct dtv_render_params {"):txt.index("} dtv_render_params;")]
+    names = re.findall(r"^\s+(?:float|int32_t)\s+(\w+)(?:\[\d+\])?;", body, flags=re.M)
+    assert names == [f[0] for f in _lib.DtvRenderParams._fields_]
+    assert re.search(r"#define DTV_RENDERER_PROJECTION 1\b", txt) and _lib.DTV_RENDERER_PROJECTION == 1
 
 
 def test_unlink_tets():
