@@ -187,6 +187,54 @@ class TetMesh:
         self._boundary = None
         self._setup_grads()
 
+    def set_triangle_mesh_data(self, triangle_indices, vertex_positions, vertex_colors):
+        """Stores a triangle surface mesh (TetMesh::setTriangleMeshData, TetMesh.cpp:369-478): int32 triangle
+        indices [F,3], float32 positions [V,3] and colours [V,4], with the reference's checks and messages.  The
+        tet renderers refuse such a mesh (it has no tets); the tensors and counts are available as for tets."""
+        def _t(x):
+            return x if torch.is_tensor(x) else torch.as_tensor(np.asarray(x))
+        tri, pos, col = _t(triangle_indices), _t(vertex_positions), _t(vertex_colors)
+        name = "Error in TetMesh::setTriangleMeshData: "
+        if tri.dtype != torch.int32:
+            raise RuntimeError(name + "triangleIndicesTensor.dtype() != torch::kInt32.")
+        if pos.dtype != torch.float32:
+            raise RuntimeError(name + "vertexPositionsTensor.dtype() != torch::kFloat32.")
+        if col.dtype != torch.float32:
+            raise RuntimeError(name + "vertexColorsTensor.dtype() != torch::kFloat32.")
+        for t, what in ((tri, "triangleIndicesTensor"), (pos, "vertexPositionsTensor"), (col, "vertexColorsTensor")):
+            if t.dim() != 2:
+                raise RuntimeError(name + f"{what}.sizes().size() != 2.")
+        if pos.shape[0] != col.shape[0]:
+            raise RuntimeError(name + "vertexPositionsTensor.size(0) != vertexColorsTensor.size(0).")
+        if tri.shape[1] != 3:
+            raise RuntimeError(name + "triangleIndicesTensor.size(1) != 3.")
+        if pos.shape[1] != 3:
+            raise RuntimeError(name + "vertexPositionsTensor.size(1) != 3.")
+        if col.shape[1] != 4:
+            raise RuntimeError(name + "vertexColorsTensor.size(1) != 3.")
+        dev = _device()
+        self._cells = tri.to(device=dev).contiguous()
+        self._positions = pos.to(device=dev).detach().clone().contiguous()
+        self._colors = col.to(device=dev).detach().clone().contiguous()
+        self._color_storage = ColorStorage.PER_VERTEX
+        self._boundary = None
+        self._setup_grads()
+
+    def get_has_triangle_mesh_data(self):
+        """True after set_triangle_mesh_data (TetMesh::getHasTriangleMeshData)."""
+        return self._cells is not None and self._cells.dim() == 2 and self._cells.shape[1] == 3
+
+    def _face_boundary_bits(self):
+        """bool [T,4]: face f of tet t lies on the mesh boundary (no other tet shares it); cached per mesh."""
+        key = (self._cells.data_ptr(), self._cells.shape[0])
+        if getattr(self, "_fb_key", None) != key:
+            c = self._cells.cpu().numpy().astype(np.int64)
+            faces = np.sort(c[:, meshgen.TET_FACE_TABLE[:, :3]], axis=2).reshape(-1, 3)
+            _, inv, cnt = np.unique(faces, axis=0, return_inverse=True, return_counts=True)
+            self._fb = (cnt[inv.reshape(-1)] == 1).reshape(-1, 4)
+            self._fb_key = key
+        return self._fb
+
     def _set_like(self, cells, pos, col):
         """set_tet_mesh_data with this mesh's colour storage."""
         if self._color_storage == ColorStorage.PER_CELL:
@@ -355,6 +403,8 @@ class TetMesh:
 
     def _desc(self):
         self._require()
+        if self.get_has_triangle_mesh_data():
+            raise RuntimeError("The mesh holds triangle data (set_triangle_mesh_data): there are no tets to render.")
         per_cell = self._color_storage == ColorStorage.PER_CELL
         return _lib.DtvMesh(self._cells.data_ptr(), self._positions.data_ptr(),
                             None if per_cell else self._colors.data_ptr(), self._cells.shape[0],
@@ -564,6 +614,72 @@ class Renderer:
                                                        ctypes.c_void_p(out.data_ptr()), _stream_ptr()),
                    "get_fragment_ids")
         return out
+
+    # -- PPLL linked-list export (Module.cpp:960-981, TetMeshRendererPPLL.cpp:809-860) -------------------------
+    def set_export_linked_list_data(self):
+        if self._renderer_type != RendererType.PPLL:
+            raise RuntimeError("Error in set_export_linked_list_data: The renderer must be of type RendererType.PPLL.")
+        self._export_ll = True
+
+    def _linked_list_checked(self, fn, what):
+        if self._renderer_type != RendererType.PPLL:
+            raise RuntimeError(f"Error in {fn}: The renderer must be of type RendererType.PPLL.")
+        if not getattr(self, "_export_ll", False):
+            raise RuntimeError(f"Error in {what}: exportLinkedListData is set to false.")
+        if self._rendered_mesh is None:
+            raise RuntimeError(f"Error in {fn}: render() must be called first.")
+        return self._linked_list()
+
+    def get_fragment_buffer(self):
+        """float32 [N, 3] of 12-byte nodes {faceBits, depth, next} (LinkedListHeader.glsl:35-42; the first and
+        third words are uint32 bit patterns): one node per composited fragment, faceBits = (fragment id tet*4 +
+        face) << 2 | boundary << 1 | 1 (the entered face, front-facing), depth = the face's window depth at the
+        pixel centre, next = the pixel's next fragment front to back (0xFFFFFFFF at the end)."""
+        return self._linked_list_checked("get_fragment_buffer", "getFragmentBufferTensor")[0]
+
+    def get_start_offset_buffer(self):
+        """int32 [H, W]: index of each pixel's first (nearest) node, -1 where none."""
+        return self._linked_list_checked("get_start_offset_buffer", "getStartOffsetBufferTensor")[1]
+
+    def _linked_list(self):
+        mesh = self._rendered_mesh
+        counts = self.get_fragment_counts()
+        m = max(int(counts.max()), 1)
+        ids = self.get_fragment_ids(m)
+        valid = ids >= 0
+        fid = ids[valid].to(torch.int64)  # pixel-major, front to back within a pixel
+        n = int(fid.numel())
+        tet, face = fid >> 2, fid & 3
+        ft = torch.as_tensor(meshgen.TET_FACE_TABLE[:, :3], device=fid.device)
+        cells = mesh.get_cell_indices().to(torch.int64)
+        vid = cells[tet.unsqueeze(1), ft[face]]  # [n, 3]
+        u = torch.as_tensor(self.camera_uniforms(), device=fid.device)
+        vp = u[:16].view(4, 4).t()
+        p = mesh.get_vertex_positions().detach()[vid]  # [n, 3, 3]
+        clip = p @ vp[:3, :3].t() + vp[:3, 3]
+        w = p @ vp[3, :3] + vp[3, 3]
+        ndc = clip / w.unsqueeze(-1)
+        pix = valid.nonzero()[:, :2].to(torch.float32)  # (y, x) of each node
+        px = 2.0 * ((pix[:, 1] + 0.5) / self._width) - 1.0
+        py = 2.0 * ((pix[:, 0] + 0.5) / self._height) - 1.0
+        X, Y, Z = ndc[..., 0], ndc[..., 1], ndc[..., 2]
+        den = (X[:, 1] - X[:, 0]) * (Y[:, 2] - Y[:, 0]) - (X[:, 2] - X[:, 0]) * (Y[:, 1] - Y[:, 0])
+        l0 = ((X[:, 1] - px) * (Y[:, 2] - py) - (X[:, 2] - px) * (Y[:, 1] - py)) / den
+        l1 = ((X[:, 2] - px) * (Y[:, 0] - py) - (X[:, 0] - px) * (Y[:, 2] - py)) / den
+        depth = l0 * Z[:, 0] + l1 * Z[:, 1] + (1.0 - l0 - l1) * Z[:, 2]
+        boundary = torch.as_tensor(mesh._face_boundary_bits(), device=fid.device)[tet, face].to(torch.int64)
+        bits = ((fid << 2) | (boundary << 1) | 1).to(torch.int32)
+        start = torch.full((self._height, self._width), -1, dtype=torch.int32, device=fid.device)
+        first = torch.cumsum(counts.reshape(-1).to(torch.int64), 0) - counts.reshape(-1).to(torch.int64)
+        has = counts.reshape(-1) > 0
+        start.view(-1)[has] = first[has].to(torch.int32)
+        nxt = torch.arange(1, n + 1, dtype=torch.int64, device=fid.device)
+        last = (first + counts.reshape(-1).to(torch.int64) - 1)[has]
+        nxt[last] = 0xFFFFFFFF
+        nxt = nxt.to(torch.int64).bitwise_and(0xFFFFFFFF)
+        nxt32 = torch.where(nxt > 0x7FFFFFFF, nxt - (1 << 32), nxt).to(torch.int32)
+        buf = torch.stack([bits.view(torch.float32), depth.to(torch.float32), nxt32.view(torch.float32)], dim=1)
+        return buf, start
 
     def set_timing(self, enabled=True):
         """Enables per-stage hipEvent timers on this renderer's view."""

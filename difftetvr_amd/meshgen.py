@@ -43,7 +43,9 @@ def hex_grid(aabb_min, aabb_max, xs, ys, zs):
     return pos, cells
 
 
-_TET_FACES = np.array([[0, 1, 2], [1, 0, 3], [0, 2, 3], [2, 1, 3]], np.int64)
+# Face f of a tet uses its vertices TET_FACE_TABLE[f] (Common/TetFaceTable.glsl:29-34, TetMesh.cpp:1175-1181).
+TET_FACE_TABLE = np.array([[0, 1, 2], [1, 0, 3], [0, 2, 3], [2, 1, 3]], np.int64)
+_TET_FACES = TET_FACE_TABLE
 
 
 def boundary_vertex_mask(cells, num_vertices):
