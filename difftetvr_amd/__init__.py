@@ -27,7 +27,8 @@ from . import meshgen
 from ._lib import DtvError
 
 __all__ = ["vec3", "vec4", "uvec3", "AABB3", "ColorStorage", "TestCase", "RendererType", "SplitGradientType",
-           "TetMesh", "Renderer", "BatchRenderer", "TetRegularizer", "set_device_type", "DtvError", "meshgen"]
+           "TetMesh", "Renderer", "BatchRenderer", "TetRegularizer", "set_device_type", "DtvError", "meshgen",
+           "autotune_cache"]
 
 
 class vec3:
@@ -411,6 +412,14 @@ class TetMesh:
                             self._positions.shape[0], self._colors.data_ptr() if per_cell else None)
 
 
+_TUNE_CACHE = {}
+
+
+def autotune_cache():
+    """The autotuner's choices so far: {key: {tile_size, segments_per_pass, ms, ms_default, candidates}}."""
+    return dict(_TUNE_CACHE)
+
+
 class Renderer:
     """Tet-mesh volume renderer (TetMeshVolumeRenderer, src/Renderer/TetMeshVolumeRenderer.hpp:84-280)."""
 
@@ -504,6 +513,51 @@ class Renderer:
     def set_checkpoint_interval(self, k):
         self._checkpoint_interval = int(k)
 
+    def set_tile_size(self, tile_size):
+        """Binning tile edge in pixels: 8 (default), 16 or 32 (rendered as 8x8 sub-tiles; images do not depend
+        on it).  No reference counterpart."""
+        if int(tile_size) not in (8, 16, 32):
+            raise RuntimeError("set_tile_size: 8, 16 or 32 expected.")
+        self._tile_size = int(tile_size)
+
+    def set_segments_per_pass(self, segments):
+        """Ray segments the forward shades per round: 64, 128 (default) or 256.  No reference counterpart."""
+        if int(segments) not in (64, 128, 256):
+            raise RuntimeError("set_segments_per_pass: 64, 128 or 256 expected.")
+        self._segments_per_pass = int(segments)
+
+    def set_autotune(self, enabled=True, with_adjoint=True, repeats=3):
+        """With autotuning on, the first render of a (mesh size class, viewport) times every tile size and
+        segments-per-pass setting (``dtv_autotune``: forward, and adjoint with with_adjoint) and the fastest is
+        used from then on; choices are cached per process in ``autotune_cache()``.  Tile renderer only."""
+        self._autotune = (bool(enabled), bool(with_adjoint), int(repeats))
+
+    def get_tuning(self):
+        """The (tile_size, segments_per_pass) the last render used."""
+        return (self._tile_size, self._segments_per_pass)
+
+    def _tune_key(self, items):
+        def cls(m):
+            return (int(m.get_num_cells()).bit_length(), int(m.get_color_storage()))
+        return (int(self._renderer_type), tuple((cls(m), w, h) for m, w, h in items))
+
+    def _maybe_tune(self, arr, n, items):
+        on, with_adj, repeats = getattr(self, "_autotune", (False, True, 3))
+        if not on or self._renderer_type == RendererType.PROJECTION:
+            return
+        key = self._tune_key(items)
+        if key not in _TUNE_CACHE:
+            res = _lib.DtvTuneResult()
+            prm = self._params()
+            _lib.check(self._lib.dtv_autotune(self._view, arr, n, ctypes.byref(prm), int(with_adj), repeats,
+                                              ctypes.byref(res), _stream_ptr()), "dtv_autotune")
+            _TUNE_CACHE[key] = {"tile_size": res.tile_size, "segments_per_pass": res.segments_per_pass,
+                                "ms": res.ms, "ms_default": res.ms_default,
+                                "candidates": [(res.cand_tile_size[i], res.cand_segments_per_pass[i], res.cand_ms[i])
+                                               for i in range(res.num_candidates)]}
+        choice = _TUNE_CACHE[key]
+        self._tile_size, self._segments_per_pass = choice["tile_size"], choice["segments_per_pass"]
+
     def set_tile_order(self, order):
         """Launch order of the screen tiles: 'cost' (default: heaviest first) or 'row-major'.  Images and
         gradients do not depend on it (up to the order of the gradients' float atomics)."""
@@ -541,7 +595,6 @@ class Renderer:
         # ERT is compiled in only for meshes with gradients (TetMeshVolumeRenderer.cpp:417-419).
         p.use_early_ray_termination = int(self._use_ert and self._mesh is not None and self._mesh.get_use_gradients())
         p.early_ray_out_alpha = 1.0 - self._ert_thresh
-        p.tile_size = 8
         p.checkpoint_interval = self._checkpoint_interval
         p.tile_order = self._tile_order
         # PPLL composites between consecutive faces by their linear depths (PPLL/Forward.glsl:53,188)
@@ -549,6 +602,9 @@ class Renderer:
                             else _lib.DTV_SEGMENT_EUCLIDEAN)
         p.renderer = (_lib.DTV_RENDERER_PROJECTION if self._renderer_type == RendererType.PROJECTION
                       else _lib.DTV_RENDERER_TILE)
+        proj = self._renderer_type == RendererType.PROJECTION
+        p.tile_size = 8 if proj else getattr(self, "_tile_size", 8)
+        p.segments_per_pass = getattr(self, "_segments_per_pass", 128)
         return p
 
     def camera_uniforms(self):
@@ -563,7 +619,12 @@ class Renderer:
             raise RuntimeError("Missing valid tet mesh in Renderer.render.")
         img = torch.empty((self._height, self._width, 4), dtype=torch.float32, device=_device())
         desc = self._mesh._desc()
-        cam, prm = self._camera(), self._params()
+        cam = self._camera()
+        if getattr(self, "_autotune", (False,))[0]:
+            item = (_lib.DtvRenderItem * 1)()
+            item[0].mesh, item[0].camera, item[0].out_image = desc, cam, img.data_ptr()
+            self._maybe_tune(item, 1, [(self._mesh, self._width, self._height)])
+        prm = self._params()
         _lib.check(self._lib.dtv_render(self._view, ctypes.byref(desc), ctypes.byref(cam), ctypes.byref(prm),
                                         ctypes.c_void_p(img.data_ptr()), _stream_ptr()), "Renderer.render")
         self._rendered_mesh = self._mesh
@@ -764,6 +825,8 @@ class BatchRenderer(Renderer):
             arr[i].out_image = img.data_ptr()
             images.append(img)
             off += h * w
+        if getattr(self, "_autotune", (False,))[0]:
+            self._maybe_tune(arr, len(self._items), [(m, w, h) for m, _, w, h in self._items])
         prm = self._params()
         _lib.check(self._lib.dtv_render_batch(self._view, arr, len(self._items), ctypes.byref(prm), _stream_ptr()),
                    "BatchRenderer.render")

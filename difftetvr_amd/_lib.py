@@ -37,7 +37,8 @@ class DtvCamera(ctypes.Structure):
 class DtvRenderParams(ctypes.Structure):
     _fields_ = [("attenuation", c_float), ("clear_color", c_float * 4), ("use_early_ray_termination", c_int32),
                 ("early_ray_out_alpha", c_float), ("tile_size", c_int32), ("checkpoint_interval", c_int32),
-                ("tile_order", c_int32), ("segment_length", c_int32), ("renderer", c_int32)]
+                ("tile_order", c_int32), ("segment_length", c_int32), ("renderer", c_int32),
+                ("segments_per_pass", c_int32)]
 
 
 class DtvViewStats(ctypes.Structure):
@@ -70,6 +71,12 @@ class DtvAdamGroup(ctypes.Structure):
                 ("step", c_int64), ("clamp_nonneg", c_int32), ("zero_grad", c_int32)]
 
 
+class DtvTuneResult(ctypes.Structure):
+    _fields_ = [("tile_size", c_int32), ("segments_per_pass", c_int32), ("ms", c_float), ("ms_default", c_float),
+                ("num_candidates", c_int32), ("cand_tile_size", c_int32 * 9), ("cand_segments_per_pass", c_int32 * 9),
+                ("cand_ms", c_float * 9)]
+
+
 # (name, restype, argtypes) of every exported entry point; tests check the library exports exactly these.
 SIGNATURES = [
     ("dtv_get_version", ctypes.c_int, []),
@@ -90,6 +97,8 @@ SIGNATURES = [
                                         ctypes.POINTER(DtvRenderParams), c_void_p]),
     ("dtv_render_adjoint_batch", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvAdjointItem), c_int32, c_int32, c_int32,
                                                 c_void_p]),
+    ("dtv_autotune", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvRenderItem), c_int32, ctypes.POINTER(DtvRenderParams),
+                                    c_int32, c_int32, ctypes.POINTER(DtvTuneResult), c_void_p]),
     ("dtv_view_fragment_counts", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("dtv_view_export_fragments", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvMesh), c_int32, c_void_p, c_void_p]),
     ("dtv_view_get_stats", ctypes.c_int, [c_void_p, ctypes.POINTER(DtvViewStats)]),

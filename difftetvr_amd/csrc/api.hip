@@ -215,11 +215,12 @@ struct dtv_view {
     DevBuf tile_frags, tile_order;  // per tile: forward's fragments; launch orders of the forward and adjoint
     DevBuf pt_info, pt_vert, pt_col, tri_grad;  // Projection renderer: per-tet triangles, per-corner gradients
     bool projection = false;        // the last render ran the Projection renderer
+    bool sub_tiles = false;         // the last render binned tiles larger than 8x8
     bool cost_ordered = true;       // the last render launched its tiles heaviest first
     int64_t planned_cap = 0;  // > 0: planned (synchronisation-free) rendering with this record capacity
     // state of the last render
     bool rendered = false;
-    int64_t total_blocks = 0, total_tiles = 0, total_pixels = 0;
+    int64_t total_blocks = 0, total_tiles = 0, total_subtiles = 0, total_pixels = 0;
     int64_t num_pairs = 0, num_tris = 0;
     int sort_passes = 0;
     // per-stage timers
@@ -272,9 +273,12 @@ bool make_uniforms(const dtv_camera& camera, const dtv_render_params& params, Vi
     U.znear = camera.znear;
     U.width = camera.width;
     U.height = camera.height;
-    U.tile_shift = 3;
-    U.tiles_x = (camera.width + 7) / 8;
-    U.tiles_y = (camera.height + 7) / 8;
+    U.tile_shift = params.tile_size == 16 ? 4 : params.tile_size == 32 ? 5 : 3;
+    const int S = 1 << U.tile_shift;
+    U.tiles_x = (camera.width + S - 1) / S;
+    U.tiles_y = (camera.height + S - 1) / S;
+    U.sub_x = (camera.width + 7) / 8;
+    U.sub_y = (camera.height + 7) / 8;
     U.attenuation = params.attenuation;
     U.ert_alpha = params.early_ray_out_alpha;
     U.use_ert = params.use_early_ray_termination ? 1 : 0;
@@ -296,17 +300,25 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     if (!view || !in || !params) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
     if (n <= 0 || n > kMaxBatchItems) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": 1..4096 items required");
     const int tile = params->tile_size == 0 ? 8 : params->tile_size;
-    if (tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_size must be 8");
+    if (tile != 8 && tile != 16 && tile != 32)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_size must be 8, 16 or 32");
     if (params->tile_order != 0 && params->tile_order != 1)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_order must be 0 (heaviest first) or 1 (row-major)");
     if (params->segment_length != DTV_SEGMENT_EUCLIDEAN && params->segment_length != DTV_SEGMENT_LINEAR_DEPTH)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
     if (params->renderer != DTV_RENDERER_TILE && params->renderer != DTV_RENDERER_PROJECTION)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": renderer must be 0 (tile) or 1 (projection)");
+    const int spp = params->segments_per_pass == 0 ? 128 : params->segments_per_pass;
+    if (spp != 64 && spp != 128 && spp != 256)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segments_per_pass must be 64, 128 or 256");
     const bool proj = params->renderer == DTV_RENDERER_PROJECTION;
+    if (proj && tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": the Projection renderer bins 8x8 tiles");
+    dtv_render_params prm = *params;
+    prm.tile_size = tile;
+    params = &prm;
     std::vector<BatchItem> items((size_t)n);
     std::vector<int64_t> item_V((size_t)n);
-    int64_t blocks = 0, tiles = 0, pixels = 0;
+    int64_t blocks = 0, tiles = 0, subtiles = 0, pixels = 0;
     uint32_t kinds = 0;
     for (int i = 0; i < n; i++) {
         const dtv_render_item& it = in[i];
@@ -334,12 +346,14 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
         B.T = m.num_cells;
         B.block0 = (uint32_t)blocks;
         B.tile0 = (uint32_t)tiles;
+        B.sub0 = (uint32_t)subtiles;
         B.pixel0 = pixels;
         item_V[(size_t)i] = m.num_vertices;
         blocks += count_pairs_blocks(m.num_cells);
         tiles += (int64_t)B.U.tiles_x * B.U.tiles_y;
+        subtiles += (int64_t)B.U.sub_x * B.U.sub_y;
         pixels += (int64_t)B.U.width * B.U.height;
-        if (blocks >= (1ll << 31) / 256 || tiles >= (1ll << 31) || pixels >= (1ll << 31))
+        if (blocks >= (1ll << 31) / 256 || subtiles >= (1ll << 31) || pixels >= (1ll << 31))
             return fail(DTV_ERR_INVALID_ARGUMENT, f + ": batch too large");
     }
     DTV_HIP(hipSetDevice(view->ctx->device));
@@ -459,14 +473,22 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                                          view->hist.as<uint32_t>() + 8 * 256, route, s);
     view->mark(DTV_STAGE_SORT, 1, s);
     DTV_HIP(view->ranges.ensure(sizeof(uint2) * tiles));
-    DTV_HIP(view->tile_frags.ensure(sizeof(uint32_t) * tiles));
-    DTV_HIP(view->tile_order.ensure(2 * sizeof(uint32_t) * tiles));
+    DTV_HIP(view->tile_frags.ensure(sizeof(uint32_t) * subtiles));
+    DTV_HIP(view->tile_order.ensure(2 * sizeof(uint32_t) * subtiles));
     view->cost_ordered = params->tile_order == 0;
     // forward launch order: tiles by record count, heaviest first
     const uint32_t* fwd_order = view->cost_ordered ? view->tile_order.as<uint32_t>() : nullptr;
     view->mark(DTV_STAGE_RANGES, 0, s);
     launch_tile_ranges(route, K, n_dev, tiles, view->ranges.as<uint2>(), s);
-    if (fwd_order) launch_tile_order(view->ranges.as<uint2>(), nullptr, tiles, view->tile_order.as<uint32_t>(), s);
+    if (fwd_order) {
+        if (tile == 8) {
+            launch_tile_order(view->ranges.as<uint2>(), nullptr, tiles, view->tile_order.as<uint32_t>(), s);
+        } else {  // sub-tiles by their binning tile's records (the cost goes where the adjoint's order will be)
+            uint32_t* cost = view->tile_order.as<uint32_t>() + subtiles;
+            launch_subtile_cost(dI, n, subtiles, view->ranges.as<uint2>(), cost, s);
+            launch_tile_order(nullptr, cost, subtiles, view->tile_order.as<uint32_t>(), s);
+        }
+    }
     if (proj)
         launch_pt_gather(dI, n, view->pt_info.as<uint32_t>(), view->pt_vert.as<float4>(), route, K, n_dev,
                          view->recs.as<uint4>(), s);
@@ -487,8 +509,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                           view->pt_col.as<float4>(), view->sorted_recs(), view->ranges.as<uint2>(),
                           view->pixel_state(), s);
     else
-        launch_render_forward(dI, n, tiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
-                              view->pixel_state(), kinds, s);
+        launch_render_forward(dI, n, subtiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
+                              view->pixel_state(), kinds, s, spp, tile != 8);
     view->mark(DTV_STAGE_FORWARD, 1, s);
     DTV_HIP(hipGetLastError());
     view->colour_kinds = kinds;
@@ -497,6 +519,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     view->item_V.swap(item_V);
     view->total_blocks = blocks;
     view->total_tiles = tiles;
+    view->total_subtiles = subtiles;
+    view->sub_tiles = tile != 8;
     view->total_pixels = pixels;
     view->rendered = true;
     return DTV_OK;
@@ -559,8 +583,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     // adjoint launch order: tiles by the forward's composited fragments, heaviest first
     const uint32_t* adj_order = nullptr;
     if (view->cost_ordered) {
-        uint32_t* o = view->tile_order.as<uint32_t>() + view->total_tiles;
-        launch_tile_order(nullptr, view->tile_frags.as<uint32_t>(), view->total_tiles, o, s);
+        uint32_t* o = view->tile_order.as<uint32_t>() + view->total_subtiles;
+        launch_tile_order(nullptr, view->tile_frags.as<uint32_t>(), view->total_subtiles, o, s);
         adj_order = o;
     }
     view->mark(DTV_STAGE_ADJOINT, 0, s);
@@ -573,8 +597,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
                           view->tri_grad.as<float>(), tg_floats, use_abs_grad ? 1 : 0, s);
     } else {
         launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n,
-                              view->total_tiles, adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps,
-                              use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s);
+                              view->total_subtiles, adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps,
+                              use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s, view->sub_tiles);
     }
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
@@ -712,6 +736,7 @@ int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* cam
     B.T = m.num_cells;
     B.block0 = 0;
     B.tile0 = 0;
+    B.sub0 = 0;
     B.pixel0 = 0;
     const int64_t blocks = count_pairs_blocks(m.num_cells);
     const int64_t nslots = 256 * (blocks > 0 ? blocks : 1);
@@ -788,6 +813,93 @@ int dtv_render_batch(dtv_view* view, const dtv_render_item* items, int32_t num_i
     return render_batch(view, items, num_items, params, reinterpret_cast<hipStream_t>(stream), "dtv_render_batch");
 }
 
+int dtv_autotune(dtv_view* view, const dtv_render_item* items, int32_t num_items, const dtv_render_params* params,
+                 int32_t with_adjoint, int32_t repeats, dtv_tune_result* out, void* stream) {
+    const std::string f = "dtv_autotune";
+    if (!view || !items || !params || !out) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": NULL argument");
+    if (num_items <= 0 || num_items > kMaxBatchItems || repeats < 1 || repeats > 100)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": 1..4096 items and 1..100 repeats required");
+    if (params->renderer != DTV_RENDERER_TILE) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile renderer only");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    DTV_HIP(hipSetDevice(view->ctx->device));
+    // scratch adjoint inputs: a constant image adjoint per item, gradients per distinct mesh
+    std::vector<dtv_adjoint_item> adj((size_t)num_items);
+    std::vector<void*> scratch;
+    auto release = [&]() {
+        for (void* p : scratch) (void)hipFree(p);
+        scratch.clear();
+    };
+    if (with_adjoint) {
+        for (int i = 0; i < num_items; i++) {
+            const dtv_render_item& it = items[i];
+            const int64_t P = (int64_t)it.camera.width * it.camera.height;
+            const int64_t ncol = it.mesh.cell_colors ? it.mesh.num_cells : it.mesh.num_vertices;
+            void *dimg = nullptr, *gp = nullptr, *gc = nullptr;
+            if (hipMalloc(&dimg, sizeof(float) * 4 * (size_t)(P > 0 ? P : 1)) != hipSuccess ||
+                (scratch.push_back(dimg), hipMalloc(&gp, sizeof(float) * 3 * (size_t)(it.mesh.num_vertices + 1))) !=
+                    hipSuccess ||
+                (scratch.push_back(gp), hipMalloc(&gc, sizeof(float) * 4 * (size_t)(ncol + 1))) != hipSuccess) {
+                release();
+                return fail(DTV_ERR_OUT_OF_MEMORY, f + ": scratch gradients");
+            }
+            scratch.push_back(gc);
+            fill_u32(reinterpret_cast<uint32_t*>(dimg), 0x3A83126Fu /* 1e-3f */, 4 * P, s);
+            fill_u32(reinterpret_cast<uint32_t*>(gp), 0u, 3 * it.mesh.num_vertices, s);
+            fill_u32(reinterpret_cast<uint32_t*>(gc), 0u, 4 * ncol, s);
+            adj[(size_t)i].mesh = it.mesh;
+            adj[(size_t)i].dimage = static_cast<const float*>(dimg);
+            adj[(size_t)i].grad_positions = static_cast<float*>(gp);
+            adj[(size_t)i].grad_colors = static_cast<float*>(gc);
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        release();
+        return fail(DTV_ERR_HIP, f + ": hipEventCreate failed");
+    }
+    const int tiles[3] = {8, 16, 32}, spps[3] = {64, 128, 256};
+    int nc = 0, rc = DTV_OK;
+    float best = 0.f;
+    out->ms_default = -1.f;
+    for (int ti = 0; ti < 3 && rc == DTV_OK; ti++)
+        for (int si = 0; si < 3 && rc == DTV_OK; si++) {
+            dtv_render_params prm = *params;
+            prm.tile_size = tiles[ti];
+            prm.segments_per_pass = spps[si];
+            float total = 0.f;
+            for (int r = 0; r <= repeats && rc == DTV_OK; r++) {
+                if (r == 1) (void)hipEventRecord(e0, s);
+                rc = render_batch(view, items, num_items, &prm, s, "dtv_autotune");
+                if (rc == DTV_OK && with_adjoint)
+                    rc = dtv_render_adjoint_batch(view, adj.data(), num_items, 0, DTV_ADJOINT_EXACT, stream);
+            }
+            if (rc != DTV_OK) break;
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&total, e0, e1) != hipSuccess) {
+                rc = fail(DTV_ERR_HIP, f + ": timing failed");
+                break;
+            }
+            const float ms = total / (float)repeats;
+            out->cand_tile_size[nc] = tiles[ti];
+            out->cand_segments_per_pass[nc] = spps[si];
+            out->cand_ms[nc] = ms;
+            if (tiles[ti] == 8 && spps[si] == 128) out->ms_default = ms;
+            if (nc == 0 || ms < best) {
+                best = ms;
+                out->tile_size = tiles[ti];
+                out->segments_per_pass = spps[si];
+                out->ms = ms;
+            }
+            nc++;
+        }
+    out->num_candidates = nc;
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    release();
+    return rc;
+}
+
 int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage, int32_t use_abs_grad,
                        int32_t adjoint_mode, float* grad_positions, float* grad_colors, void* stream) {
     if (!view || !mesh || !dimage || !grad_positions || !grad_colors)
@@ -846,7 +958,7 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_view_export_fragments: mesh differs from the rendered one");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     DTV_HIP(hipSetDevice(view->ctx->device));
-    launch_export_fragments(view->dItems.as<BatchItem>(), view->nitems(), view->total_tiles, view->sorted_recs(),
+    launch_export_fragments(view->dItems.as<BatchItem>(), view->nitems(), view->total_subtiles, view->sorted_recs(),
                             view->ranges.as<uint2>(), view->pixel_state(), max_per_pixel, out_ids, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_emit_pairs(const BatchItem* __restrict_
         // Bounded by the stored count: the pass recomputes the same records, but never writes past its slots.
         for (int ty = b.ty0; ty <= b.ty1 && left > 0; ty++)
             for (int tx = b.tx0; tx <= b.tx1 && left > 0; tx++) {
-                if (test && !tile_may_cover(b.S, tx, ty)) continue;
+                if (test && !tile_may_cover(b.S, tx, ty, b.shift)) continue;
                 if (o < cap) {
                     keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | k;
                     vals[o] = id;
@@ -260,9 +260,33 @@ __global__ __launch_bounds__(256) void k_gather_records(const BatchItem* __restr
     project_face(U, vid, B.pos, (int)(id & 3), PT);  // the tet passed culling when its records were emitted
     ScreenTri S;
     face_screen(U, PT, id & 3, S);
-    const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
+    // larger binning tiles: each 8x8 sub-tile's renderer tests its own pixels (no mask here)
+    const uint64_t m =
+        U.tile_shift == 3 ? tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height) : 0ull;
     out[2 * i] = make_uint4((uint32_t)vid.x, (uint32_t)vid.y, (uint32_t)vid.z, (uint32_t)vid.w);
     out[2 * i + 1] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
+}
+
+// cost[g] = records of the binning tile that holds 8x8 sub-tile g (the forward's launch order for binning tiles
+// larger than 8x8, whose records every sub-tile of the tile walks).
+__global__ __launch_bounds__(256) void k_subtile_cost(const BatchItem* __restrict__ items, int nitems, int64_t nsub,
+                                                       const uint2* __restrict__ ranges, uint32_t* __restrict__ cost) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= nsub) return;
+    const BatchItem& B = items[item_of_sub(items, nitems, (uint32_t)g)];
+    const ViewUniforms& U = B.U;
+    const int sub = (int)((uint32_t)g - B.sub0);
+    const int k = U.tile_shift - 3;
+    const uint32_t t = B.tile0 + (uint32_t)(((sub / U.sub_x) >> k) * U.tiles_x + ((sub % U.sub_x) >> k));
+    const uint2 r = ranges[t];
+    cost[g] = r.y - r.x;
+}
+
+void launch_subtile_cost(const BatchItem* items, int nitems, int64_t nsub, const uint2* ranges, uint32_t* cost,
+                         hipStream_t s) {
+    if (nsub <= 0) return;
+    hipLaunchKernelGGL(k_subtile_cost, dim3((unsigned)((nsub + 255) / 256)), dim3(256), 0, s, items, nitems, nsub,
+                       ranges, cost);
 }
 
 int64_t count_pairs_blocks(int64_t T) { return T > 0 ? (T + 255) / 256 : 0; }

@@ -47,6 +47,8 @@ struct ViewUniforms {
     int32_t max_ckpt;
     float depth_axis[3];     // unit view axis in world space, -(row 2 of the view matrix)
     int32_t linear_depth;    // 1: segment length = difference of linear eye depths (PPLL), 0: Euclidean
+    int32_t sub_x, sub_y;    // 8x8 sub-tiles across and down (the tile renderers' workgroups); tiles_x/tiles_y
+                             // count the binning tiles of 8 << (tile_shift - 3) pixels
 };
 
 // One render of a batch: a mesh seen through one camera.  A batch is a device array of these, ordered so that
@@ -63,6 +65,7 @@ struct BatchItem {
     int64_t pixel0;      // first pixel in the batch's per-pixel state
     uint32_t block0;     // first 256-tet block in the batch's count / emit grids
     uint32_t tile0;      // first tile in the batch's tile numbering
+    uint32_t sub0;       // first 8x8 sub-tile in the batch's sub-tile numbering (== tile0 for 8x8 tiles)
 };
 
 // Adjoint inputs and gradient targets of one batch item (targets of items sharing a mesh may alias).
@@ -86,6 +89,16 @@ __device__ __forceinline__ int item_of_tile(const BatchItem* __restrict__ items,
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (items[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Item owning global 8x8 sub-tile `t`.
+__device__ __forceinline__ int item_of_sub(const BatchItem* __restrict__ items, int n, uint32_t t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].sub0 <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
 }

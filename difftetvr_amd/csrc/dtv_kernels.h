@@ -58,6 +58,9 @@ void launch_tile_ranges(const SortRoute* route, int64_t n, const uint64_t* n_dev
 // tile renderer's record stream.
 void launch_gather_records(const BatchItem* items, int nitems, const SortRoute* route, int64_t n,
                            const uint64_t* n_dev, uint4* out, hipStream_t s);
+// cost[g] = record count of the binning tile holding 8x8 sub-tile g, for g < nsub.
+void launch_subtile_cost(const BatchItem* items, int nitems, int64_t nsub, const uint2* ranges, uint32_t* cost,
+                         hipStream_t s);
 // Launch order of the tile renderers: order[0, num_tiles) lists every tile once, heaviest first, by the tile's
 // cost: its record count (cost == NULL: ranges[t].y - ranges[t].x) or cost[t].  Costs are ranked in buckets of
 // 1/8 octave; the order within a bucket is unspecified.  One workgroup; needs no cleared memory.
@@ -81,12 +84,15 @@ struct PixelState {
 // colour_kinds: which colour storages the batch's items use (kColoursPerVertex | kColoursPerCell); one launch
 // per kind present, each rendering the tiles of its items.
 constexpr uint32_t kColoursPerVertex = 1, kColoursPerCell = 2;
+// round: fragments shaded per round of the forward (64, 128 or 256; other values take 128).  ntiles counts
+// 8x8 sub-tiles (one workgroup each).
+// sub: the batch bins tiles larger than 8x8 (the kernels then test coverage per 8x8 sub-tile).
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
                            const uint4* recs, const uint2* ranges, PixelState ps, uint32_t colour_kinds,
-                           hipStream_t s);
+                           hipStream_t s, int round = 128, bool sub = false);
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
-                           int mode, uint32_t colour_kinds, hipStream_t s);
+                           int mode, uint32_t colour_kinds, hipStream_t s, bool sub = false);
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 

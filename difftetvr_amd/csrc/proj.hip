@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_pt_emit(const BatchItem* __restrict__ i
         const uint32_t id = (uint32_t)(tet * 4 + k);
         for (int ty = b.ty0; ty <= b.ty1 && left > 0; ty++)
             for (int tx = b.tx0; tx <= b.tx1 && left > 0; tx++) {
-                if (test && !tile_may_cover(b.S, tx, ty)) continue;
+                if (test && !tile_may_cover(b.S, tx, ty, b.shift)) continue;
                 if (o < cap) {
                     keys[o] = ((uint64_t)(B.tile0 + (uint32_t)(ty * U.tiles_x + tx)) << 32) | (uint64_t)key;
                     vals[o] = id;

@@ -9,6 +9,7 @@ namespace dtv {
 
 struct FaceBins {
     int32_t tx0, tx1, ty0, ty1;  // tile range of the face's pixel-centre bounding box (empty: tx0 > tx1)
+    int32_t shift;               // log2 of the tile edge in pixels (3, 4 or 5)
     ScreenTri S;
 };
 
@@ -19,14 +20,15 @@ __device__ __forceinline__ FaceBins screen_bins(const ViewUniforms& U, const Scr
     b.tx1 = -1;
     b.ty0 = 0;
     b.ty1 = -1;
+    b.shift = U.tile_shift;
     b.S = S;
     int32_t x0, x1, y0, y1;
     tri_pixel_bbox(U, b.S, x0, x1, y0, y1);
     if (x0 > x1 || y0 > y1) return b;
-    b.tx0 = x0 >> 3;
-    b.tx1 = x1 >> 3;
-    b.ty0 = y0 >> 3;
-    b.ty1 = y1 >> 3;
+    b.tx0 = x0 >> b.shift;
+    b.tx1 = x1 >> b.shift;
+    b.ty0 = y0 >> b.shift;
+    b.ty1 = y1 >> b.shift;
     return b;
 }
 
@@ -44,10 +46,11 @@ struct TileEdge {
     bool tie;
 };
 
-__device__ __forceinline__ TileEdge tile_edge(const ScreenTri& S, int e, int tx, int ty) {
+// (tx, ty): a tile of 1 << shift pixels (8 by default).
+__device__ __forceinline__ TileEdge tile_edge(const ScreenTri& S, int e, int tx, int ty, int shift = 3) {
     const int a = e, b = e == 2 ? 0 : e + 1;
     const long long dx = S.X[b] - S.X[a], dy = S.Y[b] - S.Y[a];
-    const long long px0 = 256LL * (8 * tx) + 128, py0 = 256LL * (8 * ty) + 128;
+    const long long px0 = 256LL * ((long long)tx << shift) + 128, py0 = 256LL * ((long long)ty << shift) + 128;
     TileEdge t;
     t.C = dx * (py0 - S.Y[a]) - dy * (px0 - S.X[a]);
     t.sx = -dy * 256;
@@ -58,14 +61,31 @@ __device__ __forceinline__ TileEdge tile_edge(const ScreenTri& S, int e, int tx,
 
 // False when some edge function is negative at every pixel centre of tile (tx, ty): a record of that tile
 // could not cover a pixel.  (The maximum of E over the tile's centre lattice sits at a corner.)
-__device__ __forceinline__ bool tile_may_cover(const ScreenTri& S, int tx, int ty) {
+__device__ __forceinline__ bool tile_may_cover(const ScreenTri& S, int tx, int ty, int shift = 3) {
+    const long long n1 = (1 << shift) - 1;
 #pragma unroll
     for (int e = 0; e < 3; e++) {
-        const TileEdge t = tile_edge(S, e, tx, ty);
-        const long long mx = t.C + (t.sx > 0 ? 7 * t.sx : 0) + (t.sy > 0 ? 7 * t.sy : 0);
+        const TileEdge t = tile_edge(S, e, tx, ty, shift);
+        const long long mx = t.C + (t.sx > 0 ? n1 * t.sx : 0) + (t.sy > 0 ? n1 * t.sy : 0);
         if (mx < 0) return false;
     }
     return true;
+}
+
+// Pixel (x, y)'s centre inside oriented screen triangle (X, Y): every edge function E > 0, or E == 0 on a tie
+// edge; the rule of tile_coverage, for one pixel.
+__device__ __forceinline__ bool pixel_inside(const int32_t X[3], const int32_t Y[3], int x, int y) {
+    const long long px = 256LL * x + 128, py = 256LL * y + 128;
+    bool in = true;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const int a = e, b = e == 2 ? 0 : e + 1;
+        const long long dx = (long long)X[b] - X[a], dy = (long long)Y[b] - Y[a];
+        const long long E = dx * (py - Y[a]) - dy * (px - X[a]);
+        const bool tie = dy < 0 || (dy == 0 && dx > 0);
+        in = in && (E > 0 || (E == 0 && tie));
+    }
+    return in;
 }
 
 // Faces whose bounding box spans more tiles than this keep every tile of it (bounded per-thread work).
@@ -80,7 +100,7 @@ __device__ __forceinline__ uint32_t face_record_count(const FaceBins& b) {
     if (!test_tiles(b)) return (uint32_t)((b.tx1 - b.tx0 + 1) * (b.ty1 - b.ty0 + 1));
     uint32_t c = 0;
     for (int ty = b.ty0; ty <= b.ty1; ty++)
-        for (int tx = b.tx0; tx <= b.tx1; tx++) c += tile_may_cover(b.S, tx, ty) ? 1u : 0u;
+        for (int tx = b.tx0; tx <= b.tx1; tx++) c += tile_may_cover(b.S, tx, ty, b.shift) ? 1u : 0u;
     return c;
 }
 

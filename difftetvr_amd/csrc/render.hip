@@ -23,13 +23,14 @@
 
 namespace dtv {
 
-constexpr int kRound = 128;                 // fragments shaded per round (forward)
+constexpr int kRoundDefault = 128;          // fragments shaded per round of the forward (segments per pass)
 constexpr int kRoundAdj = 64;               // fragments per round in the adjoint: one per lane
 constexpr int kMaxFrags = kBatch * 64;      // fragments a batch can produce
 constexpr float kMissMarker = -1.0f;        // transmittance slot of a fragment whose ray missed the tet
 constexpr float kTinyT = 0x1p-100f;         // transmittance below which a fragment's input is not recovered
 
-// The forward's fragment list and round values.
+// The forward's fragment list and round values (kRound fragments shaded per round).
+template <int kRound>
 struct FragListLds {
     uint16_t entry[kMaxFrags];  // (pixel lane << 5) | record index
     float4 val[kRound];         // fragment colour F
@@ -74,26 +75,31 @@ __device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
 
 // kCell: renders the items with per-cell colours (BatchItem::cell_col), else those with per-vertex colours; a
 // batch that holds both is rendered by one launch of each, every tile by the one matching its item.
-template <bool kCell>
+// kRound: fragments shaded per round (64, 128 or 256: segments per pass).  kSub: binning tiles larger than 8x8
+// (the workgroup's 8x8 sub-tile tests its own coverage).
+template <bool kCell, int kRound, bool kSub>
 __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
                                                         const uint32_t* __restrict__ order,
                                                         const uint4* __restrict__ recs,
                                                         const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ RecordLds L;
-    __shared__ FragListLds F;
-    const uint32_t gtile = global_tile_of_block(order);
-    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    __shared__ FragListLds<kRound> F;
+    const uint32_t gsub = global_tile_of_block(order);
+    const BatchItem& B = items[item_of_sub(items, nitems, gsub)];
     if ((B.cell_col != nullptr) != kCell) return;
     const ViewUniforms& U = B.U;
     const float* __restrict__ pos = B.pos;
     const float* __restrict__ col = kCell ? B.cell_col : B.col;
     float4* __restrict__ out_image = reinterpret_cast<float4*>(B.image);
     const PixelState ps = item_pixel_state(ps_all, B);
-    const int tile = (int)(gtile - B.tile0);
-    const int tx = tile % U.tiles_x, ty = tile / U.tiles_x;
+    const SubTile st = sub_tile(B, gsub);
+    const uint32_t gtile = st.gtile;
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
-    const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
+    const int x = st.x0 + lx, y = st.y0 + ly;
+    // binning tiles larger than 8x8: coverage is tested here, from face triangles stashed in the fragment
+    // list's space (free while a batch is staged)
+    int32_t* stash = kSub ? reinterpret_cast<int32_t*>(F.entry) : nullptr;
     const bool valid = x < U.width && y < U.height;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
     {
@@ -121,13 +127,17 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     if (lane < kBatch && rg.x + kBatch + lane < rg.y) nxt = load_record(recs, rg.x + kBatch + lane);
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) store_record(U, cur, cd, L, lane);
+        if (lane < cnt) store_record<kSub>(U, cur, cd, L, lane, stash);
         if (lane < kBatch && base + kBatch + lane < rg.y) {
             cur = nxt;
             fetch_record_data<kCell>(pos, col, cur, cd);
         }
         if (lane < kBatch && base + 2 * kBatch + lane < rg.y) nxt = load_record(recs, base + 2 * kBatch + lane);
         __syncthreads();
+        if constexpr (kSub) {
+            subtile_masks(L, stash, cnt, x, y, valid, lane);
+            __syncthreads();
+        }
         const uint32_t mask = done ? 0u : lane_coverage(L, cnt, lane);
         uint32_t total;
         const uint32_t off = build_fragment_list(F.entry, mask, false, lane, total);
@@ -198,15 +208,16 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     }
     // the tile's composited fragments: the adjoint's launch order ranks tiles by them
     const uint32_t tile_total = wave_inclusive_scan(n);
-    if (lane == 63) ps.tile_frags[gtile] = tile_total;
+    if (lane == 63) ps.tile_frags[gsub] = tile_total;
 }
 
 // kRef = false: exact derivative of the forward.  kRef = true: the reference's adjoint as written
 // (AdjointIntersectRasterization.glsl:111-362): per pixel from the blended output image, back to front, the
 // 'over' operator is inverted per sub-segment 3, 2, 1 (sub-segment 0 is skipped) with no checkpoints, and
 // each fragment's parameter gradients follow frag_backward_reference.
-// kCell: the items with per-cell colours (exact adjoint only), as k_render_forward.
-template <bool kRef, bool kCell>
+// kCell: the items with per-cell colours (exact adjoint only), kSub: binning tiles larger than 8x8, as
+// k_render_forward.
+template <bool kRef, bool kCell, bool kSub>
 // Waves per SIMD: 3 for the exact adjoint (168 VGPRs of the 512 per lane, 13 KB of LDS), 2 for the reference
 // one, whose sub-segment arrays need more of both.
 __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const BatchItem* __restrict__ items,
@@ -218,8 +229,8 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
     __shared__ RecordLds L;
     __shared__ AdjFragLds A;
     __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
-    const uint32_t gtile = global_tile_of_block(order);
-    const int item = item_of_tile(items, nitems, gtile);
+    const uint32_t gsub = global_tile_of_block(order);
+    const int item = item_of_sub(items, nitems, gsub);
     const BatchItem& B = items[item];
     if ((B.cell_col != nullptr) != kCell) return;
     const ViewUniforms& U = B.U;
@@ -230,12 +241,13 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
     float* __restrict__ grad_pos = targets[item].grad_pos;
     float* __restrict__ grad_col = targets[item].grad_col;
     const PixelState ps = item_pixel_state(ps_all, B);
-    const int tile = (int)(gtile - B.tile0);
-    const int tx = tile % U.tiles_x, ty = tile / U.tiles_x;
+    const SubTile st = sub_tile(B, gsub);
+    const uint32_t gtile = st.gtile;
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
-    const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
+    const int x = st.x0 + lx, y = st.y0 + ly;
     const bool valid = x < U.width && y < U.height;
+    int32_t* stash = kSub ? reinterpret_cast<int32_t*>(A.entry) : nullptr;
     const float3_ ro = f3(U.camPos[0], U.camPos[1], U.camPos[2]);
     const float3_ my_rd = ray_direction(U, x, y);
     const float att = U.attenuation;
@@ -297,13 +309,17 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
     for (int bi = nbatch - 1; bi >= 0; bi--) {
         const uint32_t base = rg.x + (uint32_t)bi * kBatch;
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) store_record(U, nxt, nd, L, lane);
+        if (lane < cnt) store_record<kSub>(U, nxt, nd, L, lane, stash);
         if (bi > 0 && lane < kBatch) nxt = load_record(recs, base - kBatch + lane);
         // lanes j and j + 32 own record j's 28 values (first / second half of 14) for this batch
         float acc[14];
 #pragma unroll
         for (int q = 0; q < 14; q++) acc[q] = 0.f;
         __syncthreads();
+        if constexpr (kSub) {
+            subtile_masks(L, stash, cnt, x, y, valid, lane);
+            __syncthreads();
+        }
         // records at or after `last` were not composited in the forward (early ray termination)
         const uint32_t nlive = last > base ? min((uint32_t)cnt, last - base) : 0u;
         const uint32_t live = nlive >= 32u ? 0xFFFFFFFFu : ((1u << nlive) - 1u);
@@ -616,26 +632,39 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
                                                           const uint2* __restrict__ ranges, PixelState ps_all,
                                                           int max_per_pixel, int32_t* __restrict__ out_ids_all) {
     __shared__ RecordLds L;
-    const uint32_t gtile = global_tile_of_block(nullptr);
-    const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
+    __shared__ int32_t s_stash[6 * kBatch];
+    const uint32_t gsub = global_tile_of_block(nullptr);
+    const BatchItem& B = items[item_of_sub(items, nitems, gsub)];
     const ViewUniforms& U = B.U;
     const float* __restrict__ pos = B.pos;
     const PixelState ps = item_pixel_state(ps_all, B);
     int32_t* __restrict__ out_ids = out_ids_all + B.pixel0 * max_per_pixel;
-    const int tile = (int)(gtile - B.tile0);
-    const int tx = tile % U.tiles_x, ty = tile / U.tiles_x;
+    const SubTile st = sub_tile(B, gsub);
     const int lane = threadIdx.x;
     const int lx = lane & 7, ly = lane >> 3;
-    const int x = tx * kTileEdge + lx, y = ty * kTileEdge + ly;
+    const int x = st.x0 + lx, y = st.y0 + ly;
     const bool valid = x < U.width && y < U.height;
-    const uint2 rg = ranges[gtile];
+    int32_t* stash = U.tile_shift != 3 ? s_stash : nullptr;
+    const uint2 rg = ranges[st.gtile];
     const int pix = valid ? pixel_index(U, x, y) : 0;
     const uint32_t last = valid ? ps.last_rec[pix] : 0u;
     int k = 0;
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt) setup_record(U, pos, nullptr, load_record(recs, base + lane), L, lane);
+        if (lane < cnt) {
+            RecordData d;
+            const RecordIn r = load_record(recs, base + lane);
+            fetch_record_data(pos, nullptr, r, d);
+            if (stash != nullptr)
+                store_record<true>(U, r, d, L, lane, stash);
+            else
+                store_record<false>(U, r, d, L, lane, nullptr);
+        }
         __syncthreads();
+        if (stash != nullptr) {
+            subtile_masks(L, stash, cnt, x, y, valid, lane);
+            __syncthreads();
+        }
         uint32_t m = valid ? lane_coverage(L, cnt, lane) : 0u;
         while (m) {
             const int j = __ffs(m) - 1;
@@ -651,33 +680,65 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
         for (; k < max_per_pixel; k++) out_ids[(int64_t)pix * max_per_pixel + k] = -1;
 }
 
+template <bool kCell, bool kSub>
+static void launch_forward_kind(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
+                                const uint4* recs, const uint2* ranges, PixelState ps, int round, hipStream_t s) {
+    if (round == 64)
+        hipLaunchKernelGGL((k_render_forward<kCell, 64, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                           nitems, order, recs, ranges, ps);
+    else if (round == 256)
+        hipLaunchKernelGGL((k_render_forward<kCell, 256, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                           nitems, order, recs, ranges, ps);
+    else
+        hipLaunchKernelGGL((k_render_forward<kCell, kRoundDefault, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s,
+                           items, nitems, order, recs, ranges, ps);
+}
+
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
                            const uint4* recs, const uint2* ranges, PixelState ps, uint32_t colour_kinds,
-                           hipStream_t s) {
+                           hipStream_t s, int round, bool sub) {
     if (ntiles <= 0) return;
-    if (colour_kinds & kColoursPerVertex)
-        hipLaunchKernelGGL(k_render_forward<false>, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order,
-                           recs, ranges, ps);
-    if (colour_kinds & kColoursPerCell)
-        hipLaunchKernelGGL(k_render_forward<true>, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order,
-                           recs, ranges, ps);
+    if (colour_kinds & kColoursPerVertex) {
+        if (sub)
+            launch_forward_kind<false, true>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+        else
+            launch_forward_kind<false, false>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+    }
+    if (colour_kinds & kColoursPerCell) {
+        if (sub)
+            launch_forward_kind<true, true>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+        else
+            launch_forward_kind<true, false>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+    }
+}
+
+template <bool kSub>
+static void launch_adjoint_sub(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
+                               const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps,
+                               int use_abs, int mode, uint32_t colour_kinds, hipStream_t s) {
+    if (colour_kinds & kColoursPerVertex) {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_render_adjoint<true, false, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                               targets, nitems, order, recs, ranges, ps, use_abs);
+        else
+            hipLaunchKernelGGL((k_render_adjoint<false, false, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                               targets, nitems, order, recs, ranges, ps, use_abs);
+    }
+    if (colour_kinds & kColoursPerCell)  // exact adjoint only (the caller rejects mode 1 for such items)
+        hipLaunchKernelGGL((k_render_adjoint<false, true, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
+                           targets, nitems, order, recs, ranges, ps, use_abs);
 }
 
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
-                           int mode, uint32_t colour_kinds, hipStream_t s) {
+                           int mode, uint32_t colour_kinds, hipStream_t s, bool sub) {
     if (ntiles <= 0) return;
-    if (colour_kinds & kColoursPerVertex) {
-        if (mode == 1)
-            hipLaunchKernelGGL((k_render_adjoint<true, false>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                               targets, nitems, order, recs, ranges, ps, use_abs);
-        else
-            hipLaunchKernelGGL((k_render_adjoint<false, false>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                               targets, nitems, order, recs, ranges, ps, use_abs);
-    }
-    if (colour_kinds & kColoursPerCell)  // exact adjoint only (the caller rejects mode 1 for such items)
-        hipLaunchKernelGGL((k_render_adjoint<false, true>), dim3((unsigned)ntiles), dim3(64), 0, s, items, targets,
-                           nitems, order, recs, ranges, ps, use_abs);
+    if (sub)
+        launch_adjoint_sub<true>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs, mode,
+                                 colour_kinds, s);
+    else
+        launch_adjoint_sub<false>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs, mode,
+                                  colour_kinds, s);
 }
 
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,

@@ -2,6 +2,7 @@
 #pragma once
 #include "dtv_kernels.h"
 #include "dtv_prims.h"
+#include "raster.h"
 
 namespace dtv {
 
@@ -59,8 +60,11 @@ __device__ __forceinline__ void fetch_record_data(const float* __restrict__ pos,
     }
 }
 
+// kStash (binning tiles larger than 8x8): stash [6][kBatch] int32 receives the record's face as the oriented
+// fixed-point screen triangle of binning (project_face + face_screen arithmetic), for subtile_masks.
+template <bool kStash = false>
 __device__ __forceinline__ void store_record(const ViewUniforms& U, const RecordIn& r, const RecordData& d,
-                                             RecordLds& L, int i) {
+                                             RecordLds& L, int i, int32_t* stash = nullptr) {
     const uint4 rec = r.rec;
     const int32_t vid[4] = {(int32_t)r.vid.x, (int32_t)r.vid.y, (int32_t)r.vid.z, (int32_t)r.vid.w};
     const float3_* p = d.p;
@@ -91,6 +95,75 @@ __device__ __forceinline__ void store_record(const ViewUniforms& U, const Record
         if (g == 0 && dot(n, sub(p[3], p[0])) < 0.0f) meta |= 8u;
     }
     L.meta[i] = meta;
+    if constexpr (kStash) {
+        const int f = (int)(rec.x & 3u);
+        const float hw = 0.5f * (float)U.width, hh = 0.5f * (float)U.height;
+        int32_t X[3], Y[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int v = face_vertex(f, k);
+            const float px = sel4(v, p[0].x, p[1].x, p[2].x, p[3].x);
+            const float py = sel4(v, p[0].y, p[1].y, p[2].y, p[3].y);
+            const float pz = sel4(v, p[0].z, p[1].z, p[2].z, p[3].z);
+            float c[4];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                c[rr] = ((U.viewProj[0 * 4 + rr] * px + U.viewProj[1 * 4 + rr] * py) + U.viewProj[2 * 4 + rr] * pz) +
+                        U.viewProj[3 * 4 + rr] * 1.0f;
+            const float nx = c[0] / c[3], ny = c[1] / c[3];
+            const float xf = nx * hw + hw;
+            const float yf = ny * hh + hh;
+            X[k] = (int32_t)rintf(xf * 256.0f);
+            Y[k] = (int32_t)rintf(yf * 256.0f);
+        }
+        const int64_t area =
+            (int64_t)(X[1] - X[0]) * (Y[2] - Y[0]) - (int64_t)(Y[1] - Y[0]) * (X[2] - X[0]);
+        if (area < 0) {
+            const int32_t tx = X[1], ty = Y[1];
+            X[1] = X[2];
+            Y[1] = Y[2];
+            X[2] = tx;
+            Y[2] = ty;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            stash[k * kBatch + i] = X[k];
+            stash[(3 + k) * kBatch + i] = Y[k];
+        }
+    }
+}
+
+// Coverage masks of the batch's records for this wave's 8x8 sub-tile (binning tiles larger than 8x8), from the
+// screen triangles store_record stashed: L.mask as the binning's gather writes it for 8x8 tiles.
+__device__ __forceinline__ void subtile_masks(RecordLds& L, const int32_t* stash, int cnt, int x, int y,
+                                              bool valid, int lane) {
+    for (int j = 0; j < cnt; j++) {
+        const int32_t X[3] = {stash[j], stash[kBatch + j], stash[2 * kBatch + j]};
+        const int32_t Y[3] = {stash[3 * kBatch + j], stash[4 * kBatch + j], stash[5 * kBatch + j]};
+        const uint64_t m = ballot(valid && pixel_inside(X, Y, x, y));
+        if (lane == 0) {
+            L.mask[0][j] = (uint32_t)m;
+            L.mask[1][j] = (uint32_t)(m >> 32);
+        }
+    }
+}
+
+// The 8x8 sub-tile a workgroup renders: global sub-tile gsub of item B, its binning tile (records and ranges
+// are per binning tile) and its first pixel.
+struct SubTile {
+    uint32_t gtile;
+    int x0, y0;
+};
+__device__ __forceinline__ SubTile sub_tile(const BatchItem& B, uint32_t gsub) {
+    const ViewUniforms& U = B.U;
+    const int sub = (int)(gsub - B.sub0);
+    const int sx = sub % U.sub_x, sy = sub / U.sub_x;
+    const int k = U.tile_shift - 3;
+    SubTile t;
+    t.gtile = B.tile0 + (uint32_t)((sy >> k) * U.tiles_x + (sx >> k));
+    t.x0 = sx * kTileEdge;
+    t.y0 = sy * kTileEdge;
+    return t;
 }
 
 __device__ __forceinline__ void setup_record(const ViewUniforms& U, const float* __restrict__ pos,

@@ -65,7 +65,8 @@ typedef struct dtv_render_params {
     float clear_color[4];
     int32_t use_early_ray_termination; /* applied only when nonzero (the module passes use_ert && use_gradients) */
     float early_ray_out_alpha;         /* stop once accumulated alpha > this; reference default 1 - 1e-4 */
-    int32_t tile_size;                 /* screen tile edge in pixels: 8 (default) */
+    int32_t tile_size;                 /* binning tile edge in pixels: 8 (0 = default), 16 or 32; rendered as
+                                          8x8 sub-tiles (no reference counterpart: the reference rasterises) */
     int32_t checkpoint_interval;       /* fragments between adjoint checkpoints; 0 = 64 */
     int32_t tile_order;                /* launch order of screen tiles (no reference counterpart: the reference
                                           rasterises): 0 = heaviest first (default; forward by record count,
@@ -73,6 +74,8 @@ typedef struct dtv_render_params {
     int32_t segment_length;            /* DTV_SEGMENT_EUCLIDEAN (INTERSECTION renderer) or
                                           DTV_SEGMENT_LINEAR_DEPTH (PPLL renderer) */
     int32_t renderer;                  /* DTV_RENDERER_TILE (0) or DTV_RENDERER_PROJECTION (1) */
+    int32_t segments_per_pass;         /* ray segments the forward shades per round: 64, 128 (0 = default) or
+                                          256 (no reference counterpart) */
 } dtv_render_params;
 
 /* Which renderer dtv_render / dtv_render_batch runs.  TILE: per-pixel ray-tet segments of the INTERSECTION and
@@ -205,6 +208,25 @@ int dtv_view_export_fragments(dtv_view* view, const dtv_mesh* mesh, int32_t max_
                               void* stream);
 
 int dtv_view_get_stats(dtv_view* view, dtv_view_stats* out);
+
+/* Autotuner (no reference counterpart; the nearest is the PPLL renderer's per-mesh-size fragment budget,
+ * src/Renderer/TetMeshRendererPPLL.cpp:290-318, .hpp:48-54).  Times dtv_render_batch (and, with_adjoint, the
+ * exact adjoint into scratch gradients) of these items for every tile_size in {8, 16, 32} and segments_per_pass
+ * in {64, 128, 256}: one untimed run, then `repeats` timed runs each, hipEvents on `stream`.  params supplies
+ * the other settings.  The items' out_image buffers are written; the view is left rendered with the last
+ * candidate.  The tile renderer only (params->renderer 0). */
+typedef struct dtv_tune_result {
+    int32_t tile_size;          /* the fastest candidate */
+    int32_t segments_per_pass;
+    float ms;                   /* its mean time per run */
+    float ms_default;           /* 8x8 tiles, 128 segments per pass */
+    int32_t num_candidates;
+    int32_t cand_tile_size[9];
+    int32_t cand_segments_per_pass[9];
+    float cand_ms[9];
+} dtv_tune_result;
+int dtv_autotune(dtv_view* view, const dtv_render_item* items, int32_t num_items, const dtv_render_params* params,
+                 int32_t with_adjoint, int32_t repeats, dtv_tune_result* out, void* stream);
 
 /* Planned (synchronisation-free) rendering.  With record_capacity > 0, dtv_render / dtv_render_batch size the
  * (tile, face) record buffers for that many records instead of synchronising on the count, sort all four
