@@ -319,6 +319,23 @@ def bench_uneven(args):
     st = br.get_stats()
     ms_b = (tb - ta) * 1000.0 / args.steps
     ms_s = (tc - tb) * 1000.0 / args.steps
+    # autotuner: time every tile size / segments-per-pass setting on this batch, then the step with its choice
+    tune = None
+    if not args.no_autotune:
+        br.set_autotune(True, with_adjoint=True, repeats=3)
+        batched_step()  # the first render tunes
+        choice = list(d.autotune_cache().values())[-1]
+        for _ in range(args.warmup):
+            batched_step()
+        torch.cuda.synchronize()
+        td = time.perf_counter()
+        for _ in range(args.steps):
+            batched_step()
+        torch.cuda.synchronize()
+        ms_t = (time.perf_counter() - td) * 1000.0 / args.steps
+        tune = {"tile_size": choice["tile_size"], "segments_per_pass": choice["segments_per_pass"],
+                "ms_per_step_tuned": ms_t, "ms_per_step_8x8_128": ms_b, "gain": ms_b / ms_t,
+                "render_adjoint_ms": {f"{t}/{sp}": round(ms, 3) for t, sp, ms in choice["candidates"]}}
     out = {
         "metric": "uneven 8-mesh batch fwd+adjoint+Adam ms/step, Delaunay 2k..512k points @768x768",
         "value": ms_b, "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -328,6 +345,8 @@ def bench_uneven(args):
                    "single_view_ms_per_step": ms_s, "speedup_vs_single_view": ms_s / ms_b,
                    "mesh_setup_s": round(t_setup, 1), "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()}},
     }
+    if tune is not None:
+        out["autotune"] = tune
     print(json.dumps(out), flush=True)
 
 
@@ -489,6 +508,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--res", type=int, default=None, help="viewport edge (default 1024; 512 for multiview)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="uneven workload: skip the autotuner comparison (tile size, segments per pass)")
     ap.add_argument("--workload", choices=["single", "multiview", "uneven", "serve"], default="single",
                     help="single: the plain single-view step (default); multiview: 16 views per batched step; "
                          "uneven: 8 Delaunay meshes in one batch; serve: Poisson request stream over 32 meshes")
