@@ -14,6 +14,11 @@
  *   5. per-fragment ray-tet compositing, front to back    IntersectRasterization.glsl:139-273
  *   6. adjoints (exact; and the reference's own, quirks kept)  AdjointIntersectRasterization.glsl:111-362
  *   7. torch.optim.Adam + colour clamp                    pytests/train.py:318,351-356
+ *   8. per-cell colours (accumulateConst)                 IntersectRasterization.glsl:250-255, TetMesh.cpp:294-367
+ *   9. the Projection renderer (projected tetrahedra): per-tet class and thick vertex, 3 or 4 screen triangles
+ *      sorted by centroid depth, linear interpolation of colour and thickness, alpha = 1-exp(-a d)
+ *                                                         Projection/GenerateTrianglesVTK.glsl:129-403,
+ *                                                         Projection/ProjectedRasterization.glsl:85-105
  *
  * Parity status (how each part is pinned; see tests/test_oracle_*.py):
  *   camera uniforms      closed-form numpy float64 check
@@ -31,6 +36,13 @@
  *                        fragments (sub-segment 0 never inverted) by FP64 central differences of the closed form
  *                        it gives each tet's gradient (tests/test_reference_adjoint_pinning.py)
  *   Adam                 golden vectors written by torch.optim.Adam (tests/golden/adam_golden.npz)
+ *   per-cell colours     closed form of a constant field (1-exp(-sigma*chord) per cell); the per-cell path
+ *                        equals the per-vertex path on the unlinked mesh whose four corners carry the cell
+ *                        colour; colour gradient by FP64 central differences (tests/test_cell_colors.py)
+ *   Projection           one tet, class 1 and class 2: thick vertex, crossing and thickness against an
+ *                        independent float64 construction (line crossings, eye-ray closest points); seen from
+ *                        afar through a narrow field of view, the interpolated thickness sums to the chord of
+ *                        the constant-colour mesh; adjoint by FP64 central differences (tests/test_projection.py)
  */
 #include <math.h>
 #include <stdint.h>
