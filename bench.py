@@ -124,19 +124,36 @@ def bench_multiview(args, emit=True):
         br.set_attenuation(1.0)
         br.set_camera_fovy(fovy)
         br.set_views(m, views, W, H)
+        br.set_tile_launch(getattr(args, "tile_launch", "grid"))
         return br
 
     targets = batch_renderer(mesh_of(w["pos_gt"], False)).render().detach().clone()
     mesh = mesh_of(w["pos"], True)
     br = batch_renderer(mesh)
+    graph_mode = bool(getattr(args, "graph", False))
     opt = FusedAdam([{"params": [mesh.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
-                     {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True)
+                     {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True,
+                    device_step=graph_mode)
 
-    def batched_step():
+    def eager_step():
         imgs = br.render()
         _, dimg = mse_loss_grad_views(imgs, targets)
         br.render_adjoint(dimg, False)
         opt.step()
+
+    batched_step = eager_step
+    graph = None
+    if graph_mode:
+        # planned rendering (record buffers sized from one ordinary render) and the whole step in one graph
+        eager_step()
+        torch.cuda.synchronize()
+        br.set_record_capacity(int(br.get_stats()["num_pairs"] * 1.25) + 4096)
+        eager_step()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            eager_step()
+        batched_step = graph.replay
 
     # the plain single-view path on a second copy of the mesh
     mesh1 = mesh_of(w["pos"], True)
@@ -173,11 +190,13 @@ def bench_multiview(args, emit=True):
         single_steps()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    if graph is not None and br.records_overflowed()[0]:
+        raise RuntimeError("bench: record capacity exceeded during the graph replays")
     br.set_timing(True)
     acc = {}
     nt = 3
     for _ in range(nt):
-        batched_step()
+        eager_step()  # stage timers need the eager launches
         for k, v in br.get_timings().items():
             acc[k] = acc.get(k, 0.0) + v / nt
     br.set_timing(False)
@@ -204,6 +223,7 @@ def bench_multiview(args, emit=True):
         "config": {"workload": f"{nv} views on a Fibonacci sphere of radius 2, {W}x{H} each, one batched "
                                f"render + summed MSE + batched adjoint + Adam per step, {args.grid}^3 grid x6 tets",
                    "tets": T, "vertices": V, "pixels": nv * P, "tile_pairs": stats["num_pairs"],
+                   "hip_graph": graph is not None, "tile_launch": getattr(args, "tile_launch", "grid"),
                    "single_view_views_per_s": 1000.0 / ms_single_view,
                    "speedup_vs_single_view": ms_single_view * nv / ms_batch,
                    "stage_ms": {k: round(v, 4) for k, v in acc.items()}},
@@ -519,11 +539,14 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="serve: worker threads (one stream and view each)")
     ap.add_argument("--max-batch", type=int, default=16, help="serve: most requests per batch")
     ap.add_argument("--graph", action="store_true",
-                    help="single: capture the step in a HIP graph (planned rendering) and replay it")
+                    help="single, multiview: capture the step in a HIP graph (planned rendering) and replay it")
     ap.add_argument("--checkpoint-interval", type=int, default=64,
                     help="single: segments between the adjoint's stored states (K)")
     ap.add_argument("--tile-order", choices=["cost", "row-major"], default="cost",
                     help="single: launch order of the screen tiles (heaviest first, or row-major)")
+    ap.add_argument("--tile-launch", choices=["grid", "queue"], default="grid",
+                    help="single, multiview: one workgroup per tile, or persistent workgroups taking tiles from a "
+                         "device work queue")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="single: skip the multi-view and serving legs the default line also reports")
@@ -595,6 +618,7 @@ def main():
     mesh, r = make(w["pos"], True)
     r.set_checkpoint_interval(args.checkpoint_interval)
     r.set_tile_order(args.tile_order)
+    r.set_tile_launch(args.tile_launch)
     opt = FusedAdam([{"params": [mesh.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
                      {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True,
                     device_step=args.graph)
@@ -682,7 +706,7 @@ def main():
                    "segments": int(r.get_fragment_counts().to(torch.int64).sum()),
                    "gradient_atomics": stats["num_gradient_atomics"],
                    "hip_graph": bool(args.graph), "checkpoint_interval": args.checkpoint_interval,
-                   "tile_order": args.tile_order,
+                   "tile_order": args.tile_order, "tile_launch": args.tile_launch,
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
         "roofline": {"bound": "hbm", "kernel": f"k_render_{dominant}", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

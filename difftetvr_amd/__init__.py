@@ -526,6 +526,15 @@ class Renderer:
             raise RuntimeError("set_segments_per_pass: 64, 128 or 256 expected.")
         self._segments_per_pass = int(segments)
 
+    def set_tile_launch(self, launch):
+        """How the tile kernels are launched: 'grid' (default: one workgroup per 8x8 tile) or 'queue' (a
+        persistent grid of the workgroups the GPU holds at once, taking the tiles in launch order from a device
+        work queue; the adjoint follows its render).  Images are identical either way.  No reference counterpart."""
+        o = {"grid": _lib.DTV_TILE_LAUNCH_GRID, "queue": _lib.DTV_TILE_LAUNCH_QUEUE}.get(str(launch).lower())
+        if o is None:
+            raise RuntimeError(f"set_tile_launch: unknown launch '{launch}' ('grid' or 'queue').")
+        self._tile_launch = o
+
     def set_autotune(self, enabled=True, with_adjoint=True, repeats=3):
         """With autotuning on, the first render of a (mesh size class, viewport) times every tile size and
         segments-per-pass setting (``dtv_autotune``: forward, and adjoint with with_adjoint) and the fastest is
@@ -605,6 +614,7 @@ class Renderer:
         proj = self._renderer_type == RendererType.PROJECTION
         p.tile_size = 8 if proj else getattr(self, "_tile_size", 8)
         p.segments_per_pass = getattr(self, "_segments_per_pass", 128)
+        p.tile_launch = _lib.DTV_TILE_LAUNCH_GRID if proj else getattr(self, "_tile_launch", _lib.DTV_TILE_LAUNCH_GRID)
         return p
 
     def camera_uniforms(self):

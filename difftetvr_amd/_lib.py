@@ -16,6 +16,8 @@ DTV_SEGMENT_EUCLIDEAN = 0
 DTV_SEGMENT_LINEAR_DEPTH = 1
 DTV_RENDERER_TILE = 0
 DTV_RENDERER_PROJECTION = 1
+DTV_TILE_LAUNCH_GRID = 0
+DTV_TILE_LAUNCH_QUEUE = 1
 STAGES = ["count_pairs", "scan", "emit", "sort", "ranges", "forward", "adjoint"]
 
 c_int32 = ctypes.c_int32
@@ -38,7 +40,7 @@ class DtvRenderParams(ctypes.Structure):
     _fields_ = [("attenuation", c_float), ("clear_color", c_float * 4), ("use_early_ray_termination", c_int32),
                 ("early_ray_out_alpha", c_float), ("tile_size", c_int32), ("checkpoint_interval", c_int32),
                 ("tile_order", c_int32), ("segment_length", c_int32), ("renderer", c_int32),
-                ("segments_per_pass", c_int32)]
+                ("segments_per_pass", c_int32), ("tile_launch", c_int32)]
 
 
 class DtvViewStats(ctypes.Structure):

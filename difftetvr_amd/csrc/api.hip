@@ -216,6 +216,7 @@ struct dtv_view {
     DevBuf pt_info, pt_vert, pt_col, tri_grad;  // Projection renderer: per-tet triangles, per-corner gradients
     bool projection = false;        // the last render ran the Projection renderer
     bool sub_tiles = false;         // the last render binned tiles larger than 8x8
+    bool queued = false;            // the last render took its tiles from a work queue (the adjoint does too)
     bool cost_ordered = true;       // the last render launched its tiles heaviest first
     int64_t planned_cap = 0;  // > 0: planned (synchronisation-free) rendering with this record capacity
     // state of the last render
@@ -234,6 +235,7 @@ struct dtv_view {
     }
     int nitems() const { return (int)items.size(); }
     const uint4* sorted_recs() const { return recs.as<uint4>(); }
+    uint32_t* tile_queue() const { return reinterpret_cast<uint32_t*>(small.as<char>() + 384); }
     PixelState pixel_state() const {
         PixelState ps;
         ps.ray_color = ray_color.as<float4>();
@@ -311,8 +313,12 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     const int spp = params->segments_per_pass == 0 ? 128 : params->segments_per_pass;
     if (spp != 64 && spp != 128 && spp != 256)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segments_per_pass must be 64, 128 or 256");
+    if (params->tile_launch != DTV_TILE_LAUNCH_GRID && params->tile_launch != DTV_TILE_LAUNCH_QUEUE)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": tile_launch must be 0 (grid) or 1 (queue)");
     const bool proj = params->renderer == DTV_RENDERER_PROJECTION;
     if (proj && tile != 8) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": the Projection renderer bins 8x8 tiles");
+    if (proj && params->tile_launch != DTV_TILE_LAUNCH_GRID)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": the Projection renderer launches a workgroup per tile");
     dtv_render_params prm = *params;
     prm.tile_size = tile;
     params = &prm;
@@ -389,8 +395,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     DTV_HIP(view->counts.ensure(sizeof(uint32_t) * nslots));
     DTV_HIP(view->offsets.ensure(sizeof(uint32_t) * nslots));
     DTV_HIP(view->scan_status.ensure(sizeof(uint64_t) * (scan_partitions(nslots) + 1)));
-    // stats 0, scan counter 64, total 128, overflow 192, atomics 256, sort route 320 (48 B)
-    DTV_HIP(view->small.ensure(384));
+    // stats 0, scan counter 64, total 128, overflow 192, atomics 256, sort route 320 (48 B), tile queue 384
+    DTV_HIP(view->small.ensure(448));
     BinStats* dstats = view->small.as<BinStats>();
     uint32_t* scan_counter = reinterpret_cast<uint32_t*>(view->small.as<char>() + 64);
     uint64_t* scan_total = reinterpret_cast<uint64_t*>(view->small.as<char>() + 128);
@@ -510,7 +516,8 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
                           view->pixel_state(), s);
     else
         launch_render_forward(dI, n, subtiles, fwd_order, view->sorted_recs(), view->ranges.as<uint2>(),
-                              view->pixel_state(), kinds, s, spp, tile != 8);
+                              view->pixel_state(), kinds, s, spp, tile != 8,
+                              params->tile_launch == DTV_TILE_LAUNCH_QUEUE ? view->tile_queue() : nullptr);
     view->mark(DTV_STAGE_FORWARD, 1, s);
     DTV_HIP(hipGetLastError());
     view->colour_kinds = kinds;
@@ -521,6 +528,7 @@ int render_batch(dtv_view* view, const dtv_render_item* in, int n, const dtv_ren
     view->total_tiles = tiles;
     view->total_subtiles = subtiles;
     view->sub_tiles = tile != 8;
+    view->queued = params->tile_launch == DTV_TILE_LAUNCH_QUEUE;
     view->total_pixels = pixels;
     view->rendered = true;
     return DTV_OK;
@@ -598,7 +606,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     } else {
         launch_render_adjoint(view->dItems.as<BatchItem>(), view->dTargets.as<AdjointTarget>(), n,
                               view->total_subtiles, adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps,
-                              use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s, view->sub_tiles);
+                              use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s, view->sub_tiles,
+                              view->queued ? view->tile_queue() : nullptr);
     }
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
@@ -713,6 +722,8 @@ int dtv_render_noalloc(dtv_ctx* ctx, const dtv_mesh* mesh, const dtv_camera* cam
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": segment_length must be 0 (Euclidean) or 1 (linear depth)");
     if (params->renderer != DTV_RENDERER_TILE)
         return fail(DTV_ERR_INVALID_ARGUMENT, f + ": renders with the tile renderer only (renderer 0)");
+    if (params->tile_launch != DTV_TILE_LAUNCH_GRID)
+        return fail(DTV_ERR_INVALID_ARGUMENT, f + ": launches a workgroup per tile only (tile_launch 0)");
     if (!valid_camera(*camera)) return fail(DTV_ERR_INVALID_ARGUMENT, f + ": invalid viewport, fovy or clip planes");
     const dtv_mesh& m = *mesh;
     if (const int rc = check_mesh(m, f); rc != DTV_OK) return rc;

@@ -87,12 +87,15 @@ constexpr uint32_t kColoursPerVertex = 1, kColoursPerCell = 2;
 // round: fragments shaded per round of the forward (64, 128 or 256; other values take 128).  ntiles counts
 // 8x8 sub-tiles (one workgroup each).
 // sub: the batch bins tiles larger than 8x8 (the kernels then test coverage per 8x8 sub-tile).
+// queue: NULL launches one workgroup per tile; else a persistent grid (the workgroups the GPU holds at once)
+// takes the tiles in launch order from this device counter, which each launch zeroes first.
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
                            const uint4* recs, const uint2* ranges, PixelState ps, uint32_t colour_kinds,
-                           hipStream_t s, int round = 128, bool sub = false);
+                           hipStream_t s, int round = 128, bool sub = false, uint32_t* queue = nullptr);
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
-                           int mode, uint32_t colour_kinds, hipStream_t s, bool sub = false);
+                           int mode, uint32_t colour_kinds, hipStream_t s, bool sub = false,
+                           uint32_t* queue = nullptr);
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 

@@ -19,6 +19,9 @@
 // checkpoints (every K fragments, and before any fragment that takes the transmittance below 1/8 of the last
 // checkpoint's) to bound inversion drift.  Gradients of the 4 tet vertices are reduced per record in LDS before one global
 // atomic per value.
+#include <map>
+#include <mutex>
+
 #include "render_common.h"
 
 namespace dtv {
@@ -78,13 +81,10 @@ __device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
 // kRound: fragments shaded per round (64, 128 or 256: segments per pass).  kSub: binning tiles larger than 8x8
 // (the workgroup's 8x8 sub-tile tests its own coverage).
 template <bool kCell, int kRound, bool kSub>
-__global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
-                                                        const uint32_t* __restrict__ order,
-                                                        const uint4* __restrict__ recs,
-                                                        const uint2* __restrict__ ranges, PixelState ps_all) {
-    __shared__ RecordLds L;
-    __shared__ FragListLds<kRound> F;
-    const uint32_t gsub = global_tile_of_block(order);
+__device__ __forceinline__ void forward_tile(const BatchItem* __restrict__ items, int nitems,
+                                             const uint4* __restrict__ recs, const uint2* __restrict__ ranges,
+                                             const PixelState& ps_all, uint32_t gsub, RecordLds& L,
+                                             FragListLds<kRound>& F) {
     const BatchItem& B = items[item_of_sub(items, nitems, gsub)];
     if ((B.cell_col != nullptr) != kCell) return;
     const ViewUniforms& U = B.U;
@@ -211,6 +211,37 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
     if (lane == 63) ps.tile_frags[gsub] = tile_total;
 }
 
+// Next entry of the launch order for a persistent workgroup: one claim per tile from the launch's queue
+// counter (zeroed before the launch), uniform across the wave.  Every tile is claimed, none assigned by block
+// index: a workgroup the GPU cannot hold at first (the occupancy estimate is an upper bound) starts only when
+// another exits, and must not be holding a tile meanwhile.
+__device__ __forceinline__ uint32_t claim_tile(uint32_t* __restrict__ queue) {
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(queue, 1u);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)t, 0, 64));
+}
+
+// kQueue = false: one workgroup per tile of the launch order.  kQueue = true: as many workgroups as the GPU
+// holds at once, each claiming the next tile of the launch order from `queue` until all ntiles are taken.
+template <bool kCell, int kRound, bool kSub, bool kQueue>
+__global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restrict__ items, int nitems,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint4* __restrict__ recs,
+                                                        const uint2* __restrict__ ranges, PixelState ps_all,
+                                                        uint32_t* __restrict__ queue, uint32_t ntiles) {
+    __shared__ RecordLds L;
+    __shared__ FragListLds<kRound> F;
+    if constexpr (!kQueue) {
+        forward_tile<kCell, kRound, kSub>(items, nitems, recs, ranges, ps_all, global_tile_of_block(order), L, F);
+    } else {
+        for (uint32_t t = claim_tile(queue); t < ntiles; t = claim_tile(queue)) {
+            forward_tile<kCell, kRound, kSub>(items, nitems, recs, ranges, ps_all,
+                                              order != nullptr ? order[t] : t, L, F);
+            __syncthreads();
+        }
+    }
+}
+
 // kRef = false: exact derivative of the forward.  kRef = true: the reference's adjoint as written
 // (AdjointIntersectRasterization.glsl:111-362): per pixel from the blended output image, back to front, the
 // 'over' operator is inverted per sub-segment 3, 2, 1 (sub-segment 0 is skipped) with no checkpoints, and
@@ -218,18 +249,11 @@ __global__ __launch_bounds__(64) void k_render_forward(const BatchItem* __restri
 // kCell: the items with per-cell colours (exact adjoint only), kSub: binning tiles larger than 8x8, as
 // k_render_forward.
 template <bool kRef, bool kCell, bool kSub>
-// Waves per SIMD: 3 for the exact adjoint (168 VGPRs of the 512 per lane, 13 KB of LDS), 2 for the reference
-// one, whose sub-segment arrays need more of both.
-__global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const BatchItem* __restrict__ items,
-                                                        const AdjointTarget* __restrict__ targets, int nitems,
-                                                        const uint32_t* __restrict__ order,
-                                                        const uint4* __restrict__ recs,
-                                                        const uint2* __restrict__ ranges, PixelState ps_all,
-                                                        int use_abs) {
-    __shared__ RecordLds L;
-    __shared__ AdjFragLds A;
-    __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
-    const uint32_t gsub = global_tile_of_block(order);
+__device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items,
+                                             const AdjointTarget* __restrict__ targets, int nitems,
+                                             const uint4* __restrict__ recs, const uint2* __restrict__ ranges,
+                                             const PixelState& ps_all, int use_abs, uint32_t gsub, RecordLds& L,
+                                             AdjFragLds& A, float4 (&s_seg)[kRef ? 3 : 1][kRoundAdj]) {
     const int item = item_of_sub(items, nitems, gsub);
     const BatchItem& B = items[item];
     if ((B.cell_col != nullptr) != kCell) return;
@@ -626,6 +650,31 @@ __global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const Batch
     }
 }
 
+// Waves per SIMD: 3 for the exact adjoint (168 VGPRs of the 512 per lane, 13 KB of LDS), 2 for the reference
+// one, whose sub-segment arrays need more of both.  kQueue as in k_render_forward.
+template <bool kRef, bool kCell, bool kSub, bool kQueue>
+__global__ __launch_bounds__(64, kRef ? 2 : 3) void k_render_adjoint(const BatchItem* __restrict__ items,
+                                                        const AdjointTarget* __restrict__ targets, int nitems,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint4* __restrict__ recs,
+                                                        const uint2* __restrict__ ranges, PixelState ps_all,
+                                                        int use_abs, uint32_t* __restrict__ queue,
+                                                        uint32_t ntiles) {
+    __shared__ RecordLds L;
+    __shared__ AdjFragLds A;
+    __shared__ float4 s_seg[kRef ? 3 : 1][kRoundAdj];  // reference mode: sub-segment colours, then their adjoints
+    if constexpr (!kQueue) {
+        adjoint_tile<kRef, kCell, kSub>(items, targets, nitems, recs, ranges, ps_all, use_abs,
+                                        global_tile_of_block(order), L, A, s_seg);
+    } else {
+        for (uint32_t t = claim_tile(queue); t < ntiles; t = claim_tile(queue)) {
+            adjoint_tile<kRef, kCell, kSub>(items, targets, nitems, recs, ranges, ps_all, use_abs,
+                                            order != nullptr ? order[t] : t, L, A, s_seg);
+            __syncthreads();
+        }
+    }
+}
+
 // Ordered fragment export (tet*4 + face per covered record), up to max_per_pixel per pixel.
 __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __restrict__ items, int nitems,
                                                           const uint4* __restrict__ recs,
@@ -680,65 +729,113 @@ __global__ __launch_bounds__(64) void k_export_fragments(const BatchItem* __rest
         for (; k < max_per_pixel; k++) out_ids[(int64_t)pix * max_per_pixel + k] = -1;
 }
 
+// Workgroups of `kernel` the GPU holds at once (the size of a persistent launch), computed once per kernel and
+// device.
+static unsigned resident_groups(const void* kernel) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, unsigned> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kernel, dev});
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const unsigned n = (unsigned)(per_cu > 0 ? per_cu : 1) * (unsigned)(cus > 0 ? cus : 1);
+    cache[{kernel, dev}] = n;
+    return n;
+}
+
+// One launch of a tile kernel: a workgroup per tile, or (queue != NULL) a persistent grid whose workgroups claim
+// tiles from the queue counter, zeroed here first (a fill kernel, so that the launch can be graph-captured).
+template <typename Kernel, typename... Args>
+static void launch_tiles(Kernel grid_kernel, Kernel queue_kernel, int64_t ntiles, uint32_t* queue, hipStream_t s,
+                         Args... args) {
+    if (queue == nullptr) {
+        hipLaunchKernelGGL(grid_kernel, dim3((unsigned)ntiles), dim3(64), 0, s, args..., queue, (uint32_t)ntiles);
+        return;
+    }
+    unsigned grid = resident_groups(reinterpret_cast<const void*>(queue_kernel));
+    if (grid == 0 || grid > (uint64_t)ntiles) grid = (unsigned)ntiles;
+    fill_u32(queue, 0u, 1, s);
+    hipLaunchKernelGGL(queue_kernel, dim3(grid), dim3(64), 0, s, args..., queue, (uint32_t)ntiles);
+}
+
+template <bool kCell, bool kSub, int kRound>
+static void launch_forward_round(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
+                                 const uint4* recs, const uint2* ranges, PixelState ps, uint32_t* queue,
+                                 hipStream_t s) {
+    launch_tiles(k_render_forward<kCell, kRound, kSub, false>, k_render_forward<kCell, kRound, kSub, true>, ntiles,
+                 queue, s, items, nitems, order, recs, ranges, ps);
+}
+
 template <bool kCell, bool kSub>
 static void launch_forward_kind(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
-                                const uint4* recs, const uint2* ranges, PixelState ps, int round, hipStream_t s) {
+                                const uint4* recs, const uint2* ranges, PixelState ps, int round, uint32_t* queue,
+                                hipStream_t s) {
     if (round == 64)
-        hipLaunchKernelGGL((k_render_forward<kCell, 64, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                           nitems, order, recs, ranges, ps);
+        launch_forward_round<kCell, kSub, 64>(items, nitems, ntiles, order, recs, ranges, ps, queue, s);
     else if (round == 256)
-        hipLaunchKernelGGL((k_render_forward<kCell, 256, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                           nitems, order, recs, ranges, ps);
+        launch_forward_round<kCell, kSub, 256>(items, nitems, ntiles, order, recs, ranges, ps, queue, s);
     else
-        hipLaunchKernelGGL((k_render_forward<kCell, kRoundDefault, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s,
-                           items, nitems, order, recs, ranges, ps);
+        launch_forward_round<kCell, kSub, kRoundDefault>(items, nitems, ntiles, order, recs, ranges, ps, queue, s);
 }
 
 void launch_render_forward(const BatchItem* items, int nitems, int64_t ntiles, const uint32_t* order,
                            const uint4* recs, const uint2* ranges, PixelState ps, uint32_t colour_kinds,
-                           hipStream_t s, int round, bool sub) {
+                           hipStream_t s, int round, bool sub, uint32_t* queue) {
     if (ntiles <= 0) return;
     if (colour_kinds & kColoursPerVertex) {
         if (sub)
-            launch_forward_kind<false, true>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+            launch_forward_kind<false, true>(items, nitems, ntiles, order, recs, ranges, ps, round, queue, s);
         else
-            launch_forward_kind<false, false>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+            launch_forward_kind<false, false>(items, nitems, ntiles, order, recs, ranges, ps, round, queue, s);
     }
     if (colour_kinds & kColoursPerCell) {
         if (sub)
-            launch_forward_kind<true, true>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+            launch_forward_kind<true, true>(items, nitems, ntiles, order, recs, ranges, ps, round, queue, s);
         else
-            launch_forward_kind<true, false>(items, nitems, ntiles, order, recs, ranges, ps, round, s);
+            launch_forward_kind<true, false>(items, nitems, ntiles, order, recs, ranges, ps, round, queue, s);
     }
+}
+
+template <bool kRef, bool kCell, bool kSub>
+static void launch_adjoint_kind(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
+                                const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps,
+                                int use_abs, uint32_t* queue, hipStream_t s) {
+    launch_tiles(k_render_adjoint<kRef, kCell, kSub, false>, k_render_adjoint<kRef, kCell, kSub, true>, ntiles,
+                 queue, s, items, targets, nitems, order, recs, ranges, ps, use_abs);
 }
 
 template <bool kSub>
 static void launch_adjoint_sub(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                                const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps,
-                               int use_abs, int mode, uint32_t colour_kinds, hipStream_t s) {
+                               int use_abs, int mode, uint32_t colour_kinds, uint32_t* queue, hipStream_t s) {
     if (colour_kinds & kColoursPerVertex) {
         if (mode == 1)
-            hipLaunchKernelGGL((k_render_adjoint<true, false, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                               targets, nitems, order, recs, ranges, ps, use_abs);
+            launch_adjoint_kind<true, false, kSub>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs,
+                                                   queue, s);
         else
-            hipLaunchKernelGGL((k_render_adjoint<false, false, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                               targets, nitems, order, recs, ranges, ps, use_abs);
+            launch_adjoint_kind<false, false, kSub>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs,
+                                                    queue, s);
     }
     if (colour_kinds & kColoursPerCell)  // exact adjoint only (the caller rejects mode 1 for such items)
-        hipLaunchKernelGGL((k_render_adjoint<false, true, kSub>), dim3((unsigned)ntiles), dim3(64), 0, s, items,
-                           targets, nitems, order, recs, ranges, ps, use_abs);
+        launch_adjoint_kind<false, true, kSub>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs,
+                                               queue, s);
 }
 
 void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
-                           int mode, uint32_t colour_kinds, hipStream_t s, bool sub) {
+                           int mode, uint32_t colour_kinds, hipStream_t s, bool sub, uint32_t* queue) {
     if (ntiles <= 0) return;
     if (sub)
         launch_adjoint_sub<true>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs, mode,
-                                 colour_kinds, s);
+                                 colour_kinds, queue, s);
     else
         launch_adjoint_sub<false>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs, mode,
-                                  colour_kinds, s);
+                                  colour_kinds, queue, s);
 }
 
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,

@@ -76,7 +76,14 @@ typedef struct dtv_render_params {
     int32_t renderer;                  /* DTV_RENDERER_TILE (0) or DTV_RENDERER_PROJECTION (1) */
     int32_t segments_per_pass;         /* ray segments the forward shades per round: 64, 128 (0 = default) or
                                           256 (no reference counterpart) */
+    int32_t tile_launch;               /* DTV_TILE_LAUNCH_GRID (0, default): one workgroup per 8x8 tile;
+                                          DTV_TILE_LAUNCH_QUEUE (1): a persistent grid of the workgroups the GPU
+                                          holds at once takes the tiles in launch order from a device work queue
+                                          (forward and adjoint; no reference counterpart) */
 } dtv_render_params;
+
+#define DTV_TILE_LAUNCH_GRID 0
+#define DTV_TILE_LAUNCH_QUEUE 1
 
 /* Which renderer dtv_render / dtv_render_batch runs.  TILE: per-pixel ray-tet segments of the INTERSECTION and
  * PPLL renderers (segment_length chooses the segment length).  PROJECTION: the Projection renderer's projected

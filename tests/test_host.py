@@ -160,6 +160,7 @@ def test_render_params_layout_matches_header():
     names = re.findall(r"^\s+(?:float|int32_t)\s+(\w+)(?:\[\d+\])?;", body, flags=re.M)
     assert names == [f[0] for f in _lib.DtvRenderParams._fields_]
     assert re.search(r"#define DTV_RENDERER_PROJECTION 1\b", txt) and _lib.DTV_RENDERER_PROJECTION == 1
+    assert re.search(r"#define DTV_TILE_LAUNCH_QUEUE 1\b", txt) and _lib.DTV_TILE_LAUNCH_QUEUE == 1
 
 
 def test_unlink_tets():
