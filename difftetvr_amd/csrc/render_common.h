@@ -304,33 +304,36 @@ __device__ __forceinline__ bool ref_face_contains(const RecordLds& L, int j, int
     return true;
 }
 
-// ref_face_contains at P = ro + t rd, and when P is inside, its barycentric coordinates from the same edge cross
-// products: u = |(p2 - p1) x (P - p1)| / area (the sub-triangle opposite p0), v = |(p0 - p2) x (P - p2)| / area
-// (opposite p1), area = max(|N|, 1e-5) with N = (p1 - p0) x (p2 - p0): the unsigned areas of
-// BarycentricInterpolation.glsl:29-38, whose triangle (P, p0, p2) this spans from p2 instead of p0.
-__device__ __forceinline__ bool ref_face_contains_bary(const RecordLds& L, int j, int g, float3_ ro, float3_ rd,
-                                                       float t, float& u, float& v) {
+// Signed barycentrics of P = ro + t rd in face g: the shares of the sub-triangles opposite p0 and p1,
+// u = dot(N, (p2 - p1) x (P - p1)) / |N|^2 and v = dot(N, (p0 - p2) x (P - p2)) / |N|^2 with N = (p1 - p0) x
+// (p2 - p0) as staged.  Inside the triangle they equal the unsigned area ratios of
+// BarycentricInterpolation.glsl:29-38 (|(p2 - p1) x (P - p1)| / |N| and |(p0 - p2) x (P - p2)| / |N|), and
+// 1 - u - v is the share opposite p2.  Returns false for faces whose |N| is below the reference's 1e-5 area
+// floor (the caller then takes the reference procedure).
+__device__ __forceinline__ bool face_bary(const RecordLds& L, int j, int g, float3_ ro, float3_ rd, float t, float& u,
+                                          float& v) {
+#pragma clang fp contract(fast)
     const float3_ p0 = lds_p(L, face_vertex(g, 0), j), p1 = lds_p(L, face_vertex(g, 1), j),
                   p2 = lds_p(L, face_vertex(g, 2), j);
     const float3_ N = f3(L.N[3 * g][j], L.N[3 * g + 1][j], L.N[3 * g + 2][j]);
-    const float3_ P = add(ro, scale(rd, t));
-    const float3_ c0 = cross(sub(p1, p0), sub(P, p0));
-    const float3_ c1 = cross(sub(p2, p1), sub(P, p1));
-    const float3_ c2 = cross(sub(p0, p2), sub(P, p2));
-    if (dot(N, c0) < 0.0f || dot(N, c1) < 0.0f || dot(N, c2) < 0.0f) return false;
-    const float inv_area = fminf(fast_rsqrt(dot(N, N)), 1e5f);  // 1 / max(|N|, 1e-5)
-    u = fast_sqrt(dot(c1, c1)) * inv_area;
-    v = fast_sqrt(dot(c2, c2)) * inv_area;
-    return true;
+    const float3_ P = caxpy(t, rd, ro);
+    const float3_ c1 = ccross(sub(p2, p1), sub(P, p1));
+    const float3_ c2 = ccross(sub(p0, p2), sub(P, p2));
+    const float nn = cdot(N, N);
+    const float inn = fast_rcp(nn);
+    u = cdot(N, c1) * inn;
+    v = cdot(N, c2) * inn;
+    return nn >= 1e-10f;
 }
 
 constexpr float kEdgeMargin = 1e-4f;
 
 // Entry/exit faces and distances with the semantics of intersectRayTet (RayIntersectionTests.glsl:78-106).
-// Fast path: the plane-slab test (last entering / first exiting plane) proposes the two faces, and the
-// reference inside test confirms both; otherwise the reference's four-face procedure decides (edges,
-// vertices and silhouettes, where float rounding can make the reference reject a face).
-// On the fast path, bary = {u, v} at the entry and exit points and have_bary = true (else false).
+// Fast path: the plane-slab test (last entering / first exiting plane) proposes the two faces, and it stands when
+// both hits lie at least kEdgeMargin (in barycentric coordinates) inside their faces: there the reference's
+// inside test passes for the two faces and for no other.  Otherwise the reference's four-face procedure decides
+// (edges, vertices and silhouettes, where float rounding can make the reference reject a face, and the camera
+// inside the tet).  On the fast path bary = {u, v} at the entry and exit points (face_bary) and have_bary = true.
 __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, float3_ rd, int& f0, int& f1, float& t0,
                                         float& t1, float bary[4], bool& have_bary) {
     const bool outward = (L.meta[j] >> 3) & 1u;
@@ -352,13 +355,8 @@ __device__ __forceinline__ bool ray_tet(const RecordLds& L, int j, float3_ ro, f
             if (tf[g] < tout) { tout = tf[g]; fo = g; }
         }
     }
-    // The reference procedure accepts every face whose inside test passes and takes the nearest and farthest:
-    // where the ray passes within rounding of an edge, a neighbouring face of the slab test's pick passes too and
-    // may be the one chosen.  The fast path is taken only when both hits lie at least kEdgeMargin (in barycentric
-    // coordinates) inside their faces, where no other face can pass; otherwise the reference procedure decides.
-    if (fi >= 0 && fo >= 0 && tout >= tin && tin >= 0.0f &&
-        ref_face_contains_bary(L, j, fi, ro, rd, tin, bary[0], bary[1]) &&
-        ref_face_contains_bary(L, j, fo, ro, rd, tout, bary[2], bary[3]) &&
+    if (fi >= 0 && fo >= 0 && tout >= tin && tin >= 0.0f && face_bary(L, j, fi, ro, rd, tin, bary[0], bary[1]) &&
+        face_bary(L, j, fo, ro, rd, tout, bary[2], bary[3]) &&
         fminf(fminf(fminf(bary[0], bary[1]), 1.0f - bary[0] - bary[1]),
               fminf(fminf(bary[2], bary[3]), 1.0f - bary[2] - bary[3])) >= kEdgeMargin) {
         f0 = fi;
