@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/difftetvr_amd.h"
@@ -214,6 +215,7 @@ struct dtv_view {
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_t, ckpt_n;
     DevBuf tile_frags, tile_order;  // per tile: forward's fragments; launch orders of the forward and adjoint
     DevBuf pt_info, pt_vert, pt_col, tri_grad;  // Projection renderer: per-tet triangles, per-corner gradients
+    DevBuf vacc;  // tile renderer adjoint: interleaved per-vertex gradient rows of the distinct targets
     bool projection = false;        // the last render ran the Projection renderer
     bool sub_tiles = false;         // the last render binned tiles larger than 8x8
     bool queued = false;            // the last render took its tiles from a work queue (the adjoint does too)
@@ -574,7 +576,49 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         DTV_HIP(hipHostMalloc((void**)&view->pinnedTargets, sizeof(AdjointTarget) * (size_t)n, hipHostMallocDefault));
         view->pinned_targets_cap = n;
     }
+    // Tile renderer: each distinct gradient target (the items of one mesh share theirs) accumulates in zeroed
+    // interleaved rows (8 floats per vertex), folded into its gradients after the adjoint.
+    struct Fold {
+        float* acc;
+        int64_t V;
+        float* grad_pos;
+        float* grad_col;
+    };
+    std::vector<Fold> folds;
     std::memcpy(view->pinnedTargets, tg, sizeof(AdjointTarget) * (size_t)n);
+    if (!view->projection) {
+        std::unordered_map<const float*, int> first;  // grad_pos -> first item
+        std::vector<int64_t> row0((size_t)n, -1);
+        int64_t rows = 0;
+        for (int i = 0; i < n; i++) {
+            if (!tg[i].dimage) continue;
+            const bool cell = view->items[(size_t)i].cell_col != nullptr;
+            auto it = first.find(tg[i].grad_pos);
+            if (it != first.end()) {
+                const int k = it->second;
+                if (view->item_V[(size_t)k] != view->item_V[(size_t)i] || tg[k].grad_col != tg[i].grad_col ||
+                    (view->items[(size_t)k].cell_col != nullptr) != cell)
+                    return fail(DTV_ERR_INVALID_ARGUMENT,
+                                f + ": items that share a gradient buffer must render the same mesh");
+                row0[(size_t)i] = row0[(size_t)k];
+                continue;
+            }
+            first.emplace(tg[i].grad_pos, i);
+            row0[(size_t)i] = rows;
+            folds.push_back({nullptr, view->item_V[(size_t)i], tg[i].grad_pos, cell ? nullptr : tg[i].grad_col});
+            rows += view->item_V[(size_t)i];
+        }
+        DTV_HIP(view->vacc.ensure(sizeof(float) * 8 * (size_t)(rows > 0 ? rows : 1)));
+        float* base = view->vacc.as<float>();
+        int64_t r = 0;
+        for (Fold& fo : folds) {
+            fo.acc = base + 8 * r;
+            r += fo.V;
+        }
+        for (int i = 0; i < n; i++)
+            view->pinnedTargets[i].acc = row0[(size_t)i] >= 0 ? base + 8 * row0[(size_t)i] : nullptr;
+        if (rows > 0) fill_u32(reinterpret_cast<uint32_t*>(base), 0u, 8 * rows, s);
+    }
     DTV_HIP(view->dTargets.ensure(sizeof(AdjointTarget) * (size_t)n));
     DTV_HIP(hipMemcpyAsync(view->dTargets.p, view->pinnedTargets, sizeof(AdjointTarget) * (size_t)n,
                            hipMemcpyHostToDevice, s));
@@ -608,6 +652,7 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
                               view->total_subtiles, adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps,
                               use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s, view->sub_tiles,
                               view->queued ? view->tile_queue() : nullptr);
+        for (const Fold& fo : folds) launch_fold_vertex_grads(fo.acc, fo.V, fo.grad_pos, fo.grad_col, s);
     }
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
@@ -677,7 +722,7 @@ int dtv_view_destroy(dtv_view* v) {
                       &v->vals, &v->keys2,
                       &v->vals2, &v->recs, &v->hist, &v->lookback, &v->ranges, &v->ray_color, &v->frag_count, &v->last_rec,
                       &v->num_ckpt, &v->ckpt_t, &v->ckpt_n, &v->tile_frags, &v->tile_order, &v->pt_info,
-                      &v->pt_vert, &v->pt_col, &v->tri_grad};
+                      &v->pt_vert, &v->pt_col, &v->tri_grad, &v->vacc};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < DTV_NUM_STAGES; i++)
         for (int k = 0; k < 2; k++)
@@ -918,7 +963,7 @@ int dtv_render_adjoint(dtv_view* view, const dtv_mesh* mesh, const float* dimage
     if (!view->rendered) return fail(DTV_ERR_NOT_RENDERED, "dtv_render_adjoint: view has not been rendered");
     if (view->nitems() != 1 || !same_mesh(view, 0, *mesh))
         return fail(DTV_ERR_INVALID_ARGUMENT, "dtv_render_adjoint: mesh differs from the rendered one");
-    AdjointTarget t;
+    AdjointTarget t{};
     t.dimage = dimage;
     t.grad_pos = grad_positions;
     t.grad_col = grad_colors;

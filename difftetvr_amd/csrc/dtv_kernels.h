@@ -96,6 +96,9 @@ void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets,
                            const uint32_t* order, const uint4* recs, const uint2* ranges, PixelState ps, int use_abs,
                            int mode, uint32_t colour_kinds, hipStream_t s, bool sub = false,
                            uint32_t* queue = nullptr);
+// Adds the tile renderer adjoint's interleaved accumulator rows acc [V][8] into grad_pos [V][3] and, unless
+// NULL, grad_col [V][4].
+void launch_fold_vertex_grads(const float* acc, int64_t V, float* grad_pos, float* grad_col, hipStream_t s);
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
                              const uint2* ranges, PixelState ps, int max_per_pixel, int32_t* out_ids, hipStream_t s);
 

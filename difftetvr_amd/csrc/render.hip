@@ -262,8 +262,9 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
     const float* __restrict__ col = kCell ? B.cell_col : B.col;
     const float4* __restrict__ dimage = reinterpret_cast<const float4*>(targets[item].dimage);
     if (dimage == nullptr) return;  // item not differentiated
-    float* __restrict__ grad_pos = targets[item].grad_pos;
-    float* __restrict__ grad_col = targets[item].grad_col;
+    // positions and vertex colours accumulate in the item's interleaved rows (8 floats per vertex: xyz, rgba)
+    float* __restrict__ vacc = targets[item].acc;
+    float* __restrict__ grad_col = targets[item].grad_col;  // per-cell colours: straight to the tet
     const PixelState ps = item_pixel_state(ps_all, B);
     const SubTile st = sub_tile(B, gsub);
     const uint32_t gtile = st.gtile;
@@ -611,32 +612,44 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                     for (int c = 0; c < 7; c++) atomicAdd(&tabf[8 * ent[k] + 1 + c], acc[7 * k + c]);
                 }
             __syncthreads();
-            float sum[2][7];
-            bool lead[2] = {false, false};
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                // a slot that found no entry (not possible with kTab > 128 ids) issues its own values
-                lead[k] = slot_live && (owner[k] || ent[k] < 0);
-#pragma unroll
-                for (int c = 0; c < 7; c++) sum[k][c] = owner[k] ? tabf[8 * ent[k] + 1 + c] : acc[7 * k + c];
-            }
+            // The entries created this batch are listed compactly; then eight lanes per entry add its seven sums
+            // to the vertex's 32-byte row of the interleaved accumulator, one wave instruction per eight vertices
+            // (eight memory sectors, where one value per lane and vertex would touch 64).
+            int32_t* list = reinterpret_cast<int32_t*>(&A.val[0]);  // free after the rounds: 256 words
+            const uint64_t own0 = ballot(owner[0]), own1 = ballot(owner[1]);
+            const uint32_t n0 = (uint32_t)popc64(own0), nown = n0 + (uint32_t)popc64(own1);
+            if (owner[0]) list[popc64(own0 & lanemask_lt())] = ent[0];
+            if (owner[1]) list[n0 + popc64(own1 & lanemask_lt())] = ent[1];
+            __syncthreads();
             // The next batch's vertex data is requested before the atomics, so staging it does not wait for them.
             // Only lanes 0..31 hold that batch's record words.
             if (bi > 0 && lane < kBatch) fetch_record_data<kCell>(pos, col, nxt, nd);
             uint32_t issued = 0;
+            {
+                const int c = lane & 7;
+                for (uint32_t b0 = 0; b0 < nown; b0 += 8) {
+                    const uint32_t li = b0 + (uint32_t)(lane >> 3);
+                    if (li < nown && c < 7) {
+                        const int e = list[li];
+                        const float val = tabf[8 * e + 1 + c];
+                        if (val != 0.f) {
+                            atomic_add_f32(&vacc[8 * (int64_t)tab[8 * e] + c], val);
+                            issued++;
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 2; k++) {
-                if (!lead[k]) continue;
+                // a slot that found no entry (not possible with kTab > 128 ids) issues its own values
+                if (!slot_live || ent[k] >= 0) continue;
                 const int64_t g = k == 0 ? v0 : v1;
 #pragma unroll
                 for (int c = 0; c < 7; c++) {
-                    const float val = sum[k][c];
+                    const float val = acc[7 * k + c];
                     if (val == 0.f) continue;
                     issued++;
-                    if (c < 3)
-                        atomic_add_f32(&grad_pos[3 * g + c], val);
-                    else
-                        atomic_add_f32(&grad_col[4 * g + (c - 3)], val);
+                    atomic_add_f32(&vacc[8 * g + c], val);
                 }
             }
             natom += issued;
@@ -836,6 +849,31 @@ void launch_render_adjoint(const BatchItem* items, const AdjointTarget* targets,
     else
         launch_adjoint_sub<false>(items, targets, nitems, ntiles, order, recs, ranges, ps, use_abs, mode,
                                   colour_kinds, queue, s);
+}
+
+// grad_pos[v] += acc[v][0..2], grad_col[v] += acc[v][3..6] (grad_col NULL: positions only).  One thread per
+// vertex, so plain read-modify-writes.
+__global__ __launch_bounds__(256) void k_fold_vertex_grads(const float4* __restrict__ acc, int64_t V,
+                                                           float* __restrict__ grad_pos,
+                                                           float* __restrict__ grad_col) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= V) return;
+    const float4 a = acc[2 * v], b = acc[2 * v + 1];
+    grad_pos[3 * v + 0] += a.x;
+    grad_pos[3 * v + 1] += a.y;
+    grad_pos[3 * v + 2] += a.z;
+    if (grad_col != nullptr) {
+        grad_col[4 * v + 0] += a.w;
+        grad_col[4 * v + 1] += b.x;
+        grad_col[4 * v + 2] += b.y;
+        grad_col[4 * v + 3] += b.z;
+    }
+}
+
+void launch_fold_vertex_grads(const float* acc, int64_t V, float* grad_pos, float* grad_col, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(k_fold_vertex_grads, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(acc), V, grad_pos, grad_col);
 }
 
 void launch_export_fragments(const BatchItem* items, int nitems, int64_t ntiles, const uint4* recs,
