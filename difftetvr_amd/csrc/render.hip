@@ -53,10 +53,35 @@ struct AdjFragLds {
     uint32_t start[kBatch];     // their first row
 };
 
-// Coverage masks -> dense fragment list.  Each lane's fragments are contiguous, ordered by record index
-// (ascending, or descending when `reverse`).  Returns the lane's first entry; `total` = all fragments.
-__device__ __forceinline__ uint32_t build_fragment_list(uint16_t* entry, uint32_t mask, bool reverse, int lane,
-                                                        uint32_t& total) {
+// Coverage masks -> dense fragment list ordered by (rank, pixel lane): rank r holds the r-th fragment (by record
+// index, ascending, or descending when `reverse`) of every pixel that has more than r, in lane order.  A round of
+// the list then holds about one fragment per pixel, so the per-pixel compositing (or its inversion) that walks
+// each pixel's fragments in order takes few serial steps per round.  rank_mask[r] = the lanes with more than r
+// fragments (at most 32 ranks).  Returns the number of fragments.
+__device__ __forceinline__ uint32_t build_fragment_list(uint16_t* entry, uint64_t* rank_mask, uint32_t mask,
+                                                        bool reverse, int lane) {
+    const uint64_t lt = lanemask_lt();
+    uint32_t m = mask;
+    uint64_t act = ballot(m != 0u);
+    uint32_t base = 0;
+    for (int r = 0; act != 0ull; r++) {
+        if (m != 0u) {
+            const int j = reverse ? 31 - __clz(m) : __ffs(m) - 1;
+            entry[base + (uint32_t)popc64(act & lt)] = (uint16_t)((lane << 5) | j);
+            m &= ~(1u << j);
+        }
+        if (lane == 0) rank_mask[r] = act;
+        base += (uint32_t)popc64(act);
+        act = ballot(m != 0u);
+    }
+    return base;
+}
+
+// Coverage masks -> dense fragment list with each lane's fragments contiguous, ordered by record index
+// (ascending, or descending when `reverse`): a round then holds few records, each with several pixels, as the
+// adjoint's per-record reduction of a round wants.  Returns the lane's first entry; `total` = all fragments.
+__device__ __forceinline__ uint32_t build_fragment_list_by_pixel(uint16_t* entry, uint32_t mask, bool reverse,
+                                                                 int lane, uint32_t& total) {
     const uint32_t nm = (uint32_t)__popc(mask);
     const uint32_t inc = wave_inclusive_scan(nm);
     total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
@@ -70,6 +95,22 @@ __device__ __forceinline__ uint32_t build_fragment_list(uint16_t* entry, uint32_
     }
     return off;
 }
+
+// A pixel lane's walk over its own fragments of a ranked list, round by round: the position of its next
+// fragment (rank `rank`), while that lies before `end`.
+struct RankCursor {
+    uint32_t rank = 0, base = 0;  // next rank; first position of that rank
+    __device__ __forceinline__ bool next(const uint64_t* rank_mask, uint32_t count, uint64_t lt, uint32_t end,
+                                         uint32_t& e) {
+        if (rank >= count) return false;
+        const uint64_t act = rank_mask[rank];
+        e = base + (uint32_t)popc64(act & lt);
+        if (e >= end) return false;
+        base += (uint32_t)popc64(act);
+        rank++;
+        return true;
+    }
+};
 
 // Ray direction of pixel lane pl from the lanes' own (register) directions; every lane must take part.
 __device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
@@ -95,6 +136,7 @@ __device__ __forceinline__ void forward_tile(const BatchItem* __restrict__ items
     const SubTile st = sub_tile(B, gsub);
     const uint32_t gtile = st.gtile;
     const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
     const int lx = lane & 7, ly = lane >> 3;
     const int x = st.x0 + lx, y = st.y0 + ly;
     // binning tiles larger than 8x8: coverage is tested here, from face triangles stashed in the fragment
@@ -139,9 +181,11 @@ __device__ __forceinline__ void forward_tile(const BatchItem* __restrict__ items
             __syncthreads();
         }
         const uint32_t mask = done ? 0u : lane_coverage(L, cnt, lane);
-        uint32_t total;
-        const uint32_t off = build_fragment_list(F.entry, mask, false, lane, total);
-        const uint32_t mine_end = off + (uint32_t)__popc(mask);
+        // the records' coverage masks are consumed: their slots hold the list's rank masks
+        uint64_t* rank_mask = reinterpret_cast<uint64_t*>(&L.mask[0][0]);
+        const uint32_t total = build_fragment_list(F.entry, rank_mask, mask, false, lane);
+        const uint32_t nmine = (uint32_t)__popc(mask);
+        RankCursor cur_e;
         __syncthreads();
         for (uint32_t r0 = 0; r0 < total; r0 += kRound) {
             for (int i = lane; i < kRound; i += 64) {
@@ -155,8 +199,8 @@ __device__ __forceinline__ void forward_tile(const BatchItem* __restrict__ items
                 F.val[i] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
             }
             __syncthreads();
-            const uint32_t e0 = max(off, r0), e1 = min(mine_end, r0 + kRound);
-            for (uint32_t e = e0; e < e1 && !done; e++) {
+            uint32_t e;
+            while (!done && cur_e.next(rank_mask, nmine, lt, r0 + kRound, e)) {
                 if (U.use_ert && (1.0f - C.w) > U.ert_alpha) {
                     done = true;
                     break;
@@ -359,7 +403,7 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
         const uint32_t pmask = nparts == 1 ? 0xFFFFFFFFu : (part == 0 ? 0xFFFF0000u : 0x0000FFFFu);
         const uint32_t mask = done ? 0u : (mask_all & pmask);
         uint32_t total;
-        const uint32_t off = build_fragment_list(A.entry, mask, true, lane, total);
+        const uint32_t off = build_fragment_list_by_pixel(A.entry, mask, true, lane, total);
         const uint32_t mine_end = off + (uint32_t)__popc(mask);
         __syncthreads();
         for (uint32_t r0 = 0; r0 < total; r0 += kRoundAdj) {
