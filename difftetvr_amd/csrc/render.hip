@@ -592,17 +592,24 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
         }  // parts
         // Flush.  Lanes j and j + 32 hold record j's tet vertices 0, 1 and 2, 3 (7 values each): 128 vertex
         // slots, slot s = v * 32 + j.  Depth-adjacent tets share most vertices (typically 6-7 slots per distinct
-        // vertex in a batch), so the slots of one vertex are summed in an LDS hash table keyed by vertex id
-        // (180 entries of {id, 7 sums}; a slot that finds no free entry within the table issues its own
-        // atomics), and only the slot that created the entry issues the global atomics: far fewer memory-side
-        // atomics.
+        // vertex in a batch), so the slots of one vertex are summed before the global atomics: each slot finds
+        // its vertex's entry in an LDS hash table keyed by vertex id (compare-and-swap; the first slot of a
+        // vertex creates the entry), stores its values in its own LDS row and links the row into the entry's
+        // list with one exchange; the entry's creator then sums the list.  (LDS float atomics, one per value and
+        // slot, cost about 2.5 cycles per lane.)
         {
             const int j = lane & 31, half = lane >> 5;
-            constexpr int kTab = (int)((sizeof(A.entry) + sizeof(A.w)) / 32);  // entries of 8 words
-            static_assert(kTab >= 160, "hash table too small for 128 vertex slots");
-            int32_t* tab = reinterpret_cast<int32_t*>(&A.entry[0]);  // entry and w are free after the rounds
-            float* tabf = reinterpret_cast<float*>(&A.entry[0]);
-            for (int e = lane; e < kTab; e += 64) tab[8 * e] = -1;
+            constexpr int kTab = 160;
+            // over entry and w, free after the rounds: keys[kTab], head[kTab], rows[128][8] (7 values, next slot)
+            int32_t* keys = reinterpret_cast<int32_t*>(&A.entry[0]);
+            int32_t* head = keys + kTab;
+            float* rows = reinterpret_cast<float*>(head + kTab);
+            static_assert(sizeof(int32_t) * 2 * kTab + sizeof(float) * 8 * 128 <= sizeof(A.entry) + sizeof(A.w),
+                          "flush tables exceed the fragment list space");
+            for (int e = lane; e < kTab; e += 64) {
+                keys[e] = -1;
+                head[e] = -1;
+            }
             __syncthreads();
             const int s0 = (2 * half) * 32 + j, s1 = s0 + 32;
             const int32_t* vids = &L.vid[0][0];
@@ -631,7 +638,7 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                 const int32_t v = k == 0 ? v0 : v1;
                 int e = (int)(((uint32_t)v * 2654435761u) >> 24) * kTab >> 8;  // multiplicative hash -> [0, kTab)
                 for (int probe = 0; probe < kTab; probe++) {
-                    const int32_t old = atomicCAS(&tab[8 * e], -1, v);
+                    const int32_t old = atomicCAS(&keys[e], -1, v);
                     if (old == -1 || old == v) {
                         ent[k] = e;
                         owner[k] = old == -1;
@@ -640,30 +647,36 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                     e = e + 1 == kTab ? 0 : e + 1;
                 }
             }
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                if (ent[k] < 0) continue;
+                const int sl = k == 0 ? s0 : s1;
+#pragma unroll
+                for (int c = 0; c < 7; c++) rows[8 * sl + c] = acc[7 * k + c];
+                rows[8 * sl + 7] = __int_as_float(atomicExch(&head[ent[k]], sl));
+            }
             __syncthreads();
-            // the entry's creator stores its values; the other slots of that vertex then add theirs
+            // each entry's creator sums its list into its own row
 #pragma unroll
-            for (int k = 0; k < 2; k++)
-                if (owner[k]) {
+            for (int k = 0; k < 2; k++) {
+                if (!owner[k]) continue;
+                float sum[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                for (int sl = head[ent[k]]; sl >= 0;) {
 #pragma unroll
-                    for (int c = 0; c < 7; c++) tabf[8 * ent[k] + 1 + c] = acc[7 * k + c];
+                    for (int c = 0; c < 7; c++) sum[c] += rows[8 * sl + c];
+                    sl = __float_as_int(rows[8 * sl + 7]);
                 }
-            __syncthreads();
+                const int own = k == 0 ? s0 : s1;
 #pragma unroll
-            for (int k = 0; k < 2; k++)
-                if (ent[k] >= 0 && !owner[k]) {
-#pragma unroll
-                    for (int c = 0; c < 7; c++) atomicAdd(&tabf[8 * ent[k] + 1 + c], acc[7 * k + c]);
-                }
-            __syncthreads();
-            // The entries created this batch are listed compactly; then eight lanes per entry add its seven sums
-            // to the vertex's 32-byte row of the interleaved accumulator, one wave instruction per eight vertices
-            // (eight memory sectors, where one value per lane and vertex would touch 64).
+                for (int c = 0; c < 7; c++) rows[8 * own + c] = sum[c];
+            }
+            // The creators' rows are listed compactly; then eight lanes per row add its seven sums to the
+            // vertex's 32-byte row of the interleaved accumulator, one wave instruction per eight vertices.
             int32_t* list = reinterpret_cast<int32_t*>(&A.val[0]);  // free after the rounds: 256 words
             const uint64_t own0 = ballot(owner[0]), own1 = ballot(owner[1]);
             const uint32_t n0 = (uint32_t)popc64(own0), nown = n0 + (uint32_t)popc64(own1);
-            if (owner[0]) list[popc64(own0 & lanemask_lt())] = ent[0];
-            if (owner[1]) list[n0 + popc64(own1 & lanemask_lt())] = ent[1];
+            if (owner[0]) list[popc64(own0 & lanemask_lt())] = s0;
+            if (owner[1]) list[n0 + popc64(own1 & lanemask_lt())] = s1;
             __syncthreads();
             // The next batch's vertex data is requested before the atomics, so staging it does not wait for them.
             // Only lanes 0..31 hold that batch's record words.
@@ -674,10 +687,10 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                 for (uint32_t b0 = 0; b0 < nown; b0 += 8) {
                     const uint32_t li = b0 + (uint32_t)(lane >> 3);
                     if (li < nown && c < 7) {
-                        const int e = list[li];
-                        const float val = tabf[8 * e + 1 + c];
+                        const int sl = list[li];
+                        const float val = rows[8 * sl + c];
                         if (val != 0.f) {
-                            atomic_add_f32(&vacc[8 * (int64_t)tab[8 * e] + c], val);
+                            atomic_add_f32(&vacc[8 * (int64_t)vids[sl] + c], val);
                             issued++;
                         }
                     }
