@@ -47,10 +47,8 @@ struct FragListLds {
 constexpr int kAdjFrags = 1024;
 struct AdjFragLds {
     uint16_t entry[kAdjFrags];  // (pixel lane << 5) | record index, one part of a batch
-    float w[32][29];            // gradient rows of half a round (29: conflict-free column access)
+    float w[32][29];            // gradient rows of half a round, one per lane (29: conflict-free column access)
     float4 val[kRoundAdj];      // F, then dL/dF (.w: d/d(alpha), alpha = 1 - F.w); w = kMissMarker on a miss
-    uint32_t cnt[kBatch];       // half-round counting sort by record: fragments per record, then
-    uint32_t start[kBatch];     // their first row
 };
 
 // Coverage masks -> dense fragment list ordered by (rank, pixel lane): rank r holds the r-th fragment (by record
@@ -482,8 +480,8 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
             }
             if (n == 0) done = true;
             __syncthreads();
-            // C: gradients of the fragments (one per lane).  Each fragment's 28 values are stored, grouped by
-            // record, into A.w (half a round at a time); the owner lanes of each record then add its rows.
+            // C: gradients of the fragments (one per lane).  Each fragment's 28 values are stored into A.w (half
+            // a round at a time); the owner lanes of each record then add its rows.
             int key = kBatch;  // record index of this lane's fragment, kBatch = none
             float g[28];
 #pragma unroll
@@ -545,45 +543,46 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                     key = j;
                 }
             }
-            // Per half round (the fragments of lanes 0-31, then 32-63): counting sort by record in LDS, row =
-            // (fragments of lower records) + (the fragment's rank among its record's, from an LDS atomic); the
-            // owner lanes of record j (j and j + 32) get its segment [seg0, seg0 + segn) and add its rows.
+            // Per half round (the fragments of lanes 0-31, then 32-63): each fragment stores its 28 values in
+            // the row of its lane; the owner lanes of record j (j and j + 32) find the record's rows from a mask of
+            // the lanes whose key equals j, formed from five ballots of the key bits, and add them.
 #pragma unroll 1
             for (int h = 0; h < 2; h++) {
-                const int keyh = (lane >> 5) == h ? key : kBatch;
-                if (lane < kBatch) A.cnt[lane] = 0u;
-                __syncthreads();
-                uint32_t rank = 0;
-                if (keyh < kBatch) rank = atomicAdd(&A.cnt[keyh], 1u);
-                __syncthreads();
-                const uint32_t segn = A.cnt[lane & 31];
-                const uint32_t incl = wave_inclusive_scan(lane < kBatch ? segn : 0u);
-                if (lane < kBatch) A.start[lane] = incl - segn;
-                __syncthreads();
-                const uint32_t seg0 = A.start[lane & 31];
-                const uint32_t row = keyh < kBatch ? A.start[keyh] + rank : 0u;
-                if (keyh < kBatch) {
+                const bool mine = (lane >> 5) == h && key < kBatch;
+                const uint32_t jj = (uint32_t)(lane & 31);
+                uint32_t eqj = (uint32_t)(ballot(mine) >> (32 * h));
 #pragma unroll
-                    for (int q = 0; q < 28; q++) A.w[row][q] = g[q];
+                for (int b = 0; b < 5; b++) {
+                    const uint32_t ones = (uint32_t)(ballot(mine && ((key >> b) & 1)) >> (32 * h));
+                    eqj &= ((jj >> b) & 1u) ? ones : ~ones;
+                }
+                if (mine) {
+#pragma unroll
+                    for (int q = 0; q < 28; q++) A.w[lane & 31][q] = g[q];
                 }
                 __syncthreads();
                 {
                     // two rows per step: 28 independent LDS reads in flight instead of 14
                     const int half = lane >> 5;
-                    uint32_t rr = 0;
-                    for (; rr + 1 < segn; rr += 2) {
-                        float a[14], b[14];
+                    uint32_t m = eqj;
+                    while (m) {
+                        const int r1 = __ffs(m) - 1;
+                        m &= m - 1u;
+                        if (m) {
+                            const int r2 = __ffs(m) - 1;
+                            m &= m - 1u;
+                            float a[14], b[14];
 #pragma unroll
-                        for (int q = 0; q < 14; q++) {
-                            a[q] = A.w[seg0 + rr][half * 14 + q];
-                            b[q] = A.w[seg0 + rr + 1][half * 14 + q];
+                            for (int q = 0; q < 14; q++) {
+                                a[q] = A.w[r1][half * 14 + q];
+                                b[q] = A.w[r2][half * 14 + q];
+                            }
+#pragma unroll
+                            for (int q = 0; q < 14; q++) acc[q] += a[q] + b[q];
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < 14; q++) acc[q] += A.w[r1][half * 14 + q];
                         }
-#pragma unroll
-                        for (int q = 0; q < 14; q++) acc[q] += a[q] + b[q];
-                    }
-                    if (rr < segn) {
-#pragma unroll
-                        for (int q = 0; q < 14; q++) acc[q] += A.w[seg0 + rr][half * 14 + q];
                     }
                 }
                 __syncthreads();
