@@ -455,27 +455,23 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                     }
                 }
                 const float4 f = A.val[e - r0];
-                if (f.w != kMissMarker) {
-                    // the fragment's input transmittance: a checkpoint, 1 before the first fragment, or the
-                    // output divided by the fragment's transmittance (a corrected quotient: the rounding
-                    // compounds along the chain back to the next checkpoint).  Below kTinyT the forward's rule
-                    // leaves no checkpoint only where the input was below kTinyT too: that fragment's
-                    // contribution (at most kTinyT |dL/dI|) is dropped.
-                    float Tin;
-                    if (jck >= 0 && ck_n == n - 1) {
-                        Tin = ck_t;
-                    } else if (n == 1) {
-                        Tin = 1.0f;
-                    } else {
-                        Tin = C.w < kTinyT ? 0.0f : div_nr(C.w, f.w);
-                    }
-                    A.val[e - r0] = make_float4(Tin * dC[0], Tin * dC[1], Tin * dC[2], Tin * dC[3]);
-                    {
+                // the fragment's input transmittance: a checkpoint, 1 before the first fragment, or the output
+                // divided by the fragment's transmittance (a corrected quotient: the rounding compounds along the
+                // chain back to the next checkpoint).  Below kTinyT the forward's rule leaves no checkpoint only
+                // where the input was below kTinyT too: that fragment's contribution (at most kTinyT |dL/dI|) is
+                // dropped.  Formed without branches; a missed fragment (f.w = kMissMarker) keeps the state, and
+                // its slot's value is not read.
+                const bool hit = f.w != kMissMarker;
+                float Tin = C.w < kTinyT ? 0.0f : div_nr(C.w, f.w);
+                Tin = n == 1 ? 1.0f : Tin;
+                Tin = (jck >= 0 && ck_n == n - 1) ? ck_t : Tin;
+                A.val[e - r0] = make_float4(Tin * dC[0], Tin * dC[1], Tin * dC[2], Tin * dC[3]);
+                {
 #pragma clang fp contract(fast)
-                        dC[3] = dC[3] * f.w - (dC[0] * f.x + dC[1] * f.y + dC[2] * f.z);
-                    }
-                    C.w = Tin;
+                    const float d3 = dC[3] * f.w - (dC[0] * f.x + dC[1] * f.y + dC[2] * f.z);
+                    dC[3] = hit ? d3 : dC[3];
                 }
+                C.w = hit ? Tin : C.w;
                 n--;
             }
             if (n == 0) done = true;
