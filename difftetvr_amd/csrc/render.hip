@@ -204,26 +204,25 @@ __device__ __forceinline__ void forward_tile(const BatchItem* __restrict__ items
                     break;
                 }
                 const float4 f = F.val[e - r0];
-                if (f.w != kMissMarker) {
-                    // C.w is the transmittance T, the product of the fragments' transmittances (relative
-                    // precision at any depth), so the adjoint recovers each fragment's input as T_out / T_f.
-                    // Only an underflow (T_out below kTinyT) cannot be inverted: a checkpoint goes before the
-                    // fragment that causes it.  The state before the first fragment (T = 1) needs no slot.
-                    const float Tout = C.w * f.w;
-                    if (C.w >= kTinyT && Tout < kTinyT && nlast != n && (int)nck < U.max_ckpt) {
-                        ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
-                        ps.ckpt_n[(int64_t)nck * P + pix] = n;
-                        nck++;
-                        nlast = n;
-                    }
-                    {
-#pragma clang fp contract(fast)
-                        C.x = C.x + C.w * f.x;
-                        C.y = C.y + C.w * f.y;
-                        C.z = C.z + C.w * f.z;
-                    }
-                    C.w = Tout;
+                // C.w is the transmittance T, the product of the fragments' transmittances (relative precision at
+                // any depth), so the adjoint recovers each fragment's input as T_out / T_f.  Only an underflow
+                // (T_out below kTinyT) cannot be inverted: a checkpoint goes before the fragment that causes it.
+                // The state before the first fragment (T = 1) needs no slot.  A missed fragment's value is
+                // (0, 0, 0, kMissMarker): with transmittance 1 it leaves the state as it is, without a branch.
+                const float Tout = C.w * (f.w != kMissMarker ? f.w : 1.0f);
+                if (C.w >= kTinyT && Tout < kTinyT && nlast != n && (int)nck < U.max_ckpt) {
+                    ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
+                    ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                    nck++;
+                    nlast = n;
                 }
+                {
+#pragma clang fp contract(fast)
+                    C.x = C.x + C.w * f.x;
+                    C.y = C.y + C.w * f.y;
+                    C.z = C.z + C.w * f.z;
+                }
+                C.w = Tout;
                 n++;
                 last = base + (F.entry[e] & 31) + 1;
                 // periodic checkpoints bound the rounding the adjoint's chain of divisions accumulates; the last
