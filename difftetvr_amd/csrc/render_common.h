@@ -533,21 +533,19 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     float a_s[4], A_s[4], ia_s[4];
     // as in eval_fragment: not kept live since phase A
     sub_segments_ex(E.col0, E.col1, tSeg, att, seg, pre, Fr, a_s, A_s, ia_s);
-    const float dcx[3] = {E.col1.x - E.col0.x, E.col1.y - E.col0.y, E.col1.z - E.col0.z};
-    const float cb0[3] = {E.col0.x, E.col0.y, E.col0.z};
-    // c0 - c1 of every sub-segment: -(col1 - col0) / 4
-    const float diff[3] = {-0.25f * dcx[0], -0.25f * dcx[1], -0.25f * dcx[2]};
+    // dR.rgb is the same for every sub-segment: sum over RGB of dR c0 (sub-segment s starts at col0 + s/4
+    // (col1 - col0)) = p0 + (s/4) p1, and of dR (c0 - c1) = -p1 / 4
+    const float p0 = (dR[0] * E.col0.x + dR[1] * E.col0.y) + dR[2] * E.col0.z;
+    const float p1 =
+        (dR[0] * (E.col1.x - E.col0.x) + dR[1] * (E.col1.y - E.col0.y)) + dR[2] * (E.col1.z - E.col0.z);
+    const float sdf = -0.25f * p1;
+    float w0s = 0.f, w1s = 0.f;  // the RGB colour weights summed over the sub-segments
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
         const float4 sg = seg[s], st = pre[s];
         const float omA = st.w;  // transmittance before s: dL/d(sub-segment s) = omA dR
         const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
-        float sc0 = 0.f, sdf = 0.f;  // sum over RGB of dR c0 and of dR (c0 - c1)
-#pragma unroll
-        for (int ch = 0; ch < 3; ch++) {
-            sc0 += dR[ch] * (cb0[ch] + fb * dcx[ch]);
-            sdf += dR[ch] * diff[ch];
-        }
+        const float sc0 = p0 + fb * p1;
         const float dsw = omA * dR[3];
         dR[3] = dR[3] * A_s[s] - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
         const float a = a_s[s];
@@ -567,14 +565,16 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
             w0 = omA * ((1.0f - fb) * kc0 - (1.0f - fe) * K);
             w1 = omA * (fb * kc0 - fe * K);
         }
-#pragma unroll
-        for (int ch = 0; ch < 3; ch++) {
-            dcol0[ch] += w0 * dR[ch];
-            dcol1[ch] += w1 * dR[ch];
-        }
+        w0s += w0;
+        w1s += w1;
         dcol0[3] += (1.0f - fm) * att * da;
         dcol1[3] += fm * att * da;
         dLdt += 0.25f * dts;
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        dcol0[ch] = w0s * dR[ch];
+        dcol1[ch] = w1s * dR[ch];
     }
     dLdt *= tscale;  // d tSeg / d t1 = tscale / 4
     face_side_q(L, j, E.f0, rd, dcol0, -dLdt, Q0);
