@@ -413,7 +413,7 @@ __device__ __forceinline__ void adjoint_tile(const BatchItem* __restrict__ items
                 const int pl = ent >> 5, j = ent & 31;
                 const float3_ rdp = lane_ray(my_rd, pl);
                 if (e < total) {
-                    eval_fragment<kCell, true>(L, j, ro, rdp, att, seg_scale(U, rdp), E);
+                    eval_fragment<kCell, true, kRef>(L, j, ro, rdp, att, seg_scale(U, rdp), E);
                     A.val[lane] = E.hit ? E.F : make_float4(0.f, 0.f, 0.f, kMissMarker);
                     if constexpr (kRef) {
 #pragma unroll

@@ -239,9 +239,10 @@ __device__ __forceinline__ float div_nr(float x, float y) {
 // F = the composite of all four.  The colour at fraction f is formed as c0 + f (c1 - c0).  Composites carry
 // premultiplied RGB and, in .w, the transmittance (1 - alpha) as a product of the sub-segments' exp(-a tSeg):
 // a product keeps its relative precision at any depth, where 1 - alpha formed from an alpha near 1 does not.
-// The forward and the exact backward both call this, so the backward can recompute seg/pre instead of keeping
-// them in registers from phase A; it also takes each sub-segment's density a[s], transmittance
-// A[s] = exp(-a tSeg) and 1 / a (A = 1 and 1 / a = 0 where a < 1e-6: the sub-segment is empty).
+// It also returns each sub-segment's density a[s], transmittance A[s] = exp(-a tSeg) and 1 / a (A = 1 and
+// 1 / a = 0 where a < 1e-6: the sub-segment is empty).  The reference adjoint's evaluation takes seg/pre from
+// it; the forward and the exact backward need only F and the scalars (sub_segments_F, sub_segments_w below,
+// the same arithmetic per sub-segment), which the backward recomputes instead of keeping them from phase A.
 __device__ __forceinline__ void sub_segments_ex(float4 col0, float4 col1, float tSeg, float att, float4 seg[4],
                                                 float4 pre[4], float4& F, float a_s[4], float A_s[4],
                                                 float ia_s[4]) {
@@ -281,6 +282,52 @@ __device__ __forceinline__ void sub_segments(float4 col0, float4 col1, float tSe
                                              float4 pre[4], float4& F) {
     float a_s[4], A_s[4], ia_s[4];
     sub_segments_ex(col0, col1, tSeg, att, seg, pre, F, a_s, A_s, ia_s);
+}
+
+// The scalar part of sub_segments_ex: per sub-segment s the density a[s], transmittance A[s], 1 / a, the weight
+// kq[s] of the colour step in its colour (seg[s].rgb = (1 - A[s]) c0_s + kq[s] (col1 - col0), c0_s = col0 + s/4
+// (col1 - col0)) and the transmittance T[s] before it; returns the fragment's transmittance.
+__device__ __forceinline__ float sub_segments_w(float4 col0, float4 col1, float tSeg, float att, float a_s[4],
+                                                float A_s[4], float ia_s[4], float kq_s[4], float T_s[4]) {
+#pragma clang fp contract(fast)
+    const float dw = col1.w - col0.w;
+    float T = 1.0f;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const float fm = ((float)s + 0.5f) * 0.25f;
+        const float a = (col0.w + fm * dw) * att;
+        a_s[s] = a;
+        A_s[s] = 1.0f;
+        ia_s[s] = 0.0f;
+        kq_s[s] = 0.0f;
+        T_s[s] = T;
+        if (!(a < 1e-6f)) {
+            const float A = __expf(-a * tSeg);
+            const float ia = fast_rcp(a);
+            A_s[s] = A;
+            ia_s[s] = ia;
+            kq_s[s] = -0.25f * ((tSeg + ia) * A - ia);
+        }
+        T = T * A_s[s];
+    }
+    return T;
+}
+
+// sub_segments_ex's F alone: the RGB composite sum_s T[s] seg[s].rgb is col0 times sum_s T[s] (1 - A[s]) plus
+// (col1 - col0) times sum_s T[s] ((1 - A[s]) s/4 + kq[s]): two scalar sums instead of a composite per channel.
+__device__ __forceinline__ float4 sub_segments_F(float4 col0, float4 col1, float tSeg, float att) {
+#pragma clang fp contract(fast)
+    float a_s[4], A_s[4], ia_s[4], kq_s[4], T_s[4];
+    const float Tend = sub_segments_w(col0, col1, tSeg, att, a_s, A_s, ia_s, kq_s, T_s);
+    float S0 = 0.f, S1 = 0.f;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const float om = 1.0f - A_s[s];
+        S0 += T_s[s] * om;
+        S1 += T_s[s] * (om * ((float)s * 0.25f) + kq_s[s]);
+    }
+    return make_float4(S0 * col0.x + S1 * (col1.x - col0.x), S0 * col0.y + S1 * (col1.y - col0.y),
+                       S0 * col0.z + S1 * (col1.z - col0.z), Tend);
 }
 
 // Barycentric coordinates with unsigned areas and the 1e-5 area floor (BarycentricInterpolation.glsl:29-38).
@@ -397,7 +444,8 @@ __device__ __forceinline__ float seg_scale(const ViewUniforms& U, float3_ rd) {
 // kCell: the tet's own colour (staged as vertex 0's) over the whole segment, accumulateConst
 // (IntersectRasterization.glsl:250-255, Common/ForwardCommon.glsl:37-40): F = ((1 - A) c, A), A = exp(-a t);
 // the barycentrics are then needed only by the adjoint (kBary).
-template <bool kCell = false, bool kBary = true>
+// kSeg: also E.seg / E.pre (the reference adjoint's sub-segment values); else E.F alone.
+template <bool kCell = false, bool kBary = true, bool kSeg = false>
 __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_ ro, float3_ rd, float att,
                                               float tscale, FragEval& E) {
     int fi, fo;
@@ -462,7 +510,10 @@ __device__ __forceinline__ void eval_fragment(const RecordLds& L, int j, float3_
         E.col1 = make_float4((ca.x * u + cb.x * v) + cc.x * w, (ca.y * u + cb.y * v) + cc.y * w,
                              (ca.z * u + cb.z * v) + cc.z * w, (ca.w * u + cb.w * v) + cc.w * w);
     }
-    sub_segments(E.col0, E.col1, ((tout - tin) / 4.0f) * tscale, att, E.seg, E.pre, E.F);
+    if constexpr (kSeg)
+        sub_segments(E.col0, E.col1, ((tout - tin) / 4.0f) * tscale, att, E.seg, E.pre, E.F);
+    else
+        E.F = sub_segments_F(E.col0, E.col1, ((tout - tin) / 4.0f) * tscale, att);
 }
 
 __device__ __forceinline__ int pixel_index(const ViewUniforms& U, int x, int y) { return y * U.width + x; }
@@ -529,10 +580,9 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     }
     float dLdt = 0.f;
     const float tSeg = ((E.t1 - E.t0) / 4.0f) * tscale;
-    float4 seg[4], pre[4], Fr;
-    float a_s[4], A_s[4], ia_s[4];
+    float a_s[4], A_s[4], ia_s[4], kq_s[4], T_s[4];
     // as in eval_fragment: not kept live since phase A
-    sub_segments_ex(E.col0, E.col1, tSeg, att, seg, pre, Fr, a_s, A_s, ia_s);
+    (void)sub_segments_w(E.col0, E.col1, tSeg, att, a_s, A_s, ia_s, kq_s, T_s);
     // dR.rgb is the same for every sub-segment: sum over RGB of dR c0 (sub-segment s starts at col0 + s/4
     // (col1 - col0)) = p0 + (s/4) p1, and of dR (c0 - c1) = -p1 / 4
     const float p0 = (dR[0] * E.col0.x + dR[1] * E.col0.y) + dR[2] * E.col0.z;
@@ -542,12 +592,12 @@ __device__ __forceinline__ void frag_backward_exact(const FragEval& E, const Rec
     float w0s = 0.f, w1s = 0.f;  // the RGB colour weights summed over the sub-segments
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
-        const float4 sg = seg[s], st = pre[s];
-        const float omA = st.w;  // transmittance before s: dL/d(sub-segment s) = omA dR
+        const float omA = T_s[s];  // transmittance before s: dL/d(sub-segment s) = omA dR
         const float fb = (float)s * 0.25f, fm = ((float)s + 0.5f) * 0.25f, fe = ((float)s + 1.0f) * 0.25f;
         const float sc0 = p0 + fb * p1;
         const float dsw = omA * dR[3];
-        dR[3] = dR[3] * A_s[s] - (dR[0] * sg.x + dR[1] * sg.y + dR[2] * sg.z);
+        // dR.rgb . seg[s].rgb = (1 - A) sc0 + kq p1
+        dR[3] = dR[3] * A_s[s] - ((1.0f - A_s[s]) * sc0 + kq_s[s] * p1);
         const float a = a_s[s];
         float da = 0.f, dts = 0.f, w0 = 0.f, w1 = 0.f;
         if (!(a < 1e-6f)) {
