@@ -215,7 +215,7 @@ struct dtv_view {
     DevBuf ray_color, frag_count, last_rec, num_ckpt, ckpt_t, ckpt_n;
     DevBuf tile_frags, tile_order;  // per tile: forward's fragments; launch orders of the forward and adjoint
     DevBuf pt_info, pt_vert, pt_col, tri_grad;  // Projection renderer: per-tet triangles, per-corner gradients
-    DevBuf vacc;  // tile renderer adjoint: interleaved per-vertex gradient rows of the distinct targets
+    DevBuf vacc;  // adjoint: interleaved per-vertex gradient rows of the distinct targets
     bool projection = false;        // the last render ran the Projection renderer
     bool sub_tiles = false;         // the last render binned tiles larger than 8x8
     bool queued = false;            // the last render took its tiles from a work queue (the adjoint does too)
@@ -576,8 +576,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
         DTV_HIP(hipHostMalloc((void**)&view->pinnedTargets, sizeof(AdjointTarget) * (size_t)n, hipHostMallocDefault));
         view->pinned_targets_cap = n;
     }
-    // Tile renderer: each distinct gradient target (the items of one mesh share theirs) accumulates in zeroed
-    // interleaved rows (8 floats per vertex), folded into its gradients after the adjoint.
+    // Each distinct gradient target (the items of one mesh share theirs) accumulates its vertex gradients in
+    // zeroed interleaved rows (8 floats per vertex), folded into its gradients after the adjoint.
     struct Fold {
         float* acc;
         int64_t V;
@@ -586,7 +586,7 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
     };
     std::vector<Fold> folds;
     std::memcpy(view->pinnedTargets, tg, sizeof(AdjointTarget) * (size_t)n);
-    if (!view->projection) {
+    {
         std::unordered_map<const float*, int> first;  // grad_pos -> first item
         std::vector<int64_t> row0((size_t)n, -1);
         int64_t rows = 0;
@@ -652,8 +652,8 @@ int adjoint_batch(dtv_view* view, const AdjointTarget* tg, int n, int32_t use_ab
                               view->total_subtiles, adj_order, view->sorted_recs(), view->ranges.as<uint2>(), ps,
                               use_abs_grad ? 1 : 0, adjoint_mode, view->colour_kinds, s, view->sub_tiles,
                               view->queued ? view->tile_queue() : nullptr);
-        for (const Fold& fo : folds) launch_fold_vertex_grads(fo.acc, fo.V, fo.grad_pos, fo.grad_col, s);
     }
+    for (const Fold& fo : folds) launch_fold_vertex_grads(fo.acc, fo.V, fo.grad_pos, fo.grad_col, s);
     view->mark(DTV_STAGE_ADJOINT, 1, s);
     DTV_HIP(hipGetLastError());
     return DTV_OK;

@@ -73,7 +73,7 @@ struct AdjointTarget {
     const float* dimage; // [height * width * 4]
     float* grad_pos;     // [V * 3], accumulated
     float* grad_col;     // [V * 4] (per-cell items: [T * 4]), accumulated
-    float* acc;          // tile renderer: zeroed [V * 8] rows (xyz, rgba, pad) the adjoint accumulates into,
+    float* acc;          // zeroed [V * 8] rows (xyz, rgba, pad) the adjoint accumulates vertex gradients into,
                          // folded into grad_pos / grad_col afterwards (items of one mesh share them)
 };
 

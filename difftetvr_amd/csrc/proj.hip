@@ -11,9 +11,9 @@
 //
 // The adjoint walks each pixel back to front as the tet renderer's does (transmittance checkpoints and
 // division).  It sums each record's gradients with respect to its triangle's corner NDC xy, colours and
-// thicknesses (wave reductions into LDS, then one atomic per value into the triangle's slot).  A per-tet pass
-// carries them to the vertex positions and colours through the triangle construction, by forward-mode
-// derivatives of the same arithmetic.
+// thicknesses (one wave reduce-scatter per record, then one atomic per value into the triangle's slot).  A
+// per-tet pass carries them to the vertex positions and colours through the triangle construction, by
+// forward-mode derivatives of the same arithmetic.
 #include "raster.h"
 #include "render_common.h"
 
@@ -113,111 +113,33 @@ __device__ __forceinline__ PtDec pt_decide(const ViewUniforms& U, const float P[
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// Values of the five vertices (NDC xyz, colour, thickness) given the decisions, templated on the scalar: float
-// for the render (bit-identical to the decisions' arithmetic), Dual<3> for the adjoint's position derivatives.
-template <int N>
-struct Dual {
-    float v;
-    float d[N];
-};
-template <int N>
-__device__ __forceinline__ Dual<N> dconst(float v) {
-    Dual<N> r;
-    r.v = v;
-#pragma unroll
-    for (int i = 0; i < N; i++) r.d[i] = 0.f;
-    return r;
-}
-#define DTV_DUAL_BIN(OP, VAL, DER)                                                        \
-    template <int N>                                                                      \
-    __device__ __forceinline__ Dual<N> operator OP(const Dual<N>& a, const Dual<N>& b) {  \
-        Dual<N> r;                                                                        \
-        r.v = VAL;                                                                        \
-        _Pragma("unroll") for (int i = 0; i < N; i++) r.d[i] = DER;                       \
-        return r;                                                                         \
-    }
-DTV_DUAL_BIN(+, a.v + b.v, a.d[i] + b.d[i])
-DTV_DUAL_BIN(-, a.v - b.v, a.d[i] - b.d[i])
-DTV_DUAL_BIN(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
-DTV_DUAL_BIN(/, a.v / b.v, (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v))
-#undef DTV_DUAL_BIN
-template <int N>
-__device__ __forceinline__ Dual<N> operator*(float s, const Dual<N>& a) {
-    Dual<N> r;
-    r.v = s * a.v;
-#pragma unroll
-    for (int i = 0; i < N; i++) r.d[i] = s * a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, float s) {
-    return s * a;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, float s) {
-    Dual<N> r = a;
-    r.v = a.v + s;
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator-(float s, const Dual<N>& a) {
-    Dual<N> r;
-    r.v = s - a.v;
-#pragma unroll
-    for (int i = 0; i < N; i++) r.d[i] = -a.d[i];
-    return r;
-}
-template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, float s) {
-    Dual<N> r = a;
-    r.v = a.v - s;
-    return r;
-}
-__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
-template <int N>
-__device__ __forceinline__ Dual<N> tsqrt(const Dual<N>& a) {
-    Dual<N> r;
-    r.v = sqrtf(a.v);
-    const float k = r.v > 0.f ? 0.5f / r.v : 0.f;
-#pragma unroll
-    for (int i = 0; i < N; i++) r.d[i] = k * a.d[i];
-    return r;
-}
-__device__ __forceinline__ void tzero(float& x) { x = 0.f; }
-template <int N>
-__device__ __forceinline__ void tzero(Dual<N>& a) {
-    a = dconst<N>(0.f);
-}
-
-template <class T>
+// Values of the five vertices (NDC xyz, colour, thickness) given the decisions, in the decisions' arithmetic.
 struct PtVerts {
-    T ndc[5][3];
-    T col[5][4];
-    T dep[5];
+    float ndc[5][3];
+    float col[5][4];
+    float dep[5];
 };
 
 // GetCorrectedDepth without the linear depth correction, which the renderer never enables
 // (GenerateTrianglesVTK.glsl:83-126, TetMeshRendererProjection.cpp:800-808).
-template <class T>
-__device__ __forceinline__ T pt_corrected_depth(const float* iP, T x, T y, T z1, T z2) {
-    T e1[4], e2[4];
+__device__ __forceinline__ float pt_corrected_depth(const float* iP, float x, float y, float z1, float z2) {
+    float e1[4], e2[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         e1[r] = ((iP[0 * 4 + r] * x + iP[1 * 4 + r] * y) + iP[2 * 4 + r] * z1) + iP[3 * 4 + r] * 1.0f;
         e2[r] = ((iP[0 * 4 + r] * x + iP[1 * 4 + r] * y) + iP[2 * 4 + r] * z2) + iP[3 * 4 + r] * 1.0f;
     }
-    const T d0 = e1[0] / e1[3] - e2[0] / e2[3];
-    const T d1 = e1[1] / e1[3] - e2[1] / e2[3];
-    const T d2 = e1[2] / e1[3] - e2[2] / e2[3];
-    return tsqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    const float d0 = e1[0] / e1[3] - e2[0] / e2[3];
+    const float d1 = e1[1] / e1[3] - e2[1] / e2[3];
+    const float d2 = e1[2] / e1[3] - e2[2] / e2[3];
+    return sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
 }
 
-template <class T>
-__device__ __forceinline__ void pt_vertices(const PtDec& D, const ViewUniforms& U, const T P[4][3], const T C[4][4],
-                                            bool per_cell, PtVerts<T>& V) {
+__device__ __forceinline__ void pt_vertices(const PtDec& D, const ViewUniforms& U, const float P[4][3],
+                                            const float C[4][4], bool per_cell, PtVerts& V) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        T c[4];
+        float c[4];
 #pragma unroll
         for (int r = 0; r < 4; r++)
             c[r] = ((U.viewProj[0 * 4 + r] * P[i][0] + U.viewProj[1 * 4 + r] * P[i][1]) +
@@ -226,29 +148,29 @@ __device__ __forceinline__ void pt_vertices(const PtDec& D, const ViewUniforms& 
         for (int k = 0; k < 3; k++) V.ndc[i][k] = c[k] / c[3];
 #pragma unroll
         for (int k = 0; k < 4; k++) V.col[i][k] = C[i][k];
-        tzero(V.dep[i]);
+        V.dep[i] = 0.f;
     }
 #pragma unroll
-    for (int k = 0; k < 3; k++) tzero(V.ndc[4][k]);
+    for (int k = 0; k < 3; k++) V.ndc[4][k] = 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; k++) tzero(V.col[4][k]);
-    tzero(V.dep[4]);
+    for (int k = 0; k < 4; k++) V.col[4][k] = 0.f;
+    V.dep[4] = 0.f;
     int a0 = D.s10, a1 = D.s11;
     const int b0 = D.s20, b1 = D.s21;
-    T A[3], B[3], Cv[3];
+    float A[3], B[3], Cv[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         A[k] = V.ndc[a1][k] - V.ndc[a0][k];
         B[k] = V.ndc[b1][k] - V.ndc[b0][k];
         Cv[k] = V.ndc[b0][k] - V.ndc[a0][k];
     }
-    const T den = A[0] * B[1] - A[1] * B[0];
-    T alpha = (B[1] * Cv[0] - B[0] * Cv[1]) / den;
-    const T beta = (A[1] * Cv[0] - A[0] * Cv[1]) / den;
+    const float den = A[0] * B[1] - A[1] * B[0];
+    float alpha = (B[1] * Cv[0] - B[0] * Cv[1]) / den;
+    const float beta = (A[1] * Cv[0] - A[0] * Cv[1]) / den;
     if (D.cls2) {
 #pragma unroll
         for (int k = 0; k < 3; k++) V.ndc[4][k] = V.ndc[a0][k] + alpha * A[k];
-        const T depth =
+        const float depth =
             pt_corrected_depth(U.invProj, V.ndc[4][0], V.ndc[4][1], V.ndc[4][2], V.ndc[b0][2] + beta * B[2]);
 #pragma unroll
         for (int k = 0; k < 4; k++)
@@ -263,16 +185,16 @@ __device__ __forceinline__ void pt_vertices(const PtDec& D, const ViewUniforms& 
             a1 = t;
             alpha = 1.0f - alpha;
         }
-        const T edgez = V.ndc[b0][2] + beta * B[2];
-        const T pointz = V.ndc[a0][2];
-        const T facez = (edgez + (alpha - 1.0f) * pointz) / alpha;
-        const T depth = pt_corrected_depth(U.invProj, V.ndc[a1][0], V.ndc[a1][1], V.ndc[a1][2], facez);
+        const float edgez = V.ndc[b0][2] + beta * B[2];
+        const float pointz = V.ndc[a0][2];
+        const float facez = (edgez + (alpha - 1.0f) * pointz) / alpha;
+        const float depth = pt_corrected_depth(U.invProj, V.ndc[a1][0], V.ndc[a1][1], V.ndc[a1][2], facez);
         if (!per_cell) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const T edgec = C[b0][k] + beta * (C[b1][k] - C[b0][k]);
-                const T pointc = C[a0][k];
-                const T facec = (edgec + (alpha - 1.0f) * pointc) / alpha;
+                const float edgec = C[b0][k] + beta * (C[b1][k] - C[b0][k]);
+                const float pointc = C[a0][k];
+                const float facec = (edgec + (alpha - 1.0f) * pointc) / alpha;
                 V.col[a1][k] = 0.5f * (facec + C[a1][k]);
             }
         }
@@ -349,7 +271,7 @@ __global__ __launch_bounds__(256) void k_pt_count(const BatchItem* __restrict__ 
         const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;  // the tet's slot in the batch
         pt_info[g] = pt_pack(D);
         if (D.ntri) {
-            PtVerts<float> V;
+            PtVerts V;
             pt_vertices(D, U, P, C, B.cell_col != nullptr, V);
             float4 pv[5];
 #pragma unroll
@@ -608,7 +530,6 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
                                                     const uint2* __restrict__ ranges, PixelState ps_all,
                                                     float* __restrict__ tri_grad, int use_abs) {
     __shared__ PtRecLds L;
-    __shared__ float s_acc[kBatch][21];
     const uint32_t gtile = global_tile_of_block(order);
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
@@ -648,6 +569,7 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
         dC[3] = di.w * (1.0f - U.clear[3]) - (di.x * U.clear[0] + di.y * U.clear[1] + di.z * U.clear[2]);
     }
     if (ballot(valid && n > 0) == 0ull) return;
+    const int qi = wave_scatter_index(lane);
     const uint32_t nrec = rg.y - rg.x;
     const int nbatch = (int)((nrec + kBatch - 1) / kBatch);
     for (int bi = nbatch - 1; bi >= 0; bi--) {
@@ -656,7 +578,6 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
         if (lane < cnt)
             pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, recs[2 * (int64_t)(base + lane) + 1], L,
                      lane);
-        for (int e = lane; e < kBatch * 21; e += 64) (&s_acc[0][0])[e] = 0.f;
         __syncthreads();
         for (int j = cnt - 1; j >= 0; j--) {
             const bool live = valid && n > 0 && base + (uint32_t)j < last && pt_covers(L, j, lane);
@@ -722,157 +643,386 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
                 lam[2] = f.l[2];
                 }
             }
+            float p[21];
 #pragma unroll
             for (int k = 0; k < 3; k++)
 #pragma unroll
-                for (int q = 0; q < 7; q++) {
-                    const float s = wave_sum(lam[k] * g[q]);
-                    if (lane == 0) s_acc[j][7 * k + q] = s;
-                }
-        }
-        __syncthreads();
-        for (int e = lane; e < cnt * 21; e += 64) {
-            const int j = e / 21, q = e - 21 * j;
-            const float v = s_acc[j][q];
-            if (v != 0.f) atomic_add_f32(&tri_grad[21 * (int64_t)L.id[j] + q], v);
+                for (int q = 0; q < 7; q++) p[7 * k + q] = lam[k] * g[q];
+            // the record's 21 sums land in lanes 2i (value wave_scatter_index(2i)), one atomic each
+            const float s = wave_reduce_scatter(p);
+            if ((lane & 1) == 0 && qi < 21 && s != 0.f) atomic_add_f32(&tri_grad[21 * (int64_t)L.id[j] + qi], s);
         }
         __syncthreads();
     }
 }
 
-// Per tet: the gradients of its triangles' corners, summed per vertex, carried through the construction to
-// the positions (forward-mode derivatives, one pass per tet vertex) and colours (linear weights; per cell: the
-// sum over the corners).
-__global__ __launch_bounds__(256) void k_pt_backprop(const BatchItem* __restrict__ items,
-                                                      const AdjointTarget* __restrict__ targets, int nitems,
-                                                      const uint32_t* __restrict__ pt_info,
-                                                      const float* __restrict__ tri_grad) {
-    const BatchItem& B = items[item_of_block(items, nitems, blockIdx.x)];
-    const int item = item_of_block(items, nitems, blockIdx.x);
+// Reverse pass of pt_corrected_depth: the depth's gradient gdep carried to x, y, z1 and z2 (zero where the two
+// points coincide, as the square root's derivative is taken there).
+__device__ __forceinline__ void pt_corrected_depth_adj(const float* iP, float x, float y, float z1, float z2,
+                                                       float gdep, float& gx, float& gy, float& gz1, float& gz2) {
+    float e1[4], e2[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        e1[r] = ((iP[0 * 4 + r] * x + iP[1 * 4 + r] * y) + iP[2 * 4 + r] * z1) + iP[3 * 4 + r];
+        e2[r] = ((iP[0 * 4 + r] * x + iP[1 * 4 + r] * y) + iP[2 * 4 + r] * z2) + iP[3 * 4 + r];
+    }
+    const float i1 = 1.0f / e1[3], i2 = 1.0f / e2[3];
+    float p1[3], p2[3], dd[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        p1[k] = e1[k] * i1;
+        p2[k] = e2[k] * i2;
+        dd[k] = p1[k] - p2[k];
+    }
+    const float len = sqrtf((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+    const float s = len > 0.f ? gdep / len : 0.f;
+    // dL/dp1 = s dd, dL/dp2 = -s dd; p = e.xyz / e.w
+    float ge1[4], ge2[4];
+    float w1 = 0.f, w2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float u = s * dd[k];
+        ge1[k] = u * i1;
+        ge2[k] = -u * i2;
+        w1 += u * p1[k];
+        w2 -= u * p2[k];
+    }
+    ge1[3] = -w1 * i1;
+    ge2[3] = -w2 * i2;
+    gx = gy = gz1 = gz2 = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        gx += iP[0 * 4 + r] * (ge1[r] + ge2[r]);
+        gy += iP[1 * 4 + r] * (ge1[r] + ge2[r]);
+        gz1 += iP[2 * 4 + r] * ge1[r];
+        gz2 += iP[2 * 4 + r] * ge2[r];
+    }
+}
+
+// One tet: the gradients of its triangles' corners, summed per vertex, carried back through the construction
+// (pt_vertices) to the vertex positions and colours by a hand-written reverse pass.  The tet's vertices are taken
+// in role order: 0 = seg1[0], 1 = seg1[1], 2 = seg2[0], 3 = seg2[1] (a permutation of the four), 4 = the crossing;
+// every index below is then a compile-time constant and the state stays in registers.  Out: the four vertices'
+// ids and gradients {position xyz, colour rgba} (colour zero for per-cell colours, whose gradient goes to the
+// tet here).  False when the tet made no triangles.
+__device__ __forceinline__ bool pt_backprop_tet(const BatchItem& B, float* __restrict__ grad_col, int64_t tet,
+                                                int64_t slot, uint32_t info, const float* __restrict__ tri_grad,
+                                                int32_t vid[4], float val[4][7]) {
     const ViewUniforms& U = B.U;
-    const int64_t tet = (int64_t)(blockIdx.x - B.block0) * 256 + threadIdx.x;
-    if (tet >= B.T || targets[item].dimage == nullptr) return;
-    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const PtDec D = pt_unpack(pt_info[slot]);
-    if (D.ntri == 0) return;
-    float gv[5][7];
+    const PtDec D = pt_unpack(info);
+    if (D.ntri == 0) return false;
+    // corner gradients by role: g[r] = {NDC x, NDC y, colour rgba, thickness}
+    float g[5][7];
 #pragma unroll
-    for (int v = 0; v < 5; v++)
+    for (int r = 0; r < 5; r++)
 #pragma unroll
-        for (int q = 0; q < 7; q++) gv[v][q] = 0.f;
-    for (int k = 0; k < D.ntri; k++) {
-        const float* tg = tri_grad + 21 * (slot * 4 + k);
+        for (int q = 0; q < 7; q++) g[r][q] = 0.f;
+    const float4* tg4 = reinterpret_cast<const float4*>(tri_grad + 84 * slot);  // 4 triangles x 3 corners x 7
+    float tg[84];
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int v = pt_corner(D, k, j);
-#pragma unroll
-            for (int q = 0; q < 7; q++) {
-                const float val = tg[7 * j + q];
-#pragma unroll
-                for (int w = 0; w < 5; w++) gv[w][q] += w == v ? val : 0.f;
-            }
-        }
+    for (int i = 0; i < 21; i++) {
+        const float4 t = tg4[i];
+        tg[4 * i] = t.x;
+        tg[4 * i + 1] = t.y;
+        tg[4 * i + 2] = t.z;
+        tg[4 * i + 3] = t.w;
     }
-    float P[4][3], C[4][4];
-    int32_t vid[4];
-    pt_load_tet(B, tet, P, C, vid);
+    // fans (pt_corner): class 2 {4, a0, b0, a1, b1, a0}; class 1 {T, P, b0, b1, P} with the thick vertex T and
+    // the other end P of segment 1, accumulated here as roles 1 and 0 and exchanged below when flipped
+    if (D.cls2) {
+        constexpr int fan[6] = {4, 0, 2, 1, 3, 0};
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int q = 0; q < 7; q++) g[fan[j == 0 ? 0 : k + j]][q] += tg[21 * k + 7 * j + q];
+    } else {
+        constexpr int fan[5] = {1, 0, 2, 3, 0};
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int q = 0; q < 7; q++) g[fan[j == 0 ? 0 : k + j]][q] += tg[21 * k + 7 * j + q];
+    }
     const bool per_cell = B.cell_col != nullptr;
-    float* __restrict__ grad_pos = targets[item].grad_pos;
-    float* __restrict__ grad_col = targets[item].grad_col;
-    // the special vertex: 4 (class 2) or the thick vertex (class 1)
-    const int sv = D.cls2 ? 4 : (D.flip ? D.s10 : D.s11);
-    // positions: d(outputs)/d(P[i]) by Dual<3>, outputs = corner NDC xy (all vertices), the special vertex's
-    // thickness and (per vertex colours) colour
-#pragma unroll 1
-    for (int i = 0; i < 4; i++) {
-        Dual<3> Pd[4][3], Cd[4][4];
+    // vertices in role order
+    const int4 idx = *reinterpret_cast<const int4*>(B.cells + 4 * tet);
+    vid[0] = idx.x;  // seg1[0] is always vertex 0
+    {
+        const int32_t v4[4] = {idx.x, idx.y, idx.z, idx.w};
+        const int s[3] = {D.s11, D.s20, D.s21};
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                Pd[a][k] = dconst<3>(P[a][k]);
-                if (a == i) Pd[a][k].d[k] = 1.0f;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) Cd[a][k] = dconst<3>(C[a][k]);
-        }
-        PtVerts<Dual<3>> V;
-        pt_vertices(D, U, Pd, Cd, per_cell, V);
-        float gp[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            float s = 0.f;
-#pragma unroll
-            for (int v = 0; v < 5; v++) s += gv[v][0] * V.ndc[v][0].d[c] + gv[v][1] * V.ndc[v][1].d[c];
-            s += gv[sv][6] * V.dep[sv].d[c];
-            if (!per_cell) {
-#pragma unroll
-                for (int ch = 0; ch < 4; ch++) s += gv[sv][2 + ch] * V.col[sv][ch].d[c];
-            }
-            gp[c] = s;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            if (gp[c] != 0.f) atomic_add_f32(&grad_pos[3 * (int64_t)vid[i] + c], gp[c]);
+        for (int r = 1; r < 4; r++) vid[r] = s[r - 1] == 1 ? v4[1] : s[r - 1] == 2 ? v4[2] : v4[3];
     }
-    // colours: the corners' colours are the vertices' except the special vertex's, a linear combination
-    float gc[4][4];
+    float n[4][3], iw[4], C[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int r = 0; r < 4; r++) {
+        const float3_ p = load_pos(B.pos, vid[r]);
+        const float P[3] = {p.x, p.y, p.z};
+        float c[4];
 #pragma unroll
-        for (int ch = 0; ch < 4; ch++) gc[a][ch] = a == sv ? 0.f : gv[a][2 + ch];
-    if (per_cell) {
-        float s[4];
+        for (int k = 0; k < 4; k++)
+            c[k] = ((U.viewProj[0 * 4 + k] * P[0] + U.viewProj[1 * 4 + k] * P[1]) + U.viewProj[2 * 4 + k] * P[2]) +
+                   U.viewProj[3 * 4 + k] * 1.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) n[r][k] = c[k] / c[3];
+        iw[r] = 1.0f / c[3];
+        const float4 cc = per_cell ? *reinterpret_cast<const float4*>(B.cell_col + 4 * tet)
+                                   : *reinterpret_cast<const float4*>(B.col + 4 * (int64_t)vid[r]);
+        C[r][0] = cc.x;
+        C[r][1] = cc.y;
+        C[r][2] = cc.z;
+        C[r][3] = cc.w;
+    }
+    // alpha, beta: the crossing of segment 1 (roles 0 -> 1) and segment 2 (roles 2 -> 3), as pt_vertices
+    float A[3], Bv[3], Cv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        A[k] = n[1][k] - n[0][k];
+        Bv[k] = n[3][k] - n[2][k];
+        Cv[k] = n[2][k] - n[0][k];
+    }
+    const float den = A[0] * Bv[1] - A[1] * Bv[0];
+    const float alpha = (Bv[1] * Cv[0] - Bv[0] * Cv[1]) / den;
+    const float beta = (A[1] * Cv[0] - A[0] * Cv[1]) / den;
+    float gn[4][3], gC[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        gn[r][0] = g[r][0];
+        gn[r][1] = g[r][1];
+        gn[r][2] = 0.f;
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) gC[r][ch] = g[r][2 + ch];
+    }
+    float galpha = 0.f, gbeta = 0.f;
+    const float gdep = (D.cls2 ? g[4][6] : g[1][6]) * U.attenuation;
+    if (D.cls2) {
+        // crossing: ndc4 = n0 + alpha A, depth from (ndc4, n2.z + beta B.z), colour the mean of the two edge points
+        float x4[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) x4[k] = n[0][k] + alpha * A[k];
+        float dx, dy, dz1, dz2;
+        pt_corrected_depth_adj(U.invProj, x4[0], x4[1], x4[2], n[2][2] + beta * Bv[2], gdep, dx, dy, dz1, dz2);
+        const float g4[3] = {g[4][0] + dx, g[4][1] + dy, dz1};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            gn[0][k] += (1.0f - alpha) * g4[k];
+            gn[1][k] += alpha * g4[k];
+            galpha += A[k] * g4[k];
+        }
+        gn[2][2] += (1.0f - beta) * dz2;
+        gn[3][2] += beta * dz2;
+        gbeta += Bv[2] * dz2;
+        if (!per_cell) {
+#pragma unroll
+            for (int ch = 0; ch < 4; ch++) {
+                const float h = 0.5f * g[4][2 + ch];
+                gC[0][ch] += (1.0f - alpha) * h;
+                gC[1][ch] += alpha * h;
+                gC[2][ch] += (1.0f - beta) * h;
+                gC[3][ch] += beta * h;
+                galpha += h * (C[1][ch] - C[0][ch]);
+                gbeta += h * (C[3][ch] - C[2][ch]);
+            }
+        }
+    } else {
+        // thick vertex T (role 1, or role 0 when flipped) with the depth to the face point on the line from P
+        // through T: facez = (edgez + (alpha' - 1) P.z) / alpha', alpha' = flip ? 1 - alpha : alpha
+        const bool fl = D.flip;
+        const float al = fl ? 1.0f - alpha : alpha;
+        float nT[3], nP[3], gT[3] = {0.f, 0.f, 0.f}, gP[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            nT[k] = fl ? n[0][k] : n[1][k];
+            nP[k] = fl ? n[1][k] : n[0][k];
+        }
+        const float edgez = n[2][2] + beta * Bv[2];
+        const float facez = (edgez + (al - 1.0f) * nP[2]) / al;
+        float dx, dy, dz1, dz2;
+        pt_corrected_depth_adj(U.invProj, nT[0], nT[1], nT[2], facez, gdep, dx, dy, dz1, dz2);
+        gT[0] = dx;
+        gT[1] = dy;
+        gT[2] = dz1;
+        const float ia = 1.0f / al;
+        const float gedge = dz2 * ia;
+        gP[2] += dz2 * (al - 1.0f) * ia;
+        float gal = dz2 * (nP[2] - facez) * ia;
+        gn[2][2] += (1.0f - beta) * gedge;
+        gn[3][2] += beta * gedge;
+        gbeta += Bv[2] * gedge;
+        float gCT[4] = {0.f, 0.f, 0.f, 0.f}, gCP[4] = {0.f, 0.f, 0.f, 0.f};
+        if (!per_cell) {
+            // colour of T: 0.5 (facec + C_T), facec = (edgec + (alpha' - 1) C_P) / alpha'
+#pragma unroll
+            for (int ch = 0; ch < 4; ch++) {
+                float c0 = C[0][ch], c1 = C[1][ch];
+                // keeps the two colours in registers: a select between two rows of C is otherwise formed as a
+                // dynamically indexed load from a stack copy of C
+                asm volatile("" : "+v"(c0), "+v"(c1));
+                const float CT = fl ? c0 : c1, CP = fl ? c1 : c0;
+                const float h = 0.5f * g[1][2 + ch];
+                const float edgec = C[2][ch] + beta * (C[3][ch] - C[2][ch]);
+                const float facec = (edgec + (al - 1.0f) * CP) / al;
+                gCT[ch] = h;
+                const float ge = h * ia;
+                gCP[ch] = h * (al - 1.0f) * ia;
+                gC[2][ch] += (1.0f - beta) * ge;
+                gC[3][ch] += beta * ge;
+                gbeta += ge * (C[3][ch] - C[2][ch]);
+                gal += h * (CP - facec) * ia;
+            }
+        }
+        // slots 0 and 1 hold P's and T's corner gradients: P is role 0 and T role 1, exchanged when flipped
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float sP = gn[0][k] + gP[k], sT = gn[1][k] + gT[k];
+            gn[0][k] = fl ? sT : sP;
+            gn[1][k] = fl ? sP : sT;
+        }
+        if (!per_cell) {
+#pragma unroll
+            for (int ch = 0; ch < 4; ch++) {
+                // T's corner colour is the constructed one: T's input colour receives gCT only
+                const float sP = gC[0][ch] + gCP[ch], sT = gCT[ch];
+                gC[0][ch] = fl ? sT : sP;
+                gC[1][ch] = fl ? sP : sT;
+            }
+        }
+        galpha = fl ? -gal : gal;
+    }
+    // alpha = (B.y Cv.x - B.x Cv.y) / den, beta = (A.y Cv.x - A.x Cv.y) / den, den = A.x B.y - A.y B.x
+    {
+        const float ga = galpha / den, gb = gbeta / den;
+        const float gden = -(ga * alpha + gb * beta);
+        const float gA0 = -gb * Cv[1] + gden * Bv[1], gA1 = gb * Cv[0] - gden * Bv[0];
+        const float gB0 = -ga * Cv[1] - gden * A[1], gB1 = ga * Cv[0] + gden * A[0];
+        const float gC0 = ga * Bv[1] + gb * A[1], gC1 = -ga * Bv[0] - gb * A[0];
+        gn[1][0] += gA0;
+        gn[1][1] += gA1;
+        gn[0][0] -= gA0 + gC0;
+        gn[0][1] -= gA1 + gC1;
+        gn[3][0] += gB0;
+        gn[3][1] += gB1;
+        gn[2][0] += gC0 - gB0;
+        gn[2][1] += gC1 - gB1;
+    }
+    // NDC = c.xyz / c.w, c = viewProj (P, 1)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float gc[4];
+#pragma unroll
+        for (int k = 0; k < 3; k++) gc[k] = gn[r][k] * iw[r];
+        gc[3] = -((gn[r][0] * n[r][0] + gn[r][1] * n[r][1]) + gn[r][2] * n[r][2]) * iw[r];
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            val[r][j] = (U.viewProj[j * 4 + 0] * gc[0] + U.viewProj[j * 4 + 1] * gc[1]) +
+                        (U.viewProj[j * 4 + 2] * gc[2] + U.viewProj[j * 4 + 3] * gc[3]);
+#pragma unroll
+        for (int ch = 0; ch < 4; ch++) val[r][3 + ch] = per_cell ? 0.f : gC[r][ch];
+    }
+    if (per_cell) {  // every corner carries the cell's colour
 #pragma unroll
         for (int ch = 0; ch < 4; ch++) {
-            s[ch] = gv[0][2 + ch] + gv[1][2 + ch] + gv[2][2 + ch] + gv[3][2 + ch] + gv[4][2 + ch];
-            if (s[ch] != 0.f) atomic_add_f32(&grad_col[4 * tet + ch], s[ch]);
-        }
-        return;
-    }
-    // alpha, beta of the float construction (the weights of the special colour)
-    PtVerts<float> Vf;
-    pt_vertices(D, U, P, C, false, Vf);
-    float An[2], Bn[2], Cn[2];
-    float ndc[4][2];
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-        ndc[a][0] = Vf.ndc[a][0];
-        ndc[a][1] = Vf.ndc[a][1];
-    }
-    An[0] = ndc[D.s11][0] - ndc[D.s10][0];
-    An[1] = ndc[D.s11][1] - ndc[D.s10][1];
-    Bn[0] = ndc[D.s21][0] - ndc[D.s20][0];
-    Bn[1] = ndc[D.s21][1] - ndc[D.s20][1];
-    Cn[0] = ndc[D.s20][0] - ndc[D.s10][0];
-    Cn[1] = ndc[D.s20][1] - ndc[D.s10][1];
-    const float den = An[0] * Bn[1] - An[1] * Bn[0];
-    float alpha = (Bn[1] * Cn[0] - Bn[0] * Cn[1]) / den;
-    const float beta = (An[1] * Cn[0] - An[0] * Cn[1]) / den;
-#pragma unroll
-    for (int ch = 0; ch < 4; ch++) {
-        const float gs = gv[sv][2 + ch];
-        if (D.cls2) {  // col4 = 0.5 (C_a0 + alpha (C_a1 - C_a0) + C_b0 + beta (C_b1 - C_b0))
-            gc[D.s10][ch] += 0.5f * (1.0f - alpha) * gs;
-            gc[D.s11][ch] += 0.5f * alpha * gs;
-            gc[D.s20][ch] += 0.5f * (1.0f - beta) * gs;
-            gc[D.s21][ch] += 0.5f * beta * gs;
-        } else {  // col[a1] = 0.5 ((C_b0 + beta (C_b1 - C_b0) + (alpha' - 1) C_a0) / alpha' + C_a1), flipped
-            const int a0 = D.flip ? D.s11 : D.s10, a1 = D.flip ? D.s10 : D.s11;
-            const float al = D.flip ? 1.0f - alpha : alpha;
-            gc[a1][ch] += 0.5f * gs;
-            gc[a0][ch] += 0.5f * gs * (al - 1.0f) / al;
-            gc[D.s20][ch] += 0.5f * gs * (1.0f - beta) / al;
-            gc[D.s21][ch] += 0.5f * gs * beta / al;
+            const float s = ((g[0][2 + ch] + g[1][2 + ch]) + (g[2][2 + ch] + g[3][2 + ch])) + g[4][2 + ch];
+            if (s != 0.f) atomic_add_f32(&grad_col[4 * tet + ch], s);
         }
     }
+    return true;
+}
+
+// One wave per 64 tets (a quarter of a 256-tet block).  The wave's 256 (tet, vertex) slots are summed per vertex
+// before the global atomics, as the tile renderer's adjoint does: each slot finds its vertex's entry in an LDS
+// hash table keyed by vertex id (compare-and-swap; the first slot of a vertex creates the entry), stores its
+// values in its own LDS row and links the row into the entry's list with one exchange; the entry's creator sums
+// the list, and eight lanes per distinct vertex add its seven sums to the vertex's 32-byte row of the
+// interleaved accumulator, folded into the gradients after the launch.  In a mesh ordered by locality 64
+// consecutive tets share most vertices (about five slots per distinct vertex on a grid).
+__global__ __launch_bounds__(64) void k_pt_backprop(const BatchItem* __restrict__ items,
+                                                     const AdjointTarget* __restrict__ targets, int nitems,
+                                                     const uint32_t* __restrict__ pt_info,
+                                                     const float* __restrict__ tri_grad) {
+    constexpr int kTab = 320;  // > 256 slots, so every slot finds an entry
+    __shared__ int32_t keys[kTab], head[kTab], list[256];
+    __shared__ float rows[256][8];  // 7 values, next slot
+    const unsigned blk = blockIdx.x >> 2;  // the 256-tet block
+    const int item = item_of_block(items, nitems, blk);
+    const BatchItem& B = items[item];
+    const AdjointTarget& Tg = targets[item];
+    if (Tg.dimage == nullptr) return;  // uniform over the block
+    const int lane = threadIdx.x;
+    const int64_t tet = (int64_t)(blk - B.block0) * 256 + (blockIdx.x & 3u) * 64 + lane;
+    const int64_t slot = (int64_t)blk * 256 + (blockIdx.x & 3u) * 64 + lane;
+    int32_t vid[4] = {0, 0, 0, 0};
+    float val[4][7];
+    const bool live = tet < B.T && pt_backprop_tet(B, Tg.grad_col, tet, slot, pt_info[slot], tri_grad, vid, val);
+    if (ballot(live) == 0ull) return;
+    for (int e = lane; e < kTab; e += 64) {
+        keys[e] = -1;
+        head[e] = -1;
+    }
+    __syncthreads();
+    int ent[4];
+    bool owner[4];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int ch = 0; ch < 4; ch++) {
-            const float v = gc[a][ch];
-            if (v != 0.f) atomic_add_f32(&grad_col[4 * (int64_t)vid[a] + ch], v);
+    for (int k = 0; k < 4; k++) {
+        ent[k] = -1;
+        owner[k] = false;
+        if (!live) continue;
+        const int32_t v = vid[k];
+        int e = (int)((((uint32_t)v * 2654435761u) >> 16) * (uint32_t)kTab >> 16);  // multiplicative hash
+        for (int probe = 0; probe < kTab; probe++) {
+            const int32_t old = atomicCAS(&keys[e], -1, v);
+            if (old == -1 || old == v) {
+                ent[k] = e;
+                owner[k] = old == -1;
+                break;
+            }
+            e = e + 1 == kTab ? 0 : e + 1;
         }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (ent[k] < 0) continue;
+        const int sl = 64 * k + lane;
+#pragma unroll
+        for (int c = 0; c < 7; c++) rows[sl][c] = val[k][c];
+        rows[sl][7] = __int_as_float(atomicExch(&head[ent[k]], sl));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!owner[k]) continue;
+        float sum[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int sl = head[ent[k]]; sl >= 0;) {
+#pragma unroll
+            for (int c = 0; c < 7; c++) sum[c] += rows[sl][c];
+            sl = __float_as_int(rows[sl][7]);
+        }
+        const int own = 64 * k + lane;
+#pragma unroll
+        for (int c = 0; c < 7; c++) rows[own][c] = sum[c];
+        rows[own][7] = __int_as_float(vid[k]);
+    }
+    uint32_t nown = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t m = ballot(owner[k]);
+        if (owner[k]) list[nown + (uint32_t)popc64(m & lanemask_lt())] = 64 * k + lane;
+        nown += (uint32_t)popc64(m);
+    }
+    __syncthreads();
+    float* __restrict__ acc = Tg.acc;
+    const int c = lane & 7;
+    for (uint32_t b0 = 0; b0 < nown; b0 += 8) {
+        const uint32_t li = b0 + (uint32_t)(lane >> 3);
+        if (li < nown && c < 7) {
+            const int sl = list[li];
+            const float v = rows[sl][c];
+            if (v != 0.f) atomic_add_f32(&acc[8 * (int64_t)__float_as_int(rows[sl][7]) + c], v);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -915,7 +1065,7 @@ void launch_pt_adjoint(const BatchItem* items, const AdjointTarget* targets, int
     hipLaunchKernelGGL(k_pt_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, order, pt_info,
                        pt_vert, pt_col, recs, ranges, ps, tri_grad, use_abs);
     if (nblocks > 0)
-        hipLaunchKernelGGL(k_pt_backprop, dim3((unsigned)nblocks), dim3(256), 0, s, items, targets, nitems, pt_info,
+        hipLaunchKernelGGL(k_pt_backprop, dim3((unsigned)(4 * nblocks)), dim3(64), 0, s, items, targets, nitems, pt_info,
                            tri_grad);
 }
 

@@ -255,6 +255,32 @@ def test_gpu_projection_adjoint_central_differences(gpu, per_cell):
     assert np.linalg.norm(fd_p) > 1e-3 and np.linalg.norm(fd_c) > 1e-3
     assert ec <= 1e-4
     assert ep <= 1e-3
+    # every triangle construction (class 2; class 1 with the thick vertex at either end of segment 1), each
+    # through a directional derivative over all positions and colours of the vertices of that class's tets
+    groups = {"class 2": [], "class 1": [], "class 1 flipped": []}
+    for t in range(T):
+        vs = w["cells"][t]
+        cl = ccol[t][None].repeat(4, 0) if per_cell else col[vs]
+        pt = oracle.proj_tet(pos[vs], cl, ud, uf, ZNEAR, 1.0, per_cell=per_cell)
+        if pt["ntri"] == 0:
+            continue
+        # class 1: corner 0 of the first triangle is the thick vertex, tet vertex 0 when segment 1 is reversed
+        name = "class 2" if pt["cls2"] else ("class 1 flipped" if pt["corners"][0, 0] == 0 else "class 1")
+        groups[name].append(t)
+    for name, tets in groups.items():
+        assert len(tets) >= 5, f"{name}: only {len(tets)} tets in view"
+        vmask = np.zeros(pos.shape[0], bool)
+        vmask[w["cells"][tets].reshape(-1)] = True
+        dp = rng.normal(size=p64.shape) * vmask[:, None]
+        if per_cell:
+            dc = np.zeros_like(c64)
+            dc[tets] = rng.normal(size=(len(tets), 4))
+        else:
+            dc = rng.normal(size=c64.shape) * vmask[:, None]
+        fd = (loss(p64 + h * dp, c64 + h * dc) - loss(p64 - h * dp, c64 - h * dc)) / (2 * h)
+        an = float(np.sum(gp * dp) + np.sum(gc * dc))
+        print(f"  {name} ({len(tets)} tets): directional {an:.6e} vs differences {fd:.6e}", flush=True)
+        assert abs(an - fd) <= 1e-3 * abs(fd) + 1e-6
 
 
 @pytest.mark.gpu
