@@ -102,6 +102,24 @@ __device__ __forceinline__ float wave_reduce_scatter(const float (&v)[N]) {
     return e[0] + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(e[0]), 0xB1, 0xF, 0xF, true));
 }
 
+// The same within each row of 16 lanes (each row reduces its own values): lane L returns in out[k] the row's sum
+// of value 16 k + row_scatter_index(L) (meaningful when below N).  Levels over row-lane bits 3, 2, 1 and 0.
+__device__ __forceinline__ int row_scatter_index(int lane) {
+    return ((lane >> 3) & 1) | ((lane >> 1) & 2) | ((lane << 1) & 4) | ((lane << 3) & 8);
+}
+
+template <int N>
+__device__ __forceinline__ void row_reduce_scatter(const float (&v)[N], float (&out)[(N + 15) / 16]) {
+    static_assert(N >= 1 && N <= 32, "row_reduce_scatter: 1 to 32 values");
+    constexpr int N1 = (N + 1) / 2, N2 = (N1 + 1) / 2, N3 = (N2 + 1) / 2;
+    const int lane = lane_id();
+    float a[N1], b[N2], c[N3];
+    detail::dpp_halve<0x140, 3>(v, a, lane);    // row_mirror: lane i <-> 15 - i
+    detail::dpp_halve<0x141, 2>(a, b, lane);    // row_half_mirror: lane i <-> 7 - i
+    detail::dpp_halve<0x4E, 1>(b, c, lane);     // quad_perm [2,3,0,1]: lane i <-> i ^ 2
+    detail::dpp_halve<0xB1, 0>(c, out, lane);   // quad_perm [1,0,3,2]: lane i <-> i ^ 1
+}
+
 // Exclusive scan over the threads of a workgroup of BLOCK threads (BLOCK multiple of 64, <= 1024).
 template <int BLOCK>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp /* >= BLOCK/64 + 1 */,

@@ -519,8 +519,14 @@ __global__ __launch_bounds__(64) void k_pt_forward(const BatchItem* __restrict__
     if (lane == 63) ps.tile_frags[gtile] = tile_total;
 }
 
-// Adjoint: per pixel back to front; per record the 21 gradient sums (corner j: -lambda_j G.x, -lambda_j G.y,
-// lambda_j dL/dc, lambda_j dL/dd) into tri_grad[21 * id] with one atomic per nonzero value.
+// Adjoint, per pixel back to front.  The wave's lanes cover the tile as four 4x4 sub-tiles, one per row of 16
+// lanes (sub-tile g = lane / 16 at (g & 1, g >> 1)).  Each row walks the records covering its sub-tile, so one
+// step of the loop advances four records, one per row; a record covering several sub-tiles is met by each of
+// their rows.  Per step every row sums its record's 21 gradient terms (corner j: -lambda_j G.x, -lambda_j G.y,
+// lambda_j dL/dc, lambda_j dL/dd) by a reduce-scatter within the row into its own LDS copy of the batch's sums
+// (plain adds: the rows never share a copy).  After the batch the four copies are added and each nonzero sum
+// goes into tri_grad[21 * id] with one atomic, 21 consecutive words per record.  The loop itself issues no
+// global atomics, so its transmittance checkpoint loads never wait on them.
 __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__ items,
                                                     const AdjointTarget* __restrict__ targets, int nitems,
                                                     const uint32_t* __restrict__ order,
@@ -530,6 +536,7 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
                                                     const uint2* __restrict__ ranges, PixelState ps_all,
                                                     float* __restrict__ tri_grad, int use_abs) {
     __shared__ PtRecLds L;
+    __shared__ float s_acc[4][kBatch * 21];  // per row: the batch's sums, record j at [21 j]
     const uint32_t gtile = global_tile_of_block(order);
     const int item = item_of_tile(items, nitems, gtile);
     const BatchItem& B = items[item];
@@ -539,7 +546,10 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
     const PixelState ps = item_pixel_state(ps_all, B);
     const int tile = (int)(gtile - B.tile0);
     const int lane = threadIdx.x;
-    const int x = (tile % U.tiles_x) * kTileEdge + (lane & 7), y = (tile / U.tiles_x) * kTileEdge + (lane >> 3);
+    const int grp = lane >> 4;
+    const int lx = 4 * (grp & 1) + (lane & 3), ly = 4 * (grp >> 1) + ((lane >> 2) & 3);
+    const int cbit = 8 * ly + lx;  // the pixel's bit in a record's coverage mask
+    const int x = (tile % U.tiles_x) * kTileEdge + lx, y = (tile / U.tiles_x) * kTileEdge + ly;
     const bool valid = x < U.width && y < U.height;
     const float px = 2.0f * (((float)x + 0.5f) / (float)U.width) - 1.0f;
     const float py = 2.0f * (((float)y + 0.5f) / (float)U.height) - 1.0f;
@@ -569,19 +579,37 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
         dC[3] = di.w * (1.0f - U.clear[3]) - (di.x * U.clear[0] + di.y * U.clear[1] + di.z * U.clear[2]);
     }
     if (ballot(valid && n > 0) == 0ull) return;
-    const int qi = wave_scatter_index(lane);
+    // AbsGS-style absolute colour gradients: the per-cell variant only, as the reference's
+    // (AdjointProjectedRasterization.glsl:214-222)
+    const bool abs_col = use_abs && B.cell_col != nullptr;
+    const int qs = row_scatter_index(lane);  // this lane's terms of the row's sums: qs and 16 + qs
     const uint32_t nrec = rg.y - rg.x;
     const int nbatch = (int)((nrec + kBatch - 1) / kBatch);
     for (int bi = nbatch - 1; bi >= 0; bi--) {
         const uint32_t base = rg.x + (uint32_t)bi * kBatch;
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt)
-            pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, recs[2 * (int64_t)(base + lane) + 1], L,
-                     lane);
+        uint32_t sub = 0u;  // sub-tiles the record of this lane covers
+        if (lane < cnt) {
+            const uint4 rec = recs[2 * (int64_t)(base + lane) + 1];
+            pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, rec, L, lane);
+            sub = ((rec.y & 0x0F0F0F0Fu) ? 1u : 0u) | ((rec.y & 0xF0F0F0F0u) ? 2u : 0u) |
+                  ((rec.z & 0x0F0F0F0Fu) ? 4u : 0u) | ((rec.z & 0xF0F0F0F0u) ? 8u : 0u);
+        }
+        // the records of this lane's sub-tile
+        uint32_t gm;
+        {
+            const uint32_t m0 = (uint32_t)ballot(sub & 1u), m1 = (uint32_t)ballot(sub & 2u);
+            const uint32_t m2 = (uint32_t)ballot(sub & 4u), m3 = (uint32_t)ballot(sub & 8u);
+            gm = grp == 0 ? m0 : grp == 1 ? m1 : grp == 2 ? m2 : m3;
+        }
+        for (int e = lane; e < kBatch * 21; e += 64)
+            reinterpret_cast<float4*>(&s_acc[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);  // all four copies
         __syncthreads();
-        for (int j = cnt - 1; j >= 0; j--) {
-            const bool live = valid && n > 0 && base + (uint32_t)j < last && pt_covers(L, j, lane);
-            if (ballot(live) == 0ull) continue;
+        while (ballot(gm != 0u) != 0ull) {
+            const int j = gm != 0u ? 31 - __clz(gm) : -1;  // back to front
+            if (j >= 0) gm &= ~(1u << j);
+            const bool live = j >= 0 && valid && n > 0 && base + (uint32_t)j < last &&
+                              ((L.mask[cbit >> 5][j] >> (cbit & 31)) & 1u);
             float g[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             float lam[3] = {0.f, 0.f, 0.f};
             if (live) {
@@ -596,14 +624,9 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
                         ck_n2 = jck >= 1 ? ps.ckpt_n[(int64_t)(jck - 1) * P + pix] : 0xFFFFFFFFu;
                     }
                 }
-                float Tin;
-                if (jck >= 0 && ck_n == n - 1) {
-                    Tin = ck_t;
-                } else if (n == 1) {
-                    Tin = 1.0f;
-                } else {
-                    Tin = Cw < kPtTinyT ? 0.0f : div_nr(Cw, f.A);
-                }
+                float Tin = Cw < kPtTinyT ? 0.0f : div_nr(Cw, f.A);
+                Tin = n == 1 ? 1.0f : Tin;
+                Tin = (jck >= 0 && ck_n == n - 1) ? ck_t : Tin;
                 {
 #pragma clang fp contract(fast)
                 const float a = 1.0f - f.A;
@@ -614,11 +637,8 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
                 n--;
                 // F = (alpha c.rgb, alpha), alpha = 1 - exp(-c.a d)
                 const float dal = dF[3] + (dF[0] * f.c[0] + dF[1] * f.c[1] + dF[2] * f.c[2]);
-                float dc[4] = {a * dF[0], a * dF[1], a * dF[2], dal * f.d * f.A};
+                const float dc[4] = {a * dF[0], a * dF[1], a * dF[2], dal * f.d * f.A};
                 const float dd = dal * f.c[3] * f.A;
-                // AbsGS-style absolute colour gradients: the per-cell variant only, as the reference's
-                // (AdjointProjectedRasterization.glsl:214-222)
-                const bool abs_col = use_abs && B.cell_col != nullptr;
                 // dL/dlambda_k = dc . c_k + dd d_k; moving corner k by delta (NDC xy) changes lambda by
                 // -lambda_k grad(lambda) . delta, so dL/d(corner k) = -lambda_k G, G = sum_k dL/dlambda_k grad lambda_k
                 float gl[3];
@@ -648,9 +668,19 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
             for (int k = 0; k < 3; k++)
 #pragma unroll
                 for (int q = 0; q < 7; q++) p[7 * k + q] = lam[k] * g[q];
-            // the record's 21 sums land in lanes 2i (value wave_scatter_index(2i)), one atomic each
-            const float s = wave_reduce_scatter(p);
-            if ((lane & 1) == 0 && qi < 21 && s != 0.f) atomic_add_f32(&tri_grad[21 * (int64_t)L.id[j] + qi], s);
+            float s[2];
+            row_reduce_scatter(p, s);
+            if (j >= 0) {
+                float* a = &s_acc[grp][21 * j];
+                a[qs] += s[0];
+                if (qs < 5) a[16 + qs] += s[1];
+            }
+        }
+        __syncthreads();
+        for (int e = lane; e < cnt * 21; e += 64) {
+            const float v = (s_acc[0][e] + s_acc[1][e]) + (s_acc[2][e] + s_acc[3][e]);
+            const int j = e / 21;
+            if (v != 0.f) atomic_add_f32(&tri_grad[21 * (int64_t)L.id[j] + (e - 21 * j)], v);
         }
         __syncthreads();
     }
