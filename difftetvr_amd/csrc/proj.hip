@@ -352,7 +352,8 @@ __global__ __launch_bounds__(256) void k_pt_emit(const BatchItem* __restrict__ i
     }
 }
 
-// Sorted record i: out[2i + 1] = {triangle id, coverage mask low, high, 0} (out[2i] is not used).
+// Sorted record i: out[2i + 1] = {triangle id, coverage mask low, high, corner slots} (out[2i] is not used);
+// corner slots: the tet's vertex slot (0-4) of triangle corner j in bits 3 j .. 3 j + 2.
 __global__ __launch_bounds__(256) void k_pt_gather(const BatchItem* __restrict__ items, int nitems,
                                                     const uint32_t* __restrict__ pt_info,
                                                     const float4* __restrict__ pt_vert,
@@ -371,7 +372,10 @@ __global__ __launch_bounds__(256) void k_pt_gather(const BatchItem* __restrict__
     uint32_t key;
     pt_screen(U, pt_vert + 5 * g, D, (int)(id & 3), S, key);
     const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
-    out[2 * i + 1] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), 0u);
+    const int k = (int)(id & 3);
+    const uint32_t slots = (uint32_t)pt_corner(D, k, 0) | ((uint32_t)pt_corner(D, k, 1) << 3) |
+                           ((uint32_t)pt_corner(D, k, 2) << 6);
+    out[2 * i + 1] = make_uint4(id, (uint32_t)m, (uint32_t)(m >> 32), slots);
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -386,33 +390,38 @@ struct PtRecLds {
     uint32_t id[kBatch];
 };
 
+// A record's triangle corners as fetched from the per-tet state: NDC xy and thickness (q), colour (c).
+struct PtRecData {
+    float4 q[3], c[3];
+};
+
 // slot0: the item's first tet slot in the batch (block0 * 256).
-__device__ __forceinline__ void pt_stage(const float4* __restrict__ pt_vert, const float4* __restrict__ pt_col,
-                                         const uint32_t* __restrict__ pt_info, int64_t slot0, uint4 rec, PtRecLds& L,
-                                         int i) {
-    const uint32_t id = rec.x;
-    const int64_t tet = slot0 + (id >> 2);
-    const PtDec D = pt_unpack(pt_info[tet]);
-    float X[3], Y[3];
+__device__ __forceinline__ void pt_fetch(const float4* __restrict__ pt_vert, const float4* __restrict__ pt_col,
+                                         int64_t slot0, uint4 rec, PtRecData& R) {
+    const int64_t tet = slot0 + (rec.x >> 2);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const int v = pt_corner(D, (int)(id & 3), j);
-        const float4 q = pt_vert[5 * tet + v];
-        const float4 c = pt_col[5 * tet + v];
-        X[j] = q.x;
-        Y[j] = q.y;
-        L.X[j][i] = q.x;
-        L.Y[j][i] = q.y;
-        L.d[j][i] = q.w;
-        L.c[4 * j + 0][i] = c.x;
-        L.c[4 * j + 1][i] = c.y;
-        L.c[4 * j + 2][i] = c.z;
-        L.c[4 * j + 3][i] = c.w;
+        const int v = (int)((rec.w >> (3 * j)) & 7u);
+        R.q[j] = pt_vert[5 * tet + v];
+        R.c[j] = pt_col[5 * tet + v];
     }
-    L.invD[i] = 1.0f / ((X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]));
+}
+
+__device__ __forceinline__ void pt_store(int64_t slot0, uint4 rec, const PtRecData& R, PtRecLds& L, int i) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        L.X[j][i] = R.q[j].x;
+        L.Y[j][i] = R.q[j].y;
+        L.d[j][i] = R.q[j].w;
+        L.c[4 * j + 0][i] = R.c[j].x;
+        L.c[4 * j + 1][i] = R.c[j].y;
+        L.c[4 * j + 2][i] = R.c[j].z;
+        L.c[4 * j + 3][i] = R.c[j].w;
+    }
+    L.invD[i] = 1.0f / ((R.q[1].x - R.q[0].x) * (R.q[2].y - R.q[0].y) - (R.q[2].x - R.q[0].x) * (R.q[1].y - R.q[0].y));
     L.mask[0][i] = rec.y;
     L.mask[1][i] = rec.z;
-    L.id[i] = (uint32_t)(4 * tet) + (id & 3u);  // the triangle's slot in the batch
+    L.id[i] = (uint32_t)(4 * (slot0 + (rec.x >> 2))) + (rec.x & 3u);  // the triangle's slot in the batch
 }
 
 // One fragment: barycentrics of the pixel centre (px, py) (NDC), interpolated colour c and thickness d,
@@ -446,13 +455,24 @@ __device__ __forceinline__ bool pt_covers(const PtRecLds& L, int j, int lane) {
 
 constexpr float kPtTinyT = 0x1p-100f;  // transmittance below which a fragment's input is not recovered
 
+// The forward's fragment list and round values.
+constexpr int kPtRound = 128;  // fragments evaluated per round, two per lane
+struct PtFragLds {
+    uint16_t entry[kBatch * 64];  // (pixel lane << 5) | record index, ordered by (rank, pixel lane)
+    float4 val[kPtRound];         // F = ((1 - A) c.rgb, A)
+};
+
+// Forward: per batch of 32 records the covered (pixel, record) pairs form a dense list ordered by (rank,
+// pixel lane), rank r holding the r-th fragment of every pixel that has more than r; rounds of kPtRound
+// fragments are evaluated across the lanes, then each pixel lane composites its fragments of the round front
+// to back (about one per pixel per round).
 __global__ __launch_bounds__(64) void k_pt_forward(const BatchItem* __restrict__ items, int nitems,
                                                     const uint32_t* __restrict__ order,
-                                                    const uint32_t* __restrict__ pt_info,
                                                     const float4* __restrict__ pt_vert,
                                                     const float4* __restrict__ pt_col, const uint4* __restrict__ recs,
                                                     const uint2* __restrict__ ranges, PixelState ps_all) {
     __shared__ PtRecLds L;
+    __shared__ PtFragLds F;
     const uint32_t gtile = global_tile_of_block(order);
     const BatchItem& B = items[item_of_tile(items, nitems, gtile)];
     const ViewUniforms& U = B.U;
@@ -460,51 +480,86 @@ __global__ __launch_bounds__(64) void k_pt_forward(const BatchItem* __restrict__
     const PixelState ps = item_pixel_state(ps_all, B);
     const int tile = (int)(gtile - B.tile0);
     const int lane = threadIdx.x;
-    const int x = (tile % U.tiles_x) * kTileEdge + (lane & 7), y = (tile / U.tiles_x) * kTileEdge + (lane >> 3);
+    const uint64_t lt = lanemask_lt();
+    const int x0 = (tile % U.tiles_x) * kTileEdge, y0 = (tile / U.tiles_x) * kTileEdge;
+    const int x = x0 + (lane & 7), y = y0 + (lane >> 3);
     const bool valid = x < U.width && y < U.height;
-    const float px = 2.0f * (((float)x + 0.5f) / (float)U.width) - 1.0f;
-    const float py = 2.0f * (((float)y + 0.5f) / (float)U.height) - 1.0f;
     const uint2 rg = ranges[gtile];
     const int P = U.width * U.height;
     const int pix = valid ? pixel_index(U, x, y) : 0;
+    const int64_t slot0 = (int64_t)B.block0 * 256;
     float4 C = make_float4(0.f, 0.f, 0.f, 1.0f);  // premultiplied RGB, transmittance
     uint32_t n = 0, nck = 0, nlast = 0, last = rg.x;
+    // Two-stage prefetch: while a batch is shaded, the next batch's corner data and the record words of the
+    // batch after it are in flight.
+    uint4 cur = make_uint4(0u, 0u, 0u, 0u), nxt = cur;
+    PtRecData cd;
+    if (lane < kBatch && rg.x + lane < rg.y) {
+        cur = recs[2 * (int64_t)(rg.x + lane) + 1];
+        pt_fetch(pt_vert, pt_col, slot0, cur, cd);
+    }
+    if (lane < kBatch && rg.x + kBatch + lane < rg.y) nxt = recs[2 * (int64_t)(rg.x + kBatch + lane) + 1];
     for (uint32_t base = rg.x; base < rg.y; base += kBatch) {
         const int cnt = (int)min((uint32_t)kBatch, rg.y - base);
-        if (lane < cnt)
-            pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, recs[2 * (int64_t)(base + lane) + 1], L,
-                     lane);
-        __syncthreads();
-        for (int j = 0; j < cnt; j++) {
-            if (!valid || !pt_covers(L, j, lane)) continue;
-            PtFrag f;
-            pt_eval(L, j, px, py, f);
-            const float Tout = C.w * f.A;
-            // checkpoint rules of the tet renderer: before an underflow, and periodically
-            if (C.w >= kPtTinyT && Tout < kPtTinyT && nlast != n && (int)nck < U.max_ckpt) {
-                ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
-                ps.ckpt_n[(int64_t)nck * P + pix] = n;
-                nck++;
-                nlast = n;
-            }
-            {
-#pragma clang fp contract(fast)
-                const float a = 1.0f - f.A;
-                C.x = C.x + C.w * (a * f.c[0]);
-                C.y = C.y + C.w * (a * f.c[1]);
-                C.z = C.z + C.w * (a * f.c[2]);
-            }
-            C.w = Tout;
-            n++;
-            last = base + (uint32_t)j + 1;
-            if ((int)(n - nlast) >= U.ckpt_interval && C.w >= kPtTinyT && (int)nck < U.max_ckpt - 1) {
-                ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
-                ps.ckpt_n[(int64_t)nck * P + pix] = n;
-                nck++;
-                nlast = n;
-            }
+        if (lane < cnt) pt_store(slot0, cur, cd, L, lane);
+        if (lane < kBatch && base + kBatch + lane < rg.y) {
+            cur = nxt;
+            pt_fetch(pt_vert, pt_col, slot0, cur, cd);
         }
+        if (lane < kBatch && base + 2 * kBatch + lane < rg.y) nxt = recs[2 * (int64_t)(base + 2 * kBatch + lane) + 1];
         __syncthreads();
+        uint32_t mask = 0u;
+        if (valid)
+            for (int j = 0; j < cnt; j++) mask |= (uint32_t)pt_covers(L, j, lane) << j;
+        // the coverage masks are consumed: their slots hold the list's rank masks
+        uint64_t* rank_mask = reinterpret_cast<uint64_t*>(&L.mask[0][0]);
+        const uint32_t total = build_fragment_list(F.entry, rank_mask, mask, false, lane);
+        const uint32_t nmine = (uint32_t)__popc(mask);
+        RankCursor cur_e;
+        __syncthreads();
+        for (uint32_t r0 = 0; r0 < total; r0 += kPtRound) {
+            for (int i = lane; i < kPtRound; i += 64) {
+                const uint32_t e = r0 + (uint32_t)i;
+                if (e >= total) break;
+                const uint32_t ent = F.entry[e];
+                const int pl = (int)(ent >> 5), j = (int)(ent & 31u);
+                const float px = 2.0f * (((float)(x0 + (pl & 7)) + 0.5f) / (float)U.width) - 1.0f;
+                const float py = 2.0f * (((float)(y0 + (pl >> 3)) + 0.5f) / (float)U.height) - 1.0f;
+                PtFrag f;
+                pt_eval(L, j, px, py, f);
+                const float a = 1.0f - f.A;
+                F.val[i] = make_float4(a * f.c[0], a * f.c[1], a * f.c[2], f.A);
+            }
+            __syncthreads();
+            uint32_t e;
+            while (cur_e.next(rank_mask, nmine, lt, r0 + kPtRound, e)) {
+                const float4 f = F.val[e - r0];
+                const float Tout = C.w * f.w;
+                // checkpoint rules of the tet renderer: before an underflow, and periodically
+                if (C.w >= kPtTinyT && Tout < kPtTinyT && nlast != n && (int)nck < U.max_ckpt) {
+                    ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
+                    ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                    nck++;
+                    nlast = n;
+                }
+                {
+#pragma clang fp contract(fast)
+                    C.x = C.x + C.w * f.x;
+                    C.y = C.y + C.w * f.y;
+                    C.z = C.z + C.w * f.z;
+                }
+                C.w = Tout;
+                n++;
+                last = base + (F.entry[e] & 31u) + 1;
+                if ((int)(n - nlast) >= U.ckpt_interval && C.w >= kPtTinyT && (int)nck < U.max_ckpt - 1) {
+                    ps.ckpt_t[(int64_t)nck * P + pix] = C.w;
+                    ps.ckpt_n[(int64_t)nck * P + pix] = n;
+                    nck++;
+                    nlast = n;
+                }
+            }
+            __syncthreads();
+        }
     }
     if (valid) {
         const float T = C.w;
@@ -530,7 +585,6 @@ __global__ __launch_bounds__(64) void k_pt_forward(const BatchItem* __restrict__
 __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__ items,
                                                     const AdjointTarget* __restrict__ targets, int nitems,
                                                     const uint32_t* __restrict__ order,
-                                                    const uint32_t* __restrict__ pt_info,
                                                     const float4* __restrict__ pt_vert,
                                                     const float4* __restrict__ pt_col, const uint4* __restrict__ recs,
                                                     const uint2* __restrict__ ranges, PixelState ps_all,
@@ -591,7 +645,9 @@ __global__ __launch_bounds__(64) void k_pt_adjoint(const BatchItem* __restrict__
         uint32_t sub = 0u;  // sub-tiles the record of this lane covers
         if (lane < cnt) {
             const uint4 rec = recs[2 * (int64_t)(base + lane) + 1];
-            pt_stage(pt_vert, pt_col, pt_info, (int64_t)B.block0 * 256, rec, L, lane);
+            PtRecData R;
+            pt_fetch(pt_vert, pt_col, (int64_t)B.block0 * 256, rec, R);
+            pt_store((int64_t)B.block0 * 256, rec, R, L, lane);
             sub = ((rec.y & 0x0F0F0F0Fu) ? 1u : 0u) | ((rec.y & 0xF0F0F0F0u) ? 2u : 0u) |
                   ((rec.z & 0x0F0F0F0Fu) ? 4u : 0u) | ((rec.z & 0xF0F0F0F0u) ? 8u : 0u);
         }
@@ -1082,8 +1138,9 @@ void launch_pt_forward(const BatchItem* items, int nitems, int64_t ntiles, const
                        const uint32_t* pt_info, const float4* pt_vert, const float4* pt_col, const uint4* recs,
                        const uint2* ranges, PixelState ps, hipStream_t s) {
     if (ntiles <= 0) return;
-    hipLaunchKernelGGL(k_pt_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order, pt_info, pt_vert,
-                       pt_col, recs, ranges, ps);
+    (void)pt_info;  // the records carry their corners' vertex slots
+    hipLaunchKernelGGL(k_pt_forward, dim3((unsigned)ntiles), dim3(64), 0, s, items, nitems, order, pt_vert, pt_col,
+                       recs, ranges, ps);
 }
 
 void launch_pt_adjoint(const BatchItem* items, const AdjointTarget* targets, int nitems, int64_t ntiles,
@@ -1092,11 +1149,11 @@ void launch_pt_adjoint(const BatchItem* items, const AdjointTarget* targets, int
                        int64_t tri_grad_floats, int use_abs, hipStream_t s) {
     if (ntiles <= 0) return;
     fill_u32(reinterpret_cast<uint32_t*>(tri_grad), 0u, tri_grad_floats, s);
-    hipLaunchKernelGGL(k_pt_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, order, pt_info,
-                       pt_vert, pt_col, recs, ranges, ps, tri_grad, use_abs);
+    hipLaunchKernelGGL(k_pt_adjoint, dim3((unsigned)ntiles), dim3(64), 0, s, items, targets, nitems, order, pt_vert,
+                       pt_col, recs, ranges, ps, tri_grad, use_abs);
     if (nblocks > 0)
-        hipLaunchKernelGGL(k_pt_backprop, dim3((unsigned)(4 * nblocks)), dim3(64), 0, s, items, targets, nitems, pt_info,
-                           tri_grad);
+        hipLaunchKernelGGL(k_pt_backprop, dim3((unsigned)(4 * nblocks)), dim3(64), 0, s, items, targets, nitems,
+                           pt_info, tri_grad);
 }
 
 }  // namespace dtv

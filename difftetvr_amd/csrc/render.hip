@@ -51,65 +51,6 @@ struct AdjFragLds {
     float4 val[kRoundAdj];      // F, then dL/dF (.w: d/d(alpha), alpha = 1 - F.w); w = kMissMarker on a miss
 };
 
-// Coverage masks -> dense fragment list ordered by (rank, pixel lane): rank r holds the r-th fragment (by record
-// index, ascending, or descending when `reverse`) of every pixel that has more than r, in lane order.  A round of
-// the list then holds about one fragment per pixel, so the per-pixel compositing (or its inversion) that walks
-// each pixel's fragments in order takes few serial steps per round.  rank_mask[r] = the lanes with more than r
-// fragments (at most 32 ranks).  Returns the number of fragments.
-__device__ __forceinline__ uint32_t build_fragment_list(uint16_t* entry, uint64_t* rank_mask, uint32_t mask,
-                                                        bool reverse, int lane) {
-    const uint64_t lt = lanemask_lt();
-    uint32_t m = mask;
-    uint64_t act = ballot(m != 0u);
-    uint32_t base = 0;
-    for (int r = 0; act != 0ull; r++) {
-        if (m != 0u) {
-            const int j = reverse ? 31 - __clz(m) : __ffs(m) - 1;
-            entry[base + (uint32_t)popc64(act & lt)] = (uint16_t)((lane << 5) | j);
-            m &= ~(1u << j);
-        }
-        if (lane == 0) rank_mask[r] = act;
-        base += (uint32_t)popc64(act);
-        act = ballot(m != 0u);
-    }
-    return base;
-}
-
-// Coverage masks -> dense fragment list with each lane's fragments contiguous, ordered by record index
-// (ascending, or descending when `reverse`): a round then holds few records, each with several pixels, as the
-// adjoint's per-record reduction of a round wants.  Returns the lane's first entry; `total` = all fragments.
-__device__ __forceinline__ uint32_t build_fragment_list_by_pixel(uint16_t* entry, uint32_t mask, bool reverse,
-                                                                 int lane, uint32_t& total) {
-    const uint32_t nm = (uint32_t)__popc(mask);
-    const uint32_t inc = wave_inclusive_scan(nm);
-    total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-    const uint32_t off = inc - nm;
-    uint32_t k = off;
-    uint32_t m = mask;
-    while (m) {
-        const int j = reverse ? 31 - __clz(m) : __ffs(m) - 1;
-        entry[k++] = (uint16_t)((lane << 5) | j);
-        m &= ~(1u << j);
-    }
-    return off;
-}
-
-// A pixel lane's walk over its own fragments of a ranked list, round by round: the position of its next
-// fragment (rank `rank`), while that lies before `end`.
-struct RankCursor {
-    uint32_t rank = 0, base = 0;  // next rank; first position of that rank
-    __device__ __forceinline__ bool next(const uint64_t* rank_mask, uint32_t count, uint64_t lt, uint32_t end,
-                                         uint32_t& e) {
-        if (rank >= count) return false;
-        const uint64_t act = rank_mask[rank];
-        e = base + (uint32_t)popc64(act & lt);
-        if (e >= end) return false;
-        base += (uint32_t)popc64(act);
-        rank++;
-        return true;
-    }
-};
-
 // Ray direction of pixel lane pl from the lanes' own (register) directions; every lane must take part.
 __device__ __forceinline__ float3_ lane_ray(float3_ mine, int pl) {
     return f3(__shfl(mine.x, pl, 64), __shfl(mine.y, pl, 64), __shfl(mine.z, pl, 64));
