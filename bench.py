@@ -238,6 +238,75 @@ def bench_multiview(args, emit=True):
     return out
 
 
+def bench_projection(args, emit=True):
+    """The plain single-view step (fwd + adjoint + Adam, as the headline) with the Projection renderer
+    (projected tetrahedra) instead of the intersection renderer: same mesh, camera and optimiser."""
+    import numpy as np
+    import torch
+
+    import difftetvr_amd as d
+    from difftetvr_amd import meshgen
+    from difftetvr_amd.optim import FusedAdam, mse_loss_grad
+
+    W = H = args.res or 1024
+    fovy = math.pi / 4.0
+    eye = tuple(2.0 * np.array([0.3, 0.4, 1.0]) / np.linalg.norm([0.3, 0.4, 1.0]))
+    view = meshgen.look_at_view_matrix(eye)
+    w = meshgen.grid_workload(args.grid, seed=0)
+
+    def make(pos, grads):
+        m = d.TetMesh()
+        m.set_use_gradients(grads)
+        m.set_tet_mesh_data(w["cells"], pos, w["col"])
+        r = d.Renderer(d.RendererType.PROJECTION)
+        r.set_tet_mesh(m)
+        r.set_attenuation(1.0)
+        r.set_clear_color(d.vec4(0, 0, 0, 0))
+        r.set_viewport_size(W, H)
+        r.set_camera_fovy(fovy)
+        r.set_view_matrix(view)
+        return m, r
+
+    _, r_gt = make(w["pos_gt"], False)
+    target = r_gt.render().detach().clone()
+    mesh, r = make(w["pos"], True)
+    opt = FusedAdam([{"params": [mesh.get_vertex_colors()], "lr": 1e-3, "clamp_nonneg": True},
+                     {"params": [mesh.get_vertex_positions()], "lr": 1e-4}], zero_grad_after=True)
+
+    def step():
+        img = r.render()
+        _, dimg = mse_loss_grad(img, target)
+        r.render_adjoint(dimg, False)
+        opt.step()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1000.0 / args.steps
+    r.set_timing(True)
+    acc = {}
+    for _ in range(3):
+        step()
+        for k, v in r.get_timings().items():
+            acc[k] = acc.get(k, 0.0) + v / 3
+    r.set_timing(False)
+    stats = r.get_stats()
+    out = {"metric": "Projection renderer fwd+adjoint+Adam iters/s, 1.57M tets @1024²", "value": 1000.0 / ms_per_step,
+           "unit": "iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"single-view fwd+adjoint+Adam with the Projection renderer, {args.grid}^3 grid x6 "
+                                  f"tets, {W}x{H}, the headline's camera and optimiser",
+                      "triangles": stats["num_triangles"], "tile_pairs": stats["num_pairs"],
+                      "stage_ms": {k: round(v, 4) for k, v in acc.items()}}}
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
+
+
 def _delaunay_job(args):
     n, seed = args
     from difftetvr_amd import meshgen
@@ -530,9 +599,10 @@ def main():
     ap.add_argument("--res", type=int, default=None, help="viewport edge (default 1024; 512 for multiview)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="uneven workload: skip the autotuner comparison (tile size, segments per pass)")
-    ap.add_argument("--workload", choices=["single", "multiview", "uneven", "serve"], default="single",
+    ap.add_argument("--workload", choices=["single", "multiview", "uneven", "serve", "projection"], default="single",
                     help="single: the plain single-view step (default); multiview: 16 views per batched step; "
-                         "uneven: 8 Delaunay meshes in one batch; serve: Poisson request stream over 32 meshes")
+                         "uneven: 8 Delaunay meshes in one batch; serve: Poisson request stream over 32 meshes; "
+                         "projection: the single-view step with the Projection renderer")
     ap.add_argument("--qps", type=float, default=150.0, help="serve: offered requests per second")
     ap.add_argument("--duration", type=float, default=8.0, help="serve: seconds of request arrivals")
     ap.add_argument("--views", type=int, default=16)
@@ -580,6 +650,9 @@ def main():
         return
     if args.workload == "uneven":
         bench_uneven(args)
+        return
+    if args.workload == "projection":
+        bench_projection(args)
         return
 
     # the serving mix's Delaunay meshes are built in child processes while the GPU legs run (started before
@@ -738,6 +811,12 @@ def main():
                          "requests": o["config"]["requests"], "max_batch_seen": o["config"]["max_batch_seen"]}
                         for o in sv]
         out["serve_config"] = sv[0]["config"]["workload"]
+        torch.cuda.empty_cache()
+        pr_args = argparse.Namespace(**vars(args))
+        pr_args.steps, pr_args.warmup = args.extra_steps, 2
+        pr = bench_projection(pr_args, emit=False)
+        out["projection"] = {"iters_per_s": pr["value"], "ms_per_step": pr["ms_per_step"], "steps": args.extra_steps,
+                             "stage_ms": pr["config"]["stage_ms"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, view, fovy, W, H, args.cpu_row_step)
     print(json.dumps(out), flush=True)
