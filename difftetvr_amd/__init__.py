@@ -779,11 +779,12 @@ class BatchRenderer(Renderer):
 
     ``set_items([(tet_mesh, view_matrix, width, height), ...])`` or ``set_views(tet_mesh, view_matrices, width,
     height)``; ``render()`` returns [N, H, W, 4] when all items share a viewport, else a list of [H_i, W_i, 4];
-    ``render_adjoint(image_adjoints)`` accumulates into every item mesh's ``.grad``.
+    ``render_adjoint(image_adjoints)`` accumulates into every item mesh's ``.grad``.  ``renderer_type``
+    (INTERSECTION by default) applies to every item of the batch.
     """
 
-    def __init__(self):
-        super().__init__(renderer_type=RendererType.INTERSECTION)
+    def __init__(self, renderer_type=RendererType.INTERSECTION):
+        super().__init__(renderer_type=renderer_type)
         self._items = []
         self._rendered_items = None
         self._uniform_size = True

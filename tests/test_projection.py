@@ -10,8 +10,8 @@ Oracle (CPU):
     triangles sum to the chord through the cube, so every pixel is c (1 - exp(-s * chord)) (coverage, order and
     compositing together; a missing or doubled tet would show).
 GPU (marked gpu): fragment counts bit-identical and images within 1e-5 of the FP32 oracle (per-vertex and
-per-cell colours), and the adjoint against FP64 central differences of the oracle's forward with its
-fragment lists and decisions frozen.
+per-cell colours), the adjoint against FP64 central differences of the oracle's forward with its fragment
+lists and decisions frozen, and batches of items against their single renders.
 """
 import math
 
@@ -281,6 +281,77 @@ def test_gpu_projection_adjoint_central_differences(gpu, per_cell):
         an = float(np.sum(gp * dp) + np.sum(gc * dc))
         print(f"  {name} ({len(tets)} tets): directional {an:.6e} vs differences {fd:.6e}", flush=True)
         assert abs(an - fd) <= 1e-3 * abs(fd) + 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("per_cell", [False, True])
+def test_gpu_projection_batch(gpu, per_cell):
+    """A batch of Projection items (two meshes, one seen twice) gives each item's single render bit for bit, and
+    its adjoint the sum of the single adjoints (FP32 atomics reorder the sums: relative L2 <= 1e-5)."""
+    import torch
+    import difftetvr_amd as d
+    rng = np.random.default_rng(11)
+    wa, wb = meshgen.grid_workload(6, seed=1), meshgen.grid_workload(5, seed=2)
+
+    def mesh_of(w):
+        m = d.TetMesh()
+        m.set_use_gradients(True)
+        if per_cell:
+            T = w["cells"].shape[0]
+            cc = np.concatenate([rng.uniform(0, 1, (T, 3)), rng.uniform(0.5, 4, (T, 1))], axis=1).astype(np.float32)
+            m.set_tet_mesh_data(w["cells"], w["pos"], None, cc)
+        else:
+            m.set_tet_mesh_data(w["cells"], w["pos"], w["col"])
+        return m
+
+    ma, mb = mesh_of(wa), mesh_of(wb)
+    eyes = [(0.9, 0.6, 1.5), (-1.2, 0.5, 1.3), (0.4, -1.4, -1.1)]
+    W = H = 48
+    items = [(ma, view_from(eyes[0]), W, H), (mb, view_from(eyes[1]), W, H), (ma, view_from(eyes[2]), W, H)]
+    br = d.BatchRenderer(renderer_type=d.RendererType.PROJECTION)
+    br.set_attenuation(1.0)
+    br.set_clear_color(d.vec4(0, 0, 0, 0))
+    br.set_camera_fovy(FOVY)
+    br.set_items(items)
+    imgs = br.render()
+    cnts = br.get_fragment_counts()
+    dimgs = torch.randn((3, H, W, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(2))
+    br.render_adjoint(dimgs)
+    torch.cuda.synchronize()
+    # the same meshes' gradients from single renders, into copies
+    sa = d.TetMesh()
+    sb = d.TetMesh()
+    for src, dst in ((ma, sa), (mb, sb)):
+        dst.set_use_gradients(True)
+        cells = src.get_cell_indices().cpu().numpy()
+        pos = src.get_vertex_positions().detach().cpu().numpy()
+        if per_cell:
+            dst.set_tet_mesh_data(cells, pos, None, src.get_cell_colors().detach().cpu().numpy())
+        else:
+            dst.set_tet_mesh_data(cells, pos, src.get_vertex_colors().detach().cpu().numpy())
+    for i, (m, vm, w_, h_) in enumerate(items):
+        r = d.Renderer(renderer_type=d.RendererType.PROJECTION)
+        r.set_tet_mesh(sa if m is ma else sb)
+        r.set_attenuation(1.0)
+        r.set_clear_color(d.vec4(0, 0, 0, 0))
+        r.set_camera_fovy(FOVY)
+        r.set_viewport_size(w_, h_)
+        r.set_view_matrix(vm)
+        ref = r.render()
+        refc = r.get_fragment_counts()
+        r.render_adjoint(dimgs[i].contiguous())
+        torch.cuda.synchronize()
+        assert torch.equal(imgs[i], ref)
+        assert torch.equal(cnts[i], refc)
+        assert int(refc.sum()) > 0
+    for m, s_ in ((ma, sa), (mb, sb)):
+        gp, gp1 = m.get_vertex_positions().grad.cpu().numpy(), s_.get_vertex_positions().grad.cpu().numpy()
+        col = (lambda x: x.get_cell_colors()) if per_cell else (lambda x: x.get_vertex_colors())
+        gc, gc1 = col(m).grad.cpu().numpy(), col(s_).grad.cpu().numpy()
+        print(f"projection batch ({'per-cell' if per_cell else 'per-vertex'}): positions {rel_l2(gp, gp1):.2e}, "
+              f"colours {rel_l2(gc, gc1):.2e}", flush=True)
+        assert np.linalg.norm(gp1) > 0 and np.linalg.norm(gc1) > 0
+        assert rel_l2(gp, gp1) <= 1e-5 and rel_l2(gc, gc1) <= 1e-5
 
 
 @pytest.mark.gpu
