@@ -56,6 +56,26 @@ __device__ __forceinline__ int pt_corner(const PtDec& D, int k, int j) {
     return i == 0 ? a1 : (i == 1 || i == 4) ? a0 : i == 2 ? D.s20 : D.s21;
 }
 
+// Row i (0-3) of a small register array by selects: a run-time index would place the array in scratch memory.
+// The rows pass through an empty asm statement that keeps them in registers, as the compiler otherwise turns
+// the select chain back into an indexed load from a stack copy.
+template <int R, int N>
+__device__ __forceinline__ void pick_row(const float (&a)[R][N], int i, float (&out)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            v[q] = a[q][k];
+            asm volatile("" : "+v"(v[q]));
+        }
+        float r = v[0];
+#pragma unroll
+        for (int q = 1; q < 4; q++) r = i == q ? v[q] : r;
+        out[k] = r;
+    }
+}
+
 // Float decisions (GenerateTrianglesVTK.glsl:140-273).  Beyond the reference: a tet with a vertex behind the
 // near plane or beyond the guard band is culled (its triangles would be undefined there).
 __device__ __forceinline__ PtDec pt_decide(const ViewUniforms& U, const float P[4][3]) {
@@ -100,9 +120,13 @@ __device__ __forceinline__ PtDec pt_decide(const ViewUniforms& U, const float P[
     } else {
         D.s10 = 0; D.s11 = 1; D.s20 = 2; D.s21 = 3;
     }
-    const float Ax = ndc[D.s11][0] - ndc[D.s10][0], Ay = ndc[D.s11][1] - ndc[D.s10][1];
-    const float Bx = ndc[D.s21][0] - ndc[D.s20][0], By = ndc[D.s21][1] - ndc[D.s20][1];
-    const float Cx = ndc[D.s20][0] - ndc[D.s10][0], Cy = ndc[D.s20][1] - ndc[D.s10][1];
+    float n1[3], n2[3], n3[3];  // segment 1 is (0, s11), segment 2 (s20, s21)
+    pick_row(ndc, D.s11, n1);
+    pick_row(ndc, D.s20, n2);
+    pick_row(ndc, D.s21, n3);
+    const float Ax = n1[0] - ndc[0][0], Ay = n1[1] - ndc[0][1];
+    const float Bx = n3[0] - n2[0], By = n3[1] - n2[1];
+    const float Cx = n2[0] - ndc[0][0], Cy = n2[1] - ndc[0][1];
     const float den = Ax * By - Ay * Bx;
     if (fabsf(den) < 1e-8f) return D;  // degenerate
     const float alpha = (By * Cx - Bx * Cy) / den;
@@ -155,50 +179,77 @@ __device__ __forceinline__ void pt_vertices(const PtDec& D, const ViewUniforms& 
 #pragma unroll
     for (int k = 0; k < 4; k++) V.col[4][k] = 0.f;
     V.dep[4] = 0.f;
-    int a0 = D.s10, a1 = D.s11;
-    const int b0 = D.s20, b1 = D.s21;
+    // the two edges' end points (segment 1: a0 -> a1, segment 2: b0 -> b1)
+    float na0[3], na1[3], nb0[3], nb1[3], ca0[4], ca1[4], cb0[4], cb1[4];
+    {
+        float nd[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) nd[i][k] = V.ndc[i][k];
+        pick_row(nd, D.s10, na0);
+        pick_row(nd, D.s11, na1);
+        pick_row(nd, D.s20, nb0);
+        pick_row(nd, D.s21, nb1);
+        float cc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) cc[i][k] = C[i][k];
+        pick_row(cc, D.s10, ca0);
+        pick_row(cc, D.s11, ca1);
+        pick_row(cc, D.s20, cb0);
+        pick_row(cc, D.s21, cb1);
+    }
     float A[3], B[3], Cv[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        A[k] = V.ndc[a1][k] - V.ndc[a0][k];
-        B[k] = V.ndc[b1][k] - V.ndc[b0][k];
-        Cv[k] = V.ndc[b0][k] - V.ndc[a0][k];
+        A[k] = na1[k] - na0[k];
+        B[k] = nb1[k] - nb0[k];
+        Cv[k] = nb0[k] - na0[k];
     }
     const float den = A[0] * B[1] - A[1] * B[0];
     float alpha = (B[1] * Cv[0] - B[0] * Cv[1]) / den;
     const float beta = (A[1] * Cv[0] - A[0] * Cv[1]) / den;
     if (D.cls2) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) V.ndc[4][k] = V.ndc[a0][k] + alpha * A[k];
-        const float depth =
-            pt_corrected_depth(U.invProj, V.ndc[4][0], V.ndc[4][1], V.ndc[4][2], V.ndc[b0][2] + beta * B[2]);
+        for (int k = 0; k < 3; k++) V.ndc[4][k] = na0[k] + alpha * A[k];
+        const float depth = pt_corrected_depth(U.invProj, V.ndc[4][0], V.ndc[4][1], V.ndc[4][2], nb0[2] + beta * B[2]);
 #pragma unroll
         for (int k = 0; k < 4; k++)
             V.col[4][k] = per_cell ? C[0][k]
-                                   : 0.5f * (((C[a0][k] + alpha * (C[a1][k] - C[a0][k])) + C[b0][k]) +
-                                             beta * (C[b1][k] - C[b0][k]));
+                                   : 0.5f * (((ca0[k] + alpha * (ca1[k] - ca0[k])) + cb0[k]) +
+                                             beta * (cb1[k] - cb0[k]));
         V.dep[4] = depth * U.attenuation;
     } else {
-        if (D.flip) {  // segment 1 reversed: the thick point is its second vertex
-            const int t = a0;
-            a0 = a1;
-            a1 = t;
-            alpha = 1.0f - alpha;
+        // the thick vertex T is segment 1's second end (its first when the segment is reversed), P the other
+        const bool fl = D.flip;
+        const int t = fl ? D.s10 : D.s11;
+        if (fl) alpha = 1.0f - alpha;
+        float nT[3], nP[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            nT[k] = fl ? na0[k] : na1[k];
+            nP[k] = fl ? na1[k] : na0[k];
         }
-        const float edgez = V.ndc[b0][2] + beta * B[2];
-        const float pointz = V.ndc[a0][2];
+        const float edgez = nb0[2] + beta * B[2];
+        const float pointz = nP[2];
         const float facez = (edgez + (alpha - 1.0f) * pointz) / alpha;
-        const float depth = pt_corrected_depth(U.invProj, V.ndc[a1][0], V.ndc[a1][1], V.ndc[a1][2], facez);
+        const float depth = pt_corrected_depth(U.invProj, nT[0], nT[1], nT[2], facez);
+        const float dep = depth * U.attenuation;
+#pragma unroll
+        for (int v = 0; v < 4; v++) V.dep[v] = v == t ? dep : V.dep[v];
         if (!per_cell) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const float edgec = C[b0][k] + beta * (C[b1][k] - C[b0][k]);
-                const float pointc = C[a0][k];
+                const float edgec = cb0[k] + beta * (cb1[k] - cb0[k]);
+                const float pointc = fl ? ca1[k] : ca0[k];
                 const float facec = (edgec + (alpha - 1.0f) * pointc) / alpha;
-                V.col[a1][k] = 0.5f * (facec + C[a1][k]);
+                const float col = 0.5f * (facec + (fl ? ca0[k] : ca1[k]));
+#pragma unroll
+                for (int v = 0; v < 4; v++) V.col[v][k] = v == t ? col : V.col[v][k];
             }
         }
-        V.dep[a1] = depth * U.attenuation;
     }
 }
 
@@ -227,14 +278,37 @@ __device__ __forceinline__ void pt_load_tet(const BatchItem& B, int64_t tet, flo
     }
 }
 
-// Fixed-point screen triangle (oriented to positive area) and centroid key of triangle k.
-__device__ __forceinline__ void pt_screen(const ViewUniforms& U, const float4* __restrict__ pv, const PtDec& D, int k,
-                                          ScreenTri& S, uint32_t& key) {
+// Corners of triangle k from the tet's five vertices held in registers (selects: a run-time index would place
+// the array in scratch memory).
+__device__ __forceinline__ void pt_tri_corners(const float4 (&pv)[5], const PtDec& D, int k, float4 (&q)[3]) {
+    float x[5], y[5], z[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        x[i] = pv[i].x;
+        y[i] = pv[i].y;
+        z[i] = pv[i].z;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int v = pt_corner(D, k, j);
+        float rx = x[0], ry = y[0], rz = z[0];
+#pragma unroll
+        for (int i = 1; i < 5; i++) {
+            rx = v == i ? x[i] : rx;
+            ry = v == i ? y[i] : ry;
+            rz = v == i ? z[i] : rz;
+        }
+        q[j] = make_float4(rx, ry, rz, 0.f);  // pt_screen reads x, y, z
+    }
+}
+
+// Fixed-point screen triangle (oriented to positive area) and centroid key of the triangle with corners q.
+__device__ __forceinline__ void pt_screen(const ViewUniforms& U, const float4 (&qc)[3], ScreenTri& S, uint32_t& key) {
     const float hw = 0.5f * (float)U.width, hh = 0.5f * (float)U.height;
     float z[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const float4 q = pv[pt_corner(D, k, j)];
+        const float4 q = qc[j];
         const float xf = q.x * hw + hw;
         const float yf = q.y * hh + hh;
         S.X[j] = (int32_t)rintf(xf * 256.0f);
@@ -283,7 +357,9 @@ __global__ __launch_bounds__(256) void k_pt_count(const BatchItem* __restrict__ 
             for (int k = 0; k < D.ntri; k++) {
                 ScreenTri S;
                 uint32_t key;
-                pt_screen(U, pv, D, k, S, key);
+                float4 q[3];
+                pt_tri_corners(pv, D, k, q);
+                pt_screen(U, q, S, key);
                 const uint32_t cf = face_record_count(screen_bins(U, S));
                 c += cf;
                 ntri++;
@@ -333,7 +409,9 @@ __global__ __launch_bounds__(256) void k_pt_emit(const BatchItem* __restrict__ i
     for (int k = 0; k < D.ntri && left > 0; k++) {
         ScreenTri S;
         uint32_t key;
-        pt_screen(U, pv, D, k, S, key);
+        float4 q[3];
+        pt_tri_corners(pv, D, k, q);
+        pt_screen(U, q, S, key);
         const FaceBins b = screen_bins(U, S);
         const bool test = test_tiles(b);
         const uint32_t id = (uint32_t)(tet * 4 + k);
@@ -370,7 +448,10 @@ __global__ __launch_bounds__(256) void k_pt_gather(const BatchItem* __restrict__
     const PtDec D = pt_unpack(pt_info[g]);
     ScreenTri S;
     uint32_t key;
-    pt_screen(U, pt_vert + 5 * g, D, (int)(id & 3), S, key);
+    float4 q[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) q[j] = pt_vert[5 * g + pt_corner(D, (int)(id & 3), j)];
+    pt_screen(U, q, S, key);
     const uint64_t m = tile_coverage(S, tile % U.tiles_x, tile / U.tiles_x, U.width, U.height);
     const int k = (int)(id & 3);
     const uint32_t slots = (uint32_t)pt_corner(D, k, 0) | ((uint32_t)pt_corner(D, k, 1) << 3) |
