@@ -212,7 +212,11 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
         key[i] = valid ? kin[idx] : ~0ull;
         val[i] = valid ? vin[idx] : 0u;
     }
-    // Stable rank inside the wave: items are visited in input order (slot i, then lane).
+    // Stable rank inside the wave: items are visited in input order (slot i, then lane).  For each slot the
+    // lowest lane of every digit adds the slot's count of that digit to the wave's counter and gets the count
+    // before it back; the LDS runs one wave's atomics in issue order, so the slots need not wait on each other.
+    // The other lanes then take their leader's base by a lane permute.
+    uint32_t info[kSortItems];  // rank among the slot's lanes of the same digit | leader lane << 8
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
         int64_t idx = wbase + i * 64 + lane;
@@ -225,14 +229,17 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep_pass(
             uint64_t bb = ballot(bit);
             peers &= bit ? bb : ~bb;
         }
-        uint32_t below = (uint32_t)popc64(peers & lt);
-        uint32_t cnt = (uint32_t)popc64(peers);
-        uint32_t basecnt = valid ? s_wcnt[w][d] : 0u;
-        rank[i] = basecnt + below;
-        __builtin_amdgcn_wave_barrier();
-        if (valid && below == 0) s_wcnt[w][d] = basecnt + cnt;
-        __builtin_amdgcn_wave_barrier();
+        const uint32_t below = (uint32_t)popc64(peers & lt);
+        const uint32_t cnt = (uint32_t)popc64(peers);
+        const int leader = peers != 0ull ? __ffsll((unsigned long long)peers) - 1 : lane;
+        uint32_t base = 0u;
+        if (valid && below == 0) base = atomicAdd(&s_wcnt[w][d], cnt);
+        rank[i] = base;
+        info[i] = below | ((uint32_t)leader << 8);
     }
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++)
+        rank[i] = (uint32_t)__shfl((int)rank[i], (int)(info[i] >> 8), 64) + (info[i] & 255u);
     __syncthreads();
     // Per digit (one thread each): wave offsets, block total, look-back for the global base.
     const uint32_t d = threadIdx.x;
