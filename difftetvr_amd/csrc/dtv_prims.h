@@ -50,16 +50,6 @@ __device__ __forceinline__ float wave_sum(float v) {
     return (r0 + r1) + (r2 + r3);
 }
 
-// Reduce-scatter of N <= 32 per-lane values across the wave: lane L returns the wave-wide sum of value
-// wave_scatter_index(L) (meaningful when that index is below N; lanes 2i and 2i + 1 return the same sum).
-// Each level pairs the values and halves them across one lane bit: bit 5 and bit 4 by the permlane32 and
-// permlane16 swaps, bits 3 and 2 by the DPP row mirror and half-row mirror, bit 1 by a quad permutation; a
-// last quad permutation adds lanes 2i and 2i + 1.  About 2.5 VALU instructions per value, where a full wave
-// sum per value costs about 12.  All lanes must be active.
-__device__ __forceinline__ int wave_scatter_index(int lane) {
-    return ((lane >> 5) & 1) | ((lane >> 3) & 2) | ((lane >> 1) & 4) | ((lane << 1) & 8) | ((lane << 3) & 16);
-}
-
 namespace detail {
 // One DPP level: pairs (x, y) of n values; lanes whose BIT is clear keep x, the others keep y, each adding the
 // value the partner lane sends (the partner's copy of the kept value).
@@ -75,35 +65,11 @@ __device__ __forceinline__ void dpp_halve(const float (&in)[N], float (&out)[(N 
 }
 }  // namespace detail
 
-template <int N>
-__device__ __forceinline__ float wave_reduce_scatter(const float (&v)[N]) {
-    static_assert(N >= 1 && N <= 32, "wave_reduce_scatter: 1 to 32 values");
-    constexpr int N1 = (N + 1) / 2, N2 = (N1 + 1) / 2, N3 = (N2 + 1) / 2, N4 = (N3 + 1) / 2;
-    const int lane = lane_id();
-    float a[N1], b[N2];
-#pragma unroll
-    for (int k = 0; k < N1; k++) {  // lanes 0-31 keep value 2k, lanes 32-63 value 2k + 1
-        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[2 * k]),
-                                                        __float_as_uint(v[2 * k + 1 < N ? 2 * k + 1 : 2 * k]),
-                                                        false, false);
-        a[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-    }
-#pragma unroll
-    for (int k = 0; k < N2; k++) {  // even rows of 16 lanes keep value 2k, odd rows 2k + 1
-        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a[2 * k]),
-                                                        __float_as_uint(a[2 * k + 1 < N1 ? 2 * k + 1 : 2 * k]),
-                                                        false, false);
-        b[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-    }
-    float c[N3], d[N4], e[1];
-    detail::dpp_halve<0x140, 3>(b, c, lane);  // row_mirror: lane i <-> 15 - i
-    detail::dpp_halve<0x141, 2>(c, d, lane);  // row_half_mirror: lane i <-> 7 - i
-    detail::dpp_halve<0x4E, 1>(d, e, lane);   // quad_perm [2,3,0,1]: lane i <-> i ^ 2
-    return e[0] + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(e[0]), 0xB1, 0xF, 0xF, true));
-}
-
-// The same within each row of 16 lanes (each row reduces its own values): lane L returns in out[k] the row's sum
-// of value 16 k + row_scatter_index(L) (meaningful when below N).  Levels over row-lane bits 3, 2, 1 and 0.
+// Reduce-scatter of N <= 32 per-lane values within each row of 16 lanes (each row reduces its own values):
+// lane L returns in out[k] the row's sum of value 16 k + row_scatter_index(L) (meaningful when below N).  Each
+// level pairs the values and halves them across one row-lane bit (3 and 2 by the DPP row mirror and half-row
+// mirror, 1 and 0 by quad permutations): about 3 VALU instructions per pair and level, where a full row sum of
+// every value costs about 8 each.  All lanes must be active.
 __device__ __forceinline__ int row_scatter_index(int lane) {
     return ((lane >> 3) & 1) | ((lane >> 1) & 2) | ((lane << 1) & 4) | ((lane << 3) & 8);
 }

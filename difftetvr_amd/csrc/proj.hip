@@ -11,9 +11,9 @@
 //
 // The adjoint walks each pixel back to front as the tet renderer's does (transmittance checkpoints and
 // division).  It sums each record's gradients with respect to its triangle's corner NDC xy, colours and
-// thicknesses (one wave reduce-scatter per record, then one atomic per value into the triangle's slot).  A
-// per-tet pass carries them to the vertex positions and colours through the triangle construction, by
-// forward-mode derivatives of the same arithmetic.
+// thicknesses (reduce-scatters within rows of 16 lanes, summed per batch in LDS, then one atomic per value into
+// the triangle's slot).  A per-tet pass carries them back through the triangle construction to the vertex
+// positions and colours.
 #include "raster.h"
 #include "render_common.h"
 
