@@ -302,7 +302,8 @@ __device__ __forceinline__ void pt_tri_corners(const float4 (&pv)[5], const PtDe
     }
 }
 
-// Fixed-point screen triangle (oriented to positive area) and centroid key of the triangle with corners q.
+// Fixed-point screen triangle (oriented to positive area) and centroid key of the triangle with corners qc
+// (NDC x, y, z).
 __device__ __forceinline__ void pt_screen(const ViewUniforms& U, const float4 (&qc)[3], ScreenTri& S, uint32_t& key) {
     const float hw = 0.5f * (float)U.width, hh = 0.5f * (float)U.height;
     float z[3];
