@@ -192,5 +192,6 @@ def test_mse_loss_grad_views_matches_torch(gpu):
     ref = sum(torch.nn.functional.mse_loss(imgs[i], tgts[i]) for i in range(5))
     ref.backward()
     torch.cuda.synchronize()
-    assert abs(float(loss) - float(ref)) <= 1e-5 * float(ref)
+    r = float(ref.detach())
+    assert abs(float(loss) - r) <= 1e-5 * r
     assert torch.allclose(dimg, imgs.grad, rtol=1e-6, atol=1e-9)
