@@ -11,7 +11,8 @@ Oracle (CPU):
     compositing together; a missing or doubled tet would show).
 GPU (marked gpu): fragment counts bit-identical and images within 1e-5 of the FP32 oracle (per-vertex and
 per-cell colours), the adjoint against FP64 central differences of the oracle's forward with its fragment
-lists and decisions frozen, and batches of items against their single renders.
+lists and decisions frozen, batches of items against their single renders, and at the benchmark's size
+(1.57M tets, 1024^2) the colour and alpha gradients through two scaling identities.
 """
 import math
 
@@ -352,6 +353,40 @@ def test_gpu_projection_batch(gpu, per_cell):
               f"colours {rel_l2(gc, gc1):.2e}", flush=True)
         assert np.linalg.norm(gp1) > 0 and np.linalg.norm(gc1) > 0
         assert rel_l2(gp, gp1) <= 1e-5 and rel_l2(gc, gc1) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_projection_fullsize_scaling_identities(gpu):
+    """The benchmark's size (64^3 x 6 tets, 1024 x 1024, its camera), through two identities the construction
+    fixes: every corner colour (the crossing's and the thick vertex's too) is linear in the vertex colours and
+    alpha depends on them only through c.a times the thickness, which is proportional to the attenuation. So
+    for L = sum dI . I (clear colour 0) the RGB gradients satisfy sum_v c_rgb(v) . dL/dc_rgb(v) = sum dI_rgb .
+    I_rgb (degree-one homogeneity), and the alpha gradients sum_v c_a(v) dL/dc_a(v) = att dL/d(att), taken by
+    central differences of two more renders."""
+    import torch
+    import difftetvr_amd as d
+    w = meshgen.grid_workload(64, seed=0)
+    W = H = 1024
+    eye = tuple(2.0 * np.array([0.3, 0.4, 1.0]) / np.linalg.norm([0.3, 0.4, 1.0]))
+    mesh, r = _renderer(d, w["cells"], w["pos"], w["col"], None, W, H, eye, att=1.0)
+    img = r.render().clone()
+    dimg = torch.rand((H, W, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(4)) + 0.5
+    r.render_adjoint(dimg)
+    torch.cuda.synchronize()
+    g = mesh.get_vertex_colors().grad.cpu().numpy().astype(np.float64)
+    c = w["col"].astype(np.float64)
+    im, di = img.cpu().numpy().astype(np.float64), dimg.cpu().numpy().astype(np.float64)
+    lhs, rhs = float(np.sum(c[:, :3] * g[:, :3])), float(np.sum(di[..., :3] * im[..., :3]))
+    eps = 1e-3
+    losses = []
+    for att in (1.0 + eps, 1.0 - eps):
+        r.set_attenuation(att)
+        losses.append(float(torch.sum(dimg.double() * r.render().double())))
+    fd = (losses[0] - losses[1]) / (2 * eps)
+    lhs_a = float(np.sum(c[:, 3] * g[:, 3]))
+    print(f"projection 1024^2: RGB {lhs:.9e} vs {rhs:.9e}; alpha {lhs_a:.6e} vs differences {fd:.6e}", flush=True)
+    assert rhs > 1.0 and abs(lhs - rhs) <= 1e-5 * rhs
+    assert abs(fd) > 1.0 and abs(lhs_a - fd) <= 1e-3 * abs(fd)
 
 
 @pytest.mark.gpu
