@@ -9,7 +9,9 @@ With constant colour c and density s (attenuation 1, no early termination, clear
   * adding d to every density multiplies each pixel's transmittance by exp(-d * chord), so the density gradient
     summed over all vertices equals sum over pixels of (dL/dI_a + dL/dI_rgb . c) * (1 - a) * chord, with
     chord = -ln(1 - a) / s read back from the image itself.
-The gradient identities exercise every fragment of the exact adjoint (76M ray-tet segments) at once."""
+The gradient identities exercise every fragment of the exact adjoint (76M ray-tet segments) at once.  On the
+benchmark's own field (seeded colours and densities): sampled rows' fragment lists, images and adjoint against
+the oracle, and every pixel's colour gradients through RGB homogeneity and the density-attenuation scaling."""
 import math
 
 import numpy as np
@@ -179,3 +181,36 @@ def test_fullsize_adjoint_rows(fullsize_blob):
     _, ud = oracle.camera_uniforms(view_from(EYE), FOVY, ZNEAR, ZFAR, W, H)
     check_gradients(f"full size adjoint over {len(ROWS)} rows", gp, gc,
                     adjoint_references(oracle, fr, w["pos"], w["col"], uf, ud, 1.0, dimg))
+
+
+def test_fullsize_blob_scaling_identities(fullsize_blob):
+    """Every pixel's gradients at once, through two identities of the exact forward: the RGB of each sub-segment
+    is linear in the interpolated vertex RGB and its opacity depends on the density only through density x
+    length x attenuation. So for L = sum dI . I (clear colour 0, no early termination) sum_v c_rgb(v) .
+    dL/dc_rgb(v) = sum dI_rgb . I_rgb, and sum_v c_a(v) dL/dc_a(v) = att dL/d(att), taken by central differences
+    of two more renders."""
+    w, mesh, r, img, uf, fr = fullsize_blob
+    dimg = torch.rand((H, W, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(5)) + 0.5
+    mesh.get_vertex_colors().grad.zero_()
+    mesh.get_vertex_positions().grad.zero_()
+    r.render()
+    r.render_adjoint(dimg, False)
+    torch.cuda.synchronize()
+    g = mesh.get_vertex_colors().grad.cpu().numpy().astype(np.float64)
+    c = w["col"].astype(np.float64)
+    im, di = img.cpu().numpy().astype(np.float64), dimg.cpu().numpy().astype(np.float64)
+    lhs, rhs = float(np.sum(c[:, :3] * g[:, :3])), float(np.sum(di[..., :3] * im[..., :3]))
+    eps = 1e-3
+    losses = []
+    try:
+        for att in (1.0 + eps, 1.0 - eps):
+            r.set_attenuation(att)
+            losses.append(float(torch.sum(dimg.double() * r.render().double())))
+    finally:
+        r.set_attenuation(1.0)
+    fd = (losses[0] - losses[1]) / (2 * eps)
+    lhs_a = float(np.sum(c[:, 3] * g[:, 3]))
+    print(f"full size blob field: RGB {lhs:.9e} vs {rhs:.9e}; alpha {lhs_a:.6e} vs differences {fd:.6e}",
+          flush=True)
+    assert rhs > 1.0 and abs(lhs - rhs) <= 1e-5 * rhs
+    assert abs(fd) > 1.0 and abs(lhs_a - fd) <= 1e-3 * abs(fd)
