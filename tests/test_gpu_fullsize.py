@@ -11,7 +11,8 @@ With constant colour c and density s (attenuation 1, no early termination, clear
     chord = -ln(1 - a) / s read back from the image itself.
 The gradient identities exercise every fragment of the exact adjoint (76M ray-tet segments) at once.  On the
 benchmark's own field (seeded colours and densities): sampled rows' fragment lists, images and adjoint against
-the oracle, and every pixel's colour gradients through RGB homogeneity and the density-attenuation scaling."""
+the oracle, and every pixel's colour and position gradients through RGB homogeneity, the density-attenuation
+scaling and the scaling of the mesh about the eye."""
 import math
 
 import numpy as np
@@ -210,7 +211,13 @@ def test_fullsize_blob_scaling_identities(fullsize_blob):
         r.set_attenuation(1.0)
     fd = (losses[0] - losses[1]) / (2 * eps)
     lhs_a = float(np.sum(c[:, 3] * g[:, 3]))
-    print(f"full size blob field: RGB {lhs:.9e} vs {rhs:.9e}; alpha {lhs_a:.6e} vs differences {fd:.6e}",
-          flush=True)
+    # scaling the mesh about the eye keeps every ray's faces and barycentrics and scales every segment length:
+    # sum_v (p_v - eye) . dL/dp_v = att dL/d(att) as well
+    gp = mesh.get_vertex_positions().grad.cpu().numpy().astype(np.float64)
+    tp = (w["pos"].astype(np.float64) - np.asarray(EYE, np.float64)) * gp
+    lhs_p = float(np.sum(tp))
+    print(f"full size blob field: RGB {lhs:.9e} vs {rhs:.9e}; alpha {lhs_a:.6e}, positions {lhs_p:.6e} (terms "
+          f"{float(np.sum(np.abs(tp))):.3e}) vs differences {fd:.6e}", flush=True)
     assert rhs > 1.0 and abs(lhs - rhs) <= 1e-5 * rhs
     assert abs(fd) > 1.0 and abs(lhs_a - fd) <= 1e-3 * abs(fd)
+    assert abs(lhs_p - fd) <= 1e-3 * abs(fd) + 1e-5 * float(np.sum(np.abs(tp)))

@@ -12,7 +12,7 @@ Oracle (CPU):
 GPU (marked gpu): fragment counts bit-identical and images within 1e-5 of the FP32 oracle (per-vertex and
 per-cell colours), the adjoint against FP64 central differences of the oracle's forward with its fragment
 lists and decisions frozen, batches of items against their single renders, and at the benchmark's size
-(1.57M tets, 1024^2) the colour and alpha gradients through two scaling identities.
+(1.57M tets, 1024^2) the colour, alpha and position gradients through three scaling identities.
 """
 import math
 
@@ -384,9 +384,16 @@ def test_gpu_projection_fullsize_scaling_identities(gpu):
         losses.append(float(torch.sum(dimg.double() * r.render().double())))
     fd = (losses[0] - losses[1]) / (2 * eps)
     lhs_a = float(np.sum(c[:, 3] * g[:, 3]))
-    print(f"projection 1024^2: RGB {lhs:.9e} vs {rhs:.9e}; alpha {lhs_a:.6e} vs differences {fd:.6e}", flush=True)
+    # scaling the mesh about the eye keeps every vertex's screen position and every triangle, and scales each
+    # thickness (the distance between two points on one eye ray): sum_v (p_v - eye) . dL/dp_v = att dL/d(att)
+    gp = mesh.get_vertex_positions().grad.cpu().numpy().astype(np.float64)
+    tp = (w["pos"].astype(np.float64) - np.asarray(eye, np.float64)) * gp
+    lhs_p = float(np.sum(tp))
+    print(f"projection 1024^2: RGB {lhs:.9e} vs {rhs:.9e}; alpha {lhs_a:.6e}, positions {lhs_p:.6e} (terms "
+          f"{float(np.sum(np.abs(tp))):.3e}) vs differences {fd:.6e}", flush=True)
     assert rhs > 1.0 and abs(lhs - rhs) <= 1e-5 * rhs
     assert abs(fd) > 1.0 and abs(lhs_a - fd) <= 1e-3 * abs(fd)
+    assert abs(lhs_p - fd) <= 1e-3 * abs(fd) + 1e-5 * float(np.sum(np.abs(tp)))
 
 
 @pytest.mark.gpu
